@@ -1,0 +1,37 @@
+"""The C-ABI library builds, loads and exports every symbol include/voronoids.h declares
+(no GPU calls)."""
+import ctypes
+import os
+import re
+
+from voronoids_amd import _lib
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_symbols():
+    with open(os.path.join(ROOT, "include", "voronoids.h")) as f:
+        text = f.read()
+    return sorted(set(re.findall(r"\b(vo_[a-z_]+)\s*\(", text)))
+
+
+def test_header_and_binding_agree():
+    assert declared_symbols() == sorted(_lib.EXPORTED)
+
+
+def test_library_exports_every_declared_symbol():
+    L = _lib.lib()
+    for name in declared_symbols():
+        assert hasattr(L, name), name
+        assert ctypes.cast(getattr(L, name), ctypes.c_void_p).value
+
+
+def test_version_and_error_without_gpu():
+    L = _lib.lib()
+    assert L.vo_version() == 1
+    assert isinstance(L.vo_last_error(), bytes)
+    out = ctypes.c_void_p()
+    # invalid dimension is rejected before any device call
+    rc = L.vo_tri_create(0, 4, None, 0, ctypes.byref(out))
+    assert rc == _lib.VO_ERR_INVALID and not out.value
+    assert b"dim" in L.vo_last_error()

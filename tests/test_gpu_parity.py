@@ -1,0 +1,149 @@
+"""GPU construction (libvoronoids_hip.so through the C ABI) against the oracle, the golden
+fixtures and the size-independent invariants of a Delaunay triangulation."""
+import json
+import os
+
+import numpy as np
+import pytest
+
+import voronoids_amd as vo
+from oracle import oracle as O
+from voronoids_amd import datasets
+
+G = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+pytestmark = pytest.mark.gpu
+
+
+def canon_sha(T, E):
+    return O.Canonical(T, E).sha256
+
+
+def test_max_simplex_id_after_new():
+    # tests/test_delaunay_tree.rs:20 and :44
+    assert vo.DelaunayTree(datasets.uniform(1000, 3, seed=0)).max_simplex_id == 4
+    assert vo.DelaunayTree(np.array([[0.3, 0.1], [1.0, 0.2], [0.1, 1.0], [0.5, 0.5]])).max_simplex_id == 3
+
+
+@pytest.mark.parametrize("name", ["c1_3d_10k", "uniform_2d_10k", "clustered_3d_20k", "uniform_3d_50k"])
+def test_golden_hashes(name):
+    with open(os.path.join(G, "qhull_hashes.json")) as f:
+        ref = json.load(f)[name]
+    P = {"c1_3d_10k": lambda: np.load(os.path.join(G, "c1_points.npy")),
+         "uniform_2d_10k": lambda: datasets.uniform(10_000, 2, seed=0),
+         "clustered_3d_20k": lambda: datasets.clustered(20_000, seed=0),
+         "uniform_3d_50k": lambda: datasets.uniform(50_000, 3, seed=1)}[name]()
+    print(f"case {name} n={P.shape[0]}", flush=True)
+    t = vo.delaunay(P, seed=0)
+    T, E = t.canonical()
+    assert (T.shape[0], E.shape[0]) == (ref["nT"], ref["nE"])
+    assert canon_sha(T, E) == ref["sha256"]
+
+
+def test_small_fixtures_and_brute_force():
+    for fn in ("qhull_small.npz", "brute_small.npz"):
+        z = np.load(os.path.join(G, fn))
+        for k in sorted({k.rsplit("_", 1)[0] for k in z.files}):
+            t = vo.delaunay(z[k + "_P"], seed=3)
+            T, E = t.canonical()
+            assert np.array_equal(T, z[k + "_T"]), (fn, k)
+            assert np.array_equal(E, z[k + "_E"]), (fn, k)
+
+
+@pytest.mark.parametrize("dim,n,seed", [(3, 3000, 21), (2, 3000, 22), (3, 30000, 23), (2, 100000, 24)])
+def test_matches_oracle(dim, n, seed):
+    P = datasets.uniform(n, dim, seed=seed)
+    t = vo.delaunay(P, seed=seed)
+    T, E = t.canonical()
+    ref = O.Tree.delaunay(P, O.MODE_EXACT).canonical()
+    assert np.array_equal(T, ref.T) and np.array_equal(E, ref.E)
+
+
+def test_insertion_order_independent():
+    P = datasets.uniform(20000, 3, seed=31)
+    a = vo.delaunay(P, seed=1).canonical()
+    b = vo.delaunay(P, seed=99).canonical()
+    assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
+
+
+def test_reference_test_sequence_3d():
+    # tests/test_delaunay_tree.rs:7-38: new(vertices), insert 100 of them, then 1000 more
+    V = datasets.uniform(1000, 3, seed=40)
+    V2 = datasets.uniform(1000, 3, seed=41)
+    t = vo.DelaunayTree(V)
+    t.add_points_to_tree(V[:100], seed=5)
+    t.add_points_to_tree(V2, seed=6)
+    assert t.check_delaunay()
+    o = O.Tree(V)
+    o.insert_seq(V[:100])
+    o.add_points_parallel(V2, input_base=100, nthreads=2)
+    ref = o.canonical()
+    T, E = t.canonical()
+    assert np.array_equal(T, ref.T) and np.array_equal(E, ref.E)
+
+
+def test_invariants_2d_count_and_check():
+    n = 300_000
+    t = vo.delaunay(datasets.uniform(n, 2, seed=50))
+    nv, ns, _ = t.counts()
+    assert nv == n + 3 and ns == 2 * n + 1          # DT(P u S): 2(n+3) - 2 - 3 triangles
+    assert t.violations() == 0
+
+
+def test_invariants_3d_euler_and_check():
+    n = 200_000
+    t = vo.delaunay(datasets.uniform(n, 3, seed=51))
+    s, nb = t.export()
+    assert t.violations() == 0
+    V = n + 4
+    T = s.shape[0]
+    faces = set()
+    edges = set()
+    for a in range(4):
+        for b in range(a + 1, 4):
+            e = np.sort(s[:, [a, b]], axis=1)
+            edges.update(map(tuple, e))
+    F = (4 * T + np.count_nonzero(nb < 0)) // 2
+    assert V - len(edges) + F - T == 1               # Euler characteristic of a 3-ball
+    assert np.count_nonzero(nb < 0) == 4             # the super simplex hull
+
+
+def test_duplicates_raise_like_the_reference():
+    P = datasets.uniform(5000, 3, seed=60)
+    P[100] = P[7]
+    with pytest.raises(vo.VoError) as e:
+        vo.delaunay(P)
+    assert e.value.code == -4
+
+
+def test_near_degenerate_lattice_matches_oracle():
+    # 12^3 lattice with 1e-9 jitter: FP32 cannot separate it, FP64 and the exact tier must
+    g = (np.arange(12) / 12.0)
+    X = np.stack(np.meshgrid(g, g, g, indexing="ij"), -1).reshape(-1, 3)
+    P = X + (2 * datasets.unit(0, 70, 0, X.size).reshape(X.shape) - 1) * 1e-9
+    t = vo.delaunay(P, seed=2)
+    T, E = t.canonical()
+    ref = O.Tree.delaunay(P, O.MODE_EXACT).canonical()
+    assert np.array_equal(T, ref.T) and np.array_equal(E, ref.E)
+    st = t.stats()
+    assert st["tier2"][1] > 0
+
+
+def test_exact_tier_decides_what_fp64_cannot():
+    # points rounded onto a sphere / circle: in-sphere determinants sit at the f64 rounding
+    # level, the FP64 filter fails and the expansion tier decides (no exact degeneracy:
+    # the rounding errors are generic, so the triangulation is still unique)
+    rng = np.random.default_rng(72)
+    for dim in (3, 2):
+        u = rng.normal(size=(400, dim))
+        S = 0.5 + 0.3 * u / np.linalg.norm(u, axis=1)[:, None]
+        P = np.vstack([S, 0.5 + 0.05 * rng.random((50, dim))])
+        t = vo.delaunay(P, seed=4)
+        T, E = t.canonical()
+        ref = O.Tree.delaunay(P, O.MODE_EXACT).canonical()
+        assert np.array_equal(T, ref.T) and np.array_equal(E, ref.E), dim
+        assert t.stats()["exact_evaluated"] > 0, dim
+        # the check re-evaluates every face's in-sphere test: undecided ones are queued,
+        # evaluated exactly, then read back from the cache
+        assert t.violations() == 0
+        st = t.stats()
+        assert st["tier3"][0] + st["tier3"][1] > 0, (dim, st)

@@ -1,0 +1,92 @@
+"""ctypes binding of libvoronoids_hip.so (include/voronoids.h).
+
+Loading fails loudly when the library has not been built: there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libvoronoids_hip.so")
+
+VO_OK = 0
+VO_ERR_INVALID = -1
+VO_ERR_OOM = -2
+VO_ERR_HIP = -3
+VO_ERR_DUPLICATE = -4
+VO_ERR_OVERFLOW = -5
+VO_ERR_PRECISION = -6
+VO_ERR_INTERNAL = -7
+
+EXPORTED = [
+    "vo_tri_create", "vo_tri_destroy", "vo_tri_insert", "vo_tri_stage", "vo_tri_run", "vo_tri_counts",
+    "vo_tri_export", "vo_tri_canonical", "vo_free", "vo_tri_check", "vo_tri_stats", "vo_tri_vertices",
+    "vo_last_error", "vo_version",
+]
+
+
+class VoStats(ctypes.Structure):
+    _fields_ = [
+        ("rounds", ctypes.c_int64), ("points_inserted", ctypes.c_int64), ("points_duplicate", ctypes.c_int64),
+        ("windows_total", ctypes.c_int64), ("winners_total", ctypes.c_int64),
+        ("tier1", ctypes.c_int64 * 2), ("tier2", ctypes.c_int64 * 2), ("tier3", ctypes.c_int64 * 2),
+        ("deferred", ctypes.c_int64 * 2), ("exact_evaluated", ctypes.c_int64),
+        ("walk_steps", ctypes.c_int64), ("chain_hops", ctypes.c_int64), ("killed", ctypes.c_int64),
+        ("created", ctypes.c_int64), ("max_killed", ctypes.c_int64), ("max_facets", ctypes.c_int64),
+        ("ms_run", ctypes.c_double), ("ms_locate_cavity", ctypes.c_double), ("ms_commit", ctypes.c_double),
+        ("ms_exact", ctypes.c_double), ("ms_other", ctypes.c_double),
+        ("launches_locate_cavity", ctypes.c_int64), ("launches_commit", ctypes.c_int64),
+    ]
+
+    def as_dict(self) -> dict:
+        out = {}
+        for name, _ in self._fields_:
+            v = getattr(self, name)
+            out[name] = list(v) if isinstance(v, ctypes.Array) else v
+        return out
+
+
+class VoError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"voronoids error {code}: {msg}")
+        self.code = code
+
+
+_lib = None
+
+
+def lib() -> ctypes.CDLL:
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} is not built: run `python -m voronoids_amd.build` (hipcc, gfx950)")
+        L = ctypes.CDLL(LIB_PATH)
+        P = ctypes.c_void_p
+        dp = ctypes.POINTER(ctypes.c_double)
+        i32p = ctypes.POINTER(ctypes.c_int32)
+        i64p = ctypes.POINTER(ctypes.c_int64)
+        L.vo_tri_create.argtypes = [ctypes.c_int, ctypes.c_int, dp, ctypes.c_int64, ctypes.POINTER(P)]
+        L.vo_tri_destroy.argtypes = [P]
+        L.vo_tri_destroy.restype = None
+        for name in ("vo_tri_insert", "vo_tri_stage"):
+            getattr(L, name).argtypes = [P, dp, ctypes.c_int64, ctypes.c_uint64]
+        L.vo_tri_run.argtypes = [P]
+        L.vo_tri_counts.argtypes = [P, i64p, i64p, i64p]
+        L.vo_tri_export.argtypes = [P, i32p, i32p]
+        L.vo_tri_canonical.argtypes = [P, ctypes.POINTER(i32p), i64p, ctypes.POINTER(i32p), i64p]
+        L.vo_free.argtypes = [ctypes.c_void_p]
+        L.vo_free.restype = None
+        L.vo_tri_check.argtypes = [P, i64p]
+        L.vo_tri_stats.argtypes = [P, ctypes.POINTER(VoStats)]
+        L.vo_tri_vertices.argtypes = [P, dp]
+        L.vo_last_error.restype = ctypes.c_char_p
+        L.vo_version.restype = ctypes.c_int
+        _lib = L
+    return _lib
+
+
+def check(rc: int) -> None:
+    if rc != VO_OK:
+        msg = lib().vo_last_error()
+        raise VoError(rc, msg.decode() if msg else "")

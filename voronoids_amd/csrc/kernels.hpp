@@ -1,0 +1,606 @@
+// kernels.hpp -- the per-round device kernels of the parallel Bowyer-Watson construction.
+//
+// Reference path replaced (src/delaunay_tree.rs): locate/find_all_neighbors :33-75 become the
+// walk + cavity BFS of k_cavity; get_new_simplices :77-123, pair_simplices :674-695 and
+// insert_point / insert_points_parallel :125-334 become k_commit; the static level schedule
+// of make_queue/find_placement (src/scheduler.rs:6-55) becomes per-round reservation.
+//
+// Device state (D = 2 or 3, M = D + 1 vertices per simplex):
+//   tv[t]   int4: vertex ids, positively oriented; dead: x = -1, y = forward simplex
+//   ta[t]   int4: (neighbor << 2) | face-in-neighbor across the face opposite vertex k, -1 on
+//           the super-simplex hull
+//   kres[t], nres[t]: 64-bit reservation keys ((2^32-1-round) << 32 | priority): the least
+//           key of the round's points that kill t / that have t as an outside neighbor
+//   X64     f64 coordinates, stride 4 (3D) or 2 (2D); vertex v = super (v < M) or point v-M
+//   X32     f32 coordinates translated by the bounding-sphere center, same strides
+//   cp[p]   conflict pointer of point p: a simplex containing it (or stale: dead -> forward)
+// Simplices are append-only: a killed simplex keeps a forward pointer into its killer's
+// star, so stale conflict pointers are re-bucketed lazily by a walk when the point is next
+// in a window.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "exact.hpp"
+#include "predicates.hpp"
+
+namespace vo {
+
+enum : int { ST_OK = 0, ST_DEFER = 1, ST_DUP = 2, ST_OVERFLOW = 3, ST_BROKEN = 4, ST_IDLE = 5 };
+enum : uint32_t { ERR_OVERFLOW = 1, ERR_WALK = 2, ERR_CHAIN = 4, ERR_TETCAP = 8, ERR_HULL = 16, ERR_CACHE = 32,
+                  ERR_PRECISION = 64 };
+
+constexpr int kWalkLimit = 1 << 16;
+constexpr int kChainLimit = 1 << 20;
+
+struct Counters {                    // device-side, 64-bit, reduced per wave
+    unsigned long long t1[2], t2[2], t3[2], def[2];
+    unsigned long long walk_steps, chain_hops, killed, created, winners, dups, exact_evaluated;
+    unsigned long long max_killed, max_facets;
+};
+
+struct RoundState {
+    int n_tets;            // simplices allocated (append-only high-water mark)
+    int n_next_pending;    // losers and deferred points appended for the next window
+    int n_winners;
+    int n_q;               // exact-queue fill (may exceed capacity: excess dropped)
+    unsigned int err;      // sticky ERR_* bits
+    int n_live_delta;      // sum over winners of (created - killed) this round
+    int pad[2];
+};
+
+struct Ctx {
+    int4* tv;
+    int4* ta;
+    unsigned long long* kres;
+    unsigned long long* nres;
+    const double* X64;
+    const float* X32;
+    int* cp;
+    const int* win;
+    int* pend_next;
+    int* cav;              // killed lists, interleaved: cav[k * win_cap + slot]
+    int* fac;              // boundary facets (t << 2 | face), interleaved likewise
+    int* nk;
+    int* nf;
+    int* status;
+    int* hint;
+    float glo[3];
+    float ginv;
+    int gn[3];
+    RoundState* st;
+    Counters* cnt;
+    QEntry* q;
+    int qcap;
+    CEntry* cache;
+    uint32_t cmask;
+    int M;                 // vertex id of point p is M + p
+    int win_cap, kcap, fcap;
+    int tet_cap;
+};
+
+template <int D> struct VT;
+template <> struct VT<3> {
+    using F = float4;
+    static constexpr int S = 4;
+    static __device__ __forceinline__ F ld(const float* X, int v) { return reinterpret_cast<const float4*>(X)[v]; }
+};
+template <> struct VT<2> {
+    using F = float2;
+    static constexpr int S = 2;
+    static __device__ __forceinline__ F ld(const float* X, int v) { return reinterpret_cast<const float2*>(X)[v]; }
+};
+
+__device__ __forceinline__ int comp(int4 a, int k) {
+    return k == 0 ? a.x : (k == 1 ? a.y : (k == 2 ? a.z : a.w));
+}
+__device__ __forceinline__ void set_comp(int4& a, int k, int v) {
+    if (k == 0) a.x = v; else if (k == 1) a.y = v; else if (k == 2) a.z = v; else a.w = v;
+}
+__device__ __forceinline__ int* comp_ptr(int4* a, int k) { return reinterpret_cast<int*>(a) + k; }
+
+template <int D> __device__ __forceinline__ int index_of(int4 a, int v) {
+    if (a.x == v) return 0;
+    if (a.y == v) return 1;
+    if (a.z == v) return 2;
+    return (D == 3 && a.w == v) ? 3 : -1;
+}
+
+// Simplex t with vertex i replaced by p, reordered by an even permutation so that p comes
+// first: orientation is preserved, and faces 1..D of the result contain p.
+template <int D> __device__ __forceinline__ int4 replace_first(int4 v, int i, int p) {
+    if (D == 3) {
+        switch (i) {
+            case 0: return make_int4(p, v.y, v.z, v.w);
+            case 1: return make_int4(p, v.x, v.w, v.z);
+            case 2: return make_int4(p, v.x, v.y, v.w);
+            default: return make_int4(p, v.x, v.z, v.y);
+        }
+    } else {
+        switch (i) {
+            case 0: return make_int4(p, v.y, v.z, -1);
+            case 1: return make_int4(p, v.z, v.x, -1);
+            default: return make_int4(p, v.x, v.y, -1);
+        }
+    }
+}
+
+struct LocalCnt {
+    unsigned t1[2], t2[2], t3[2], def[2];
+    unsigned walk, chain, killed, created, wins, dups;
+    unsigned maxk, maxf;
+};
+
+__device__ __forceinline__ unsigned long long wave_sum(unsigned long long v) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+__device__ __forceinline__ long long wave_sum_i64(long long v) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+__device__ __forceinline__ unsigned wave_max(unsigned v) {
+    for (int o = 32; o > 0; o >>= 1) { unsigned w = __shfl_xor(v, o, 64); v = w > v ? w : v; }
+    return v;
+}
+
+// one atomic per wave per counter; all lanes of the wave must call this
+__device__ inline void flush_counters(Counters* c, const LocalCnt& l) {
+    const bool lead = __lane_id() == 0;
+#define VO_FLUSH(field, src)                                          \
+    {                                                                 \
+        unsigned long long s = wave_sum((unsigned long long)(src));   \
+        if (lead && s) atomicAdd(&c->field, s);                       \
+    }
+    VO_FLUSH(t1[0], l.t1[0]) VO_FLUSH(t1[1], l.t1[1]) VO_FLUSH(t2[0], l.t2[0]) VO_FLUSH(t2[1], l.t2[1])
+    VO_FLUSH(t3[0], l.t3[0]) VO_FLUSH(t3[1], l.t3[1]) VO_FLUSH(def[0], l.def[0]) VO_FLUSH(def[1], l.def[1])
+    VO_FLUSH(walk_steps, l.walk) VO_FLUSH(chain_hops, l.chain) VO_FLUSH(killed, l.killed)
+    VO_FLUSH(created, l.created) VO_FLUSH(winners, l.wins) VO_FLUSH(dups, l.dups)
+#undef VO_FLUSH
+    unsigned mk = wave_max(l.maxk), mf = wave_max(l.maxf);
+    if (lead) {
+        if (mk) atomicMax(&c->max_killed, (unsigned long long)mk);
+        if (mf) atomicMax(&c->max_facets, (unsigned long long)mf);
+    }
+}
+
+// exclusive prefix sum across the 64 lanes; all lanes must call
+__device__ __forceinline__ int wave_excl_scan(int v, int& total) {
+    int lane = __lane_id();
+    int x = v;
+    for (int o = 1; o < 64; o <<= 1) {
+        int y = __shfl_up(x, o, 64);
+        if (lane >= o) x += y;
+    }
+    total = __shfl(x, 63, 64);
+    return x - v;
+}
+
+__device__ __forceinline__ void enqueue(const Ctx& c, uint32_t type, const uint32_t* ids) {
+    int slot = atomicAdd(&c.st->n_q, 1);
+    if (slot < c.qcap) {
+        c.q[slot].type = type;
+        for (int i = 0; i < 5; ++i) c.q[slot].id[i] = ids[i];
+    }
+}
+
+// ---------------------------------------------------------------- predicates on ids
+// p strictly inside the circumsphere of simplex v (positively oriented): > 0
+template <int D> __device__ __forceinline__ int conflict(const Ctx& c, int4 v, int pv,
+                                                         typename VT<D>::F pf, LocalCnt& lc) {
+    using V = VT<D>;
+    int s;
+    if constexpr (D == 3) {
+        s = t1_insphere(V::ld(c.X32, v.x), V::ld(c.X32, v.y), V::ld(c.X32, v.z), V::ld(c.X32, v.w), pf);
+    } else {
+        s = t1_incircle(V::ld(c.X32, v.x), V::ld(c.X32, v.y), V::ld(c.X32, v.z), pf);
+    }
+    if (s != SIGN_UNDECIDED) { lc.t1[1]++; return s; }
+    const double* X = c.X64;
+    constexpr int S = V::S;
+    if (D == 3)
+        s = t2_insphere(X + S * v.x, X + S * v.y, X + S * v.z, X + S * v.w, X + S * pv);
+    else
+        s = t2_incircle(X + S * v.x, X + S * v.y, X + S * v.z, X + S * pv);
+    if (s != SIGN_UNDECIDED) { lc.t2[1]++; return s; }
+    uint32_t ids[5];
+    uint32_t type = D == 3 ? PRED_INSPHERE : PRED_INCIRCLE;
+    if (D == 3) { ids[0] = v.x; ids[1] = v.y; ids[2] = v.z; ids[3] = v.w; ids[4] = pv; }
+    else { ids[0] = v.x; ids[1] = v.y; ids[2] = v.z; ids[3] = pv; ids[4] = kEmpty; }
+    s = cache_lookup(c.cache, c.cmask, type, ids);
+    if (s != SIGN_UNDECIDED) { lc.t3[1]++; return s; }
+    lc.def[1]++;
+    enqueue(c, type, ids);
+    return SIGN_UNDECIDED;
+}
+
+// orientation of simplex v with vertex k replaced by p: < 0 means p lies beyond face k
+template <int D> __device__ __forceinline__ int orient_sub(const Ctx& c, int4 v, int k, int pv,
+                                                           typename VT<D>::F pf, LocalCnt& lc) {
+    using V = VT<D>;
+    typename V::F a[4];
+    int ids[4] = {v.x, v.y, v.z, v.w};
+    ids[k] = pv;
+    for (int i = 0; i < D + 1; ++i) a[i] = (i == k) ? pf : V::ld(c.X32, ids[i]);
+    int s;
+    if constexpr (D == 3) s = t1_orient3d(a[0], a[1], a[2], a[3]);
+    else s = t1_orient2d(a[0], a[1], a[2]);
+    if (s != SIGN_UNDECIDED) { lc.t1[0]++; return s; }
+    const double* X = c.X64;
+    constexpr int S = V::S;
+    if (D == 3) s = t2_orient3d(X + S * ids[0], X + S * ids[1], X + S * ids[2], X + S * ids[3]);
+    else s = t2_orient2d(X + S * ids[0], X + S * ids[1], X + S * ids[2]);
+    if (s != SIGN_UNDECIDED) { lc.t2[0]++; return s; }
+    uint32_t u[5] = {(uint32_t)ids[0], (uint32_t)ids[1], (uint32_t)ids[2], D == 3 ? (uint32_t)ids[3] : kEmpty, kEmpty};
+    uint32_t type = D == 3 ? PRED_ORIENT3 : PRED_ORIENT2;
+    s = cache_lookup(c.cache, c.cmask, type, u);
+    if (s != SIGN_UNDECIDED) { lc.t3[0]++; return s; }
+    lc.def[0]++;
+    enqueue(c, type, u);
+    return SIGN_UNDECIDED;
+}
+
+template <int D> __device__ __forceinline__ int grid_cell(const Ctx& c, typename VT<D>::F pf) {
+    int ix = (int)fminf(fmaxf((pf.x - c.glo[0]) * c.ginv, 0.0f), (float)(c.gn[0] - 1));
+    int iy = (int)fminf(fmaxf((pf.y - c.glo[1]) * c.ginv, 0.0f), (float)(c.gn[1] - 1));
+    int iz = 0;
+    if constexpr (D == 3) iz = (int)fminf(fmaxf((pf.z - c.glo[2]) * c.ginv, 0.0f), (float)(c.gn[2] - 1));
+    return (iz * c.gn[1] + iy) * c.gn[0] + ix;
+}
+
+__device__ __forceinline__ uint32_t mix32(uint32_t x) {
+    x ^= x >> 16; x *= 0x7feb352dU; x ^= x >> 15; x *= 0x846ca68bU; x ^= x >> 16;
+    return x;
+}
+
+// ---------------------------------------------------------------- k_cavity
+// One thread per window point: follow dead simplices forward, walk to the simplex that
+// contains p (orientation tests), BFS the conflict region (in-sphere tests), then reserve
+// the killed simplices (kres) and the outside neighbors (nres) with 64-bit atomicMin.
+template <int D>
+__global__ void __launch_bounds__(256) k_cavity(Ctx c, int nwin, unsigned long long key_hi) {
+    using V = VT<D>;
+    constexpr int M = D + 1;
+    const int slot = blockIdx.x * blockDim.x + threadIdx.x;
+    LocalCnt lc = {};
+    if (slot < nwin) {
+        const int p = c.win[slot];
+        const int pv = c.M + p;
+        const typename V::F pf = V::ld(c.X32, pv);
+        const int W = c.win_cap;
+        int st = ST_OK;
+        const int cell = grid_cell<D>(c, pf);
+        int t = c.cp[p];
+        if (t < 0) t = c.hint[cell];
+        int4 tt = c.tv[t];
+        for (int hops = 0; tt.x < 0; ++hops) {
+            if (hops > kChainLimit) { atomicOr(&c.st->err, ERR_CHAIN); st = ST_BROKEN; break; }
+            t = tt.y;
+            tt = c.tv[t];
+            lc.chain++;
+        }
+        // remembering stochastic visibility walk
+        int came = -1;
+        uint32_t rng = mix32((uint32_t)p * 0x9E3779B9u + 0x632BE5ABu);
+        for (int step = 0; st == ST_OK; ++step) {
+            if (step > kWalkLimit) { atomicOr(&c.st->err, ERR_WALK); st = ST_BROKEN; break; }
+            const int4 aa = c.ta[t];
+            rng = rng * 1664525u + 1013904223u;
+            const int r = (int)((rng >> 16) % M);
+            int moved = 0;
+            for (int kk = 0; kk < M; ++kk) {
+                int k = r + kk;
+                if (k >= M) k -= M;
+                const int nb = comp(aa, k);
+                if (nb >= 0 && (nb >> 2) == came) continue;
+                const int s = orient_sub<D>(c, tt, k, pv, pf, lc);
+                if (s == SIGN_UNDECIDED) { st = ST_DEFER; break; }
+                if (s < 0) {
+                    if (nb < 0) { atomicOr(&c.st->err, ERR_HULL); st = ST_BROKEN; break; }
+                    came = t;
+                    t = nb >> 2;
+                    tt = c.tv[t];
+                    moved = 1;
+                    break;
+                }
+            }
+            lc.walk++;
+            if (!moved) break;
+        }
+        c.cp[p] = t;
+        c.hint[cell] = t;
+        int nk = 0, nf = 0;
+        if (st == ST_OK) {
+            const int s0 = conflict<D>(c, tt, pv, pf, lc);
+            if (s0 == SIGN_UNDECIDED) st = ST_DEFER;
+            else if (s0 <= 0) st = ST_DUP;   // on the circumsphere of its own simplex: a vertex
+        }
+        if (st == ST_OK) {
+            c.cav[slot] = t;
+            nk = 1;
+            for (int qi = 0; qi < nk && st == ST_OK; ++qi) {
+                const int ct = c.cav[qi * W + slot];
+                const int4 aa = c.ta[ct];
+                for (int f = 0; f < M; ++f) {
+                    const int nb = comp(aa, f);
+                    if (nb >= 0) {
+                        const int n = nb >> 2;
+                        bool seen = false;
+                        for (int j = 0; j < nk && !seen; ++j) seen = c.cav[j * W + slot] == n;
+                        if (seen) continue;
+                        const int s = conflict<D>(c, c.tv[n], pv, pf, lc);
+                        if (s == SIGN_UNDECIDED) { st = ST_DEFER; break; }
+                        if (s > 0) {
+                            if (nk == c.kcap) { st = ST_OVERFLOW; break; }
+                            c.cav[nk * W + slot] = n;
+                            nk++;
+                            continue;
+                        }
+                    }
+                    if (nf == c.fcap) { st = ST_OVERFLOW; break; }
+                    c.fac[nf * W + slot] = (ct << 2) | f;
+                    nf++;
+                }
+            }
+        }
+        if (st == ST_OK) {
+            const unsigned long long key = key_hi | (unsigned)p;
+            for (int j = 0; j < nk; ++j) atomicMin(&c.kres[c.cav[j * W + slot]], key);
+            for (int j = 0; j < nf; ++j) {
+                const int fe = c.fac[j * W + slot];
+                const int nb = comp(c.ta[fe >> 2], fe & 3);
+                if (nb >= 0) atomicMin(&c.nres[nb >> 2], key);
+            }
+        }
+        if (st == ST_OVERFLOW) atomicOr(&c.st->err, ERR_OVERFLOW);
+        if (st == ST_DUP) lc.dups++;
+        c.nk[slot] = nk;
+        c.nf[slot] = nf;
+        c.status[slot] = st;
+    }
+    flush_counters(c.cnt, lc);
+}
+
+// ---------------------------------------------------------------- k_commit
+// One thread per window point.  A point wins when it holds the least key on every simplex it
+// kills and no earlier point of the round kills one of its outside neighbors or has one of
+// its killed simplices as an outside neighbor: the winners' cavities are then pairwise
+// disjoint and non-adjacent, and committing them in parallel equals inserting them one by
+// one.  Winners get their simplex slots from one atomicAdd per wave (exclusive wave scan),
+// build one simplex per boundary facet, link internal faces by rotating around each ridge
+// inside the cavity, patch the outside neighbors, and leave forward pointers in the killed
+// simplices.  Losers and deferred points go to the next window (ballot + popcount).
+template <int D>
+__global__ void __launch_bounds__(256) k_commit(Ctx c, int nwin, unsigned long long key_hi) {
+    constexpr int M = D + 1;
+    const int slot = blockIdx.x * blockDim.x + threadIdx.x;
+    const int W = c.win_cap;
+    LocalCnt lc = {};
+    int st = ST_IDLE, p = 0, nk = 0, nf = 0;
+    bool win = false;
+    unsigned long long key = 0;
+    if (slot < nwin) {
+        st = c.status[slot];
+        p = c.win[slot];
+        key = key_hi | (unsigned)p;
+        if (st == ST_OK) {
+            nk = c.nk[slot];
+            nf = c.nf[slot];
+            win = true;
+            for (int j = 0; j < nk && win; ++j) {
+                const int t = c.cav[j * W + slot];
+                win = c.kres[t] == key && c.nres[t] > key;
+            }
+            for (int j = 0; j < nf && win; ++j) {
+                const int fe = c.fac[j * W + slot];
+                const int nb = comp(c.ta[fe >> 2], fe & 3);
+                if (nb >= 0) win = c.kres[nb >> 2] > key;
+            }
+        }
+    }
+    int total = 0;
+    const int off = wave_excl_scan(win ? nf : 0, total);
+    int base0 = 0;
+    if (__lane_id() == 0 && total > 0) base0 = atomicAdd(&c.st->n_tets, total);
+    base0 = __shfl(base0, 0, 64);
+    const int base = base0 + off;
+    if (win && base + nf > c.tet_cap) {
+        atomicOr(&c.st->err, ERR_TETCAP);
+        win = false;
+    }
+    if (win) {
+        const int pv = c.M + p;
+        for (int j = 0; j < nf; ++j) {
+            const int fe = c.fac[j * W + slot];
+            const int t = fe >> 2, i = fe & 3;
+            const int4 tvt = c.tv[t];
+            const int4 tat = c.ta[t];
+            const int4 nv = replace_first<D>(tvt, i, pv);
+            int4 na = make_int4(-1, -1, -1, -1);
+            const int outside = comp(tat, i);
+            na.x = outside;
+            const int N = base + j;
+            if (outside >= 0) *comp_ptr(&c.ta[outside >> 2], outside & 3) = N << 2;
+            for (int k = 1; k < M; ++k) {
+                // ridge = nv minus {p, nv[k]}; rotate around it through the cavity
+                const int cv = comp(nv, k);
+                int T = t;
+                int4 TV = tvt, TA = tat;
+                int e = index_of<D>(TV, cv);
+                int y = comp(tvt, i);
+                for (int guard = 0;; ++guard) {
+                    const int nbe = comp(TA, e);
+                    if (nbe < 0 || c.kres[nbe >> 2] != key) break;
+                    if (guard > 4096) { atomicOr(&c.st->err, ERR_WALK); break; }
+                    const int T2 = nbe >> 2;
+                    const int4 TV2 = c.tv[T2];
+                    const int w = comp(TV2, nbe & 3);
+                    e = index_of<D>(TV2, y);
+                    y = w;
+                    T = T2;
+                    TV = TV2;
+                    TA = c.ta[T2];
+                }
+                const int target = (T << 2) | e;
+                int jj = -1;
+                for (int q = 0; q < nf; ++q)
+                    if (c.fac[q * W + slot] == target) { jj = q; break; }
+                const int4 nv2 = replace_first<D>(TV, e, pv);
+                int kk = 1;
+                for (int m = 1; m < M; ++m) {
+                    const int x = comp(nv2, m);
+                    bool in_ridge = false;
+                    for (int r = 1; r < M; ++r)
+                        if (r != k && comp(nv, r) == x) in_ridge = true;
+                    if (!in_ridge) { kk = m; break; }
+                }
+                set_comp(na, k, jj < 0 ? -1 : (((base + jj) << 2) | kk));
+                if (jj < 0) atomicOr(&c.st->err, ERR_WALK);
+            }
+            c.tv[N] = nv;
+            c.ta[N] = na;
+        }
+        for (int j = 0; j < nk; ++j) c.tv[c.cav[j * W + slot]] = make_int4(-1, base, 0, 0);
+        for (int j = 0; j < nf; ++j) c.tv[c.fac[j * W + slot] >> 2].y = base + j;
+        lc.killed = nk;
+        lc.created = nf;
+        lc.wins = 1;
+        lc.maxk = nk;
+        lc.maxf = nf;
+    }
+    // next window: losers (valid cavity, lost a reservation) and deferred points
+    const bool again = (st == ST_OK && !win) || st == ST_DEFER;
+    const unsigned long long mask = __ballot(again);
+    const int cnt = __popcll(mask);
+    int pos0 = 0;
+    if (__lane_id() == 0 && cnt) pos0 = atomicAdd(&c.st->n_next_pending, cnt);
+    pos0 = __shfl(pos0, 0, 64);
+    if (again) {
+        const int before = __popcll(mask & ((1ull << __lane_id()) - 1ull));
+        c.pend_next[pos0 + before] = p;
+    }
+    const unsigned long long wm = __ballot(win);
+    if (__lane_id() == 0) {
+        const int wc = __popcll(wm);
+        if (wc) atomicAdd(&c.st->n_winners, wc);
+    }
+    const long long delta = wave_sum_i64(win ? (long long)(nf - nk) : 0ll);
+    if (__lane_id() == 0 && delta) atomicAdd(&c.st->n_live_delta, (int)delta);
+    flush_counters(c.cnt, lc);
+}
+
+// next window = pending losers followed by fresh points head .. head + nfresh - 1
+__global__ void k_window(int* win, const int* pend, int npend, int head, int nfresh) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < npend) win[i] = pend[i];
+    else if (i < npend + nfresh) win[i] = head + (i - npend);
+}
+
+// ---------------------------------------------------------------- k_exact (tier 3)
+template <int S>
+__global__ void __launch_bounds__(64) k_exact(const QEntry* q, int nq, const double* X64, double* ws, int ws_cap,
+                                              CEntry* cache, uint32_t cmask, RoundState* st, Counters* cnt) {
+    const int tid = blockIdx.x * blockDim.x + threadIdx.x;
+    const int nth = gridDim.x * blockDim.x;
+    for (int i = tid; i < nq; i += nth) {
+        Ws w{ws + (size_t)tid * ws_cap, ws_cap, 0, 0};
+        const QEntry e = q[i];
+        const double* P[5];
+        for (int j = 0; j < pred_arity(e.type); ++j) P[j] = X64 + (size_t)S * e.id[j];
+        int s;
+        switch (e.type) {
+            case PRED_ORIENT2: s = ex_orient2d(w, P[0], P[1], P[2]); break;
+            case PRED_INCIRCLE: s = ex_incircle(w, P[0], P[1], P[2], P[3]); break;
+            case PRED_ORIENT3: s = ex_orient3d(w, P[0], P[1], P[2], P[3]); break;
+            default: s = ex_insphere(w, P[0], P[1], P[2], P[3], P[4]); break;
+        }
+        if (w.ovf || s == SIGN_UNDECIDED) { atomicOr(&st->err, ERR_PRECISION); continue; }
+        cache_insert(cache, cmask, e.type, e.id, s, &st->err);
+        atomicAdd(&cnt->exact_evaluated, 1ull);
+    }
+}
+
+// ---------------------------------------------------------------- export / check
+__global__ void k_live_flags(const int4* tv, int n, int* flag) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) flag[i] = tv[i].x >= 0;
+}
+
+// exclusive scan of flags within each 256-block (block sums out), then add block offsets
+__global__ void k_block_scan(const int* in, int* out, int* bsum, int n) {
+    __shared__ int s[256];
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    const int v = i < n ? in[i] : 0;
+    s[threadIdx.x] = v;
+    __syncthreads();
+    for (int o = 1; o < 256; o <<= 1) {
+        int add = threadIdx.x >= (unsigned)o ? s[threadIdx.x - o] : 0;
+        __syncthreads();
+        s[threadIdx.x] += add;
+        __syncthreads();
+    }
+    if (i < n) out[i] = s[threadIdx.x] - v;
+    if (threadIdx.x == 255) bsum[blockIdx.x] = s[255];
+}
+__global__ void k_add_offsets(int* out, const int* boff, int n) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i < n) out[i] += boff[blockIdx.x];
+}
+
+// compacted export: vertex ids mapped to the reference id space, neighbors to export indices
+template <int D>
+__global__ void k_export(const int4* tv, const int4* ta, const int* pos, int n, const int* vmap, int32_t* simp,
+                         int32_t* nbr) {
+    constexpr int M = D + 1;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int4 v = tv[i];
+    if (v.x < 0) return;
+    const int o = pos[i];
+    const int4 a = ta[i];
+    for (int k = 0; k < M; ++k) {
+        if (simp) simp[(size_t)o * M + k] = vmap[comp(v, k)];
+        if (nbr) {
+            const int nb = comp(a, k);
+            nbr[(size_t)o * M + k] = nb < 0 ? -1 : pos[nb >> 2];
+        }
+    }
+}
+
+// exact local Delaunay + adjacency symmetry + orientation, one thread per simplex
+template <int D>
+__global__ void __launch_bounds__(256) k_check(Ctx c, int n, unsigned long long* bad, int* unresolved) {
+    using V = VT<D>;
+    constexpr int M = D + 1;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    LocalCnt lc = {};
+    unsigned nbad = 0, nun = 0;
+    if (t < n) {
+        const int4 v = c.tv[t];
+        if (v.x >= 0) {
+            const int4 a = c.ta[t];
+            int s = orient_sub<D>(c, v, 0, v.x, V::ld(c.X32, v.x), lc);
+            if (s == SIGN_UNDECIDED) nun++;
+            else if (s <= 0) nbad++;
+            for (int f = 0; f < M; ++f) {
+                const int nb = comp(a, f);
+                if (nb < 0) continue;
+                const int n2 = nb >> 2, f2 = nb & 3;
+                const int4 v2 = c.tv[n2];
+                if (v2.x < 0 || comp(c.ta[n2], f2) != ((t << 2) | f)) { nbad++; continue; }
+                const int w = comp(v2, f2);
+                const int cs = conflict<D>(c, v, w, V::ld(c.X32, w), lc);
+                if (cs == SIGN_UNDECIDED) nun++;
+                else if (cs > 0) nbad++;
+            }
+        }
+    }
+    const unsigned long long sb = wave_sum(nbad), su = wave_sum(nun);
+    if (__lane_id() == 0) {
+        if (sb) atomicAdd(bad, sb);
+        if (su) atomicAdd(unresolved, (int)su);
+    }
+    flush_counters(c.cnt, lc);
+}
+
+}  // namespace vo

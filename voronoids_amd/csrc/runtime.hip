@@ -1,0 +1,749 @@
+// runtime.hip -- host runtime and C ABI of libvoronoids_hip.so (see include/voronoids.h).
+//
+// The round driver replaces add_points_to_tree / insert_points_parallel
+// (src/delaunay_tree.rs:213-386) and the insert_point loop of voronoids.delaunay
+// (src/lib.rs:104-125): every round takes a window of uninserted points (the previous
+// round's losers first, then fresh points in priority order), runs k_cavity and k_commit,
+// resolves deferred predicates exactly when any were queued, and adapts the window size to
+// keep about half of the window winning.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <array>
+#include <cmath>
+#include <cstdint>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/voronoids.h"
+#include "kernels.hpp"
+
+#pragma clang fp contract(off)
+
+using namespace vo;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+    g_err = msg;
+    return code;
+}
+
+#define VO_HIP(x)                                                                              \
+    do {                                                                                       \
+        hipError_t e_ = (x);                                                                   \
+        if (e_ != hipSuccess) return fail(VO_ERR_HIP, std::string(#x) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+// ------------------------------------------------------------------ reference geometry
+// bounding_sphere, src/geometry.rs:99-141: per-axis min/max folds, center (hi+lo)/2, radius
+// the larger corner distance, then one pass in input order multiplying the radius by 1.5 for
+// every point the current sphere does not strictly contain (in_sphere, :91-97).
+void bounding_sphere(int D, const double* pts, int64_t n, double* c, double* r) {
+    double lo[3], hi[3];
+    for (int i = 0; i < D; ++i) {
+        lo[i] = INFINITY;
+        hi[i] = -INFINITY;
+        for (int64_t k = 0; k < n; ++k) {
+            lo[i] = std::fmin(lo[i], pts[k * D + i]);
+            hi[i] = std::fmax(hi[i], pts[k * D + i]);
+        }
+        c[i] = (hi[i] + lo[i]) / 2.0;
+    }
+    double su = 0.0, sl = 0.0;
+    for (int i = 0; i < D; ++i) su += (hi[i] - c[i]) * (hi[i] - c[i]);
+    for (int i = 0; i < D; ++i) sl += (lo[i] - c[i]) * (lo[i] - c[i]);
+    const double ud = std::sqrt(su), ld = std::sqrt(sl);
+    double rad = ud > ld ? ud : ld;
+    for (int64_t k = 0; k < n; ++k) {
+        double dist = 0.0;
+        for (int i = 0; i < D; ++i) dist += (c[i] - pts[k * D + i]) * (c[i] - pts[k * D + i]);
+        if (!(dist < rad * rad)) rad = rad * 1.5;
+    }
+    *r = rad;
+}
+
+// super vertices of DelaunayTree::new: 3D src/delaunay_tree.rs:393-406, 2D :548-558
+void super_vertices(int D, const double* c, double R, double out[4][3]) {
+    const double pi = 3.14159265358979323846;
+    const double c1 = std::cos(2. * pi / 3.), s1 = std::sin(2. * pi / 3.);
+    const double c2 = std::cos(4. * pi / 3.), s2 = std::sin(4. * pi / 3.);
+    if (D == 3) {
+        const double v[4][3] = {{c[0], c[1], c[2] + R},
+                                {c[0] + R, c[1], c[2] - R},
+                                {c[0] + R * c1, c[1] + R * s1, c[2] - R},
+                                {c[0] + R * c2, c[1] + R * s2, c[2] - R}};
+        std::memcpy(out, v, sizeof(v));
+    } else {
+        const double v[3][3] = {{c[0] + R, c[1], 0}, {c[0] + R * c1, c[1] + R * s1, 0}, {c[0] + R * c2, c[1] + R * s2, 0}};
+        std::memcpy(out, v, sizeof(v));
+    }
+}
+
+uint64_t splitmix_next(uint64_t& s) {
+    uint64_t z = (s += 0x9E3779B97F4A7C15ull);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+template <class T>
+int dev_alloc(T** p, size_t count) {
+    *p = nullptr;
+    if (count == 0) count = 1;
+    hipError_t e = hipMalloc((void**)p, sizeof(T) * count);
+    if (e != hipSuccess) return fail(VO_ERR_OOM, std::string("hipMalloc: ") + hipGetErrorString(e));
+    return 0;
+}
+
+}  // namespace
+
+struct vo_tri {
+    int dev = 0, D = 3, M = 4, S = 4;
+    hipStream_t s = nullptr;
+    double c[3] = {0, 0, 0}, r = 0, R = 0;
+    double sv[4][3] = {};
+    int tet0[4] = {0, 1, 2, 3};
+    // points
+    int64_t n_pts = 0, vcap = 0, head = 0;
+    std::vector<int32_t> input_of;     // point index -> input index
+    double* X64 = nullptr;
+    float* X32 = nullptr;
+    int* cp = nullptr;
+    int* vmap = nullptr;               // vertex id -> reference id
+    // simplices
+    int4* tv = nullptr;
+    int4* ta = nullptr;
+    unsigned long long* kres = nullptr;
+    unsigned long long* nres = nullptr;
+    int64_t tcap = 0;
+    int n_tets = 1;
+    int64_t n_live = 1;
+    // window
+    int win_cap = 0, kcap = 0, fcap = 0;
+    int* win = nullptr;
+    int* pend[2] = {nullptr, nullptr};
+    int cur = 0, npend = 0;
+    int* cav = nullptr;
+    int* fac = nullptr;
+    int* nk = nullptr;
+    int* nf = nullptr;
+    int* status = nullptr;
+    // hints
+    int* hint = nullptr;
+    int gn[3] = {1, 1, 1};
+    float glo[3] = {0, 0, 0};
+    float ginv = 1;
+    // control
+    RoundState* st = nullptr;
+    RoundState* st_host = nullptr;
+    Counters* cnt = nullptr;
+    QEntry* q = nullptr;
+    int qcap = 0;
+    CEntry* cache = nullptr;
+    uint32_t cmask = 0;
+    double* ws = nullptr;
+    int ws_cap = 0, ws_threads = 0;
+    hipEvent_t ev[6] = {};
+    double gamma = 0.25;
+    int64_t inserted = 0, dups = 0, rounds = 0, windows_total = 0;
+    double ms_run = 0, ms_cav = 0, ms_com = 0, ms_ex = 0, ms_other = 0;
+    int64_t n_cav = 0, n_com = 0;
+    bool dirty_dup = false;
+};
+
+namespace {
+
+Ctx make_ctx(vo_tri* t) {
+    Ctx c;
+    c.tv = t->tv;
+    c.ta = t->ta;
+    c.kres = t->kres;
+    c.nres = t->nres;
+    c.X64 = t->X64;
+    c.X32 = t->X32;
+    c.cp = t->cp;
+    c.win = t->win;
+    c.pend_next = t->pend[t->cur ^ 1];
+    c.cav = t->cav;
+    c.fac = t->fac;
+    c.nk = t->nk;
+    c.nf = t->nf;
+    c.status = t->status;
+    c.hint = t->hint;
+    for (int i = 0; i < 3; ++i) { c.glo[i] = t->glo[i]; c.gn[i] = t->gn[i]; }
+    c.ginv = t->ginv;
+    c.st = t->st;
+    c.cnt = t->cnt;
+    c.q = t->q;
+    c.qcap = t->qcap;
+    c.cache = t->cache;
+    c.cmask = t->cmask;
+    c.M = t->M;
+    c.win_cap = t->win_cap;
+    c.kcap = t->kcap;
+    c.fcap = t->fcap;
+    c.tet_cap = (int)t->tcap;
+    return c;
+}
+
+__global__ void k_init_first(int4* tv, int4* ta, int a, int b, int c, int d) {
+    tv[0] = make_int4(a, b, c, d);
+    ta[0] = make_int4(-1, -1, -1, -1);
+}
+
+__global__ void k_reset_round(RoundState* st) {
+    st->n_next_pending = 0;
+    st->n_winners = 0;
+    st->n_q = 0;
+    st->n_live_delta = 0;
+}
+
+// grow the simplex arrays to cap (copying the first n), new reservations = max key
+int grow_tets(vo_tri* t, int64_t cap) {
+    if (cap <= t->tcap) return 0;
+    if (cap > (int64_t)1 << 30) return fail(VO_ERR_OOM, "simplex capacity above 2^30");
+    int4 *tv, *ta;
+    unsigned long long *kr, *nr;
+    int rc;
+    if ((rc = dev_alloc(&tv, cap)) || (rc = dev_alloc(&ta, cap)) || (rc = dev_alloc(&kr, cap)) || (rc = dev_alloc(&nr, cap)))
+        return rc;
+    VO_HIP(hipMemsetAsync(kr, 0xFF, sizeof(unsigned long long) * cap, t->s));
+    VO_HIP(hipMemsetAsync(nr, 0xFF, sizeof(unsigned long long) * cap, t->s));
+    if (t->tv) {
+        VO_HIP(hipMemcpyAsync(tv, t->tv, sizeof(int4) * t->n_tets, hipMemcpyDeviceToDevice, t->s));
+        VO_HIP(hipMemcpyAsync(ta, t->ta, sizeof(int4) * t->n_tets, hipMemcpyDeviceToDevice, t->s));
+        VO_HIP(hipMemcpyAsync(kr, t->kres, sizeof(unsigned long long) * t->n_tets, hipMemcpyDeviceToDevice, t->s));
+        VO_HIP(hipMemcpyAsync(nr, t->nres, sizeof(unsigned long long) * t->n_tets, hipMemcpyDeviceToDevice, t->s));
+        VO_HIP(hipStreamSynchronize(t->s));
+        hipFree(t->tv); hipFree(t->ta); hipFree(t->kres); hipFree(t->nres);
+    }
+    t->tv = tv; t->ta = ta; t->kres = kr; t->nres = nr;
+    t->tcap = cap;
+    return 0;
+}
+
+int grow_points(vo_tri* t, int64_t cap) {
+    if (cap <= t->vcap) return 0;
+    if (t->M + cap > INT32_MAX / 2) return fail(VO_ERR_INVALID, "too many points");
+    double* X64;
+    float* X32;
+    int *cp, *vmap;
+    int rc;
+    const int64_t nv = t->M + cap;
+    if ((rc = dev_alloc(&X64, nv * t->S)) || (rc = dev_alloc(&X32, nv * t->S)) || (rc = dev_alloc(&cp, cap)) ||
+        (rc = dev_alloc(&vmap, nv)))
+        return rc;
+    if (t->X64) {
+        const int64_t old = t->M + t->n_pts;
+        VO_HIP(hipMemcpyAsync(X64, t->X64, sizeof(double) * old * t->S, hipMemcpyDeviceToDevice, t->s));
+        VO_HIP(hipMemcpyAsync(X32, t->X32, sizeof(float) * old * t->S, hipMemcpyDeviceToDevice, t->s));
+        VO_HIP(hipMemcpyAsync(cp, t->cp, sizeof(int) * std::max<int64_t>(t->n_pts, 1), hipMemcpyDeviceToDevice, t->s));
+        VO_HIP(hipMemcpyAsync(vmap, t->vmap, sizeof(int) * old, hipMemcpyDeviceToDevice, t->s));
+        VO_HIP(hipStreamSynchronize(t->s));
+        hipFree(t->X64); hipFree(t->X32); hipFree(t->cp); hipFree(t->vmap);
+    }
+    t->X64 = X64; t->X32 = X32; t->cp = cp; t->vmap = vmap;
+    t->vcap = cap;
+    return 0;
+}
+
+int upload_super(vo_tri* t) {
+    std::vector<double> x64((size_t)t->M * t->S, 0.0);
+    std::vector<float> x32((size_t)t->M * t->S, 0.0f);
+    std::vector<int> vm(t->M);
+    for (int v = 0; v < t->M; ++v) {
+        for (int k = 0; k < t->D; ++k) {
+            x64[v * t->S + k] = t->sv[v][k];
+            x32[v * t->S + k] = (float)(t->sv[v][k] - t->c[k]);
+        }
+        vm[v] = v;
+    }
+    VO_HIP(hipMemcpyAsync(t->X64, x64.data(), sizeof(double) * x64.size(), hipMemcpyHostToDevice, t->s));
+    VO_HIP(hipMemcpyAsync(t->X32, x32.data(), sizeof(float) * x32.size(), hipMemcpyHostToDevice, t->s));
+    VO_HIP(hipMemcpyAsync(t->vmap, vm.data(), sizeof(int) * vm.size(), hipMemcpyHostToDevice, t->s));
+    VO_HIP(hipStreamSynchronize(t->s));
+    return 0;
+}
+
+void free_all(vo_tri* t) {
+    void* ptrs[] = {t->X64, t->X32, t->cp, t->vmap, t->tv, t->ta, t->kres, t->nres, t->win, t->pend[0], t->pend[1],
+                    t->cav, t->fac, t->nk, t->nf, t->status, t->hint, t->st, t->cnt, t->q, t->cache, t->ws};
+    for (void* p : ptrs)
+        if (p) hipFree(p);
+    if (t->st_host) hipHostFree(t->st_host);
+    for (auto& e : t->ev)
+        if (e) hipEventDestroy(e);
+    if (t->s) hipStreamDestroy(t->s);
+}
+
+int sync_state(vo_tri* t) {
+    VO_HIP(hipMemcpyAsync(t->st_host, t->st, sizeof(RoundState), hipMemcpyDeviceToHost, t->s));
+    VO_HIP(hipStreamSynchronize(t->s));
+    return 0;
+}
+
+int check_err(vo_tri* t) {
+    const unsigned e = t->st_host->err;
+    if (!e) return 0;
+    if (e & ERR_OVERFLOW) return fail(VO_ERR_OVERFLOW, "a cavity exceeded the per-point buffers");
+    if (e & ERR_TETCAP) return fail(VO_ERR_OVERFLOW, "simplex capacity exceeded");
+    if (e & ERR_PRECISION) return fail(VO_ERR_PRECISION, "exact predicate left the f64 range or workspace");
+    if (e & ERR_CACHE) return fail(VO_ERR_OVERFLOW, "exact-result cache full");
+    if (e & ERR_HULL) return fail(VO_ERR_INVALID, "a point lies outside the super simplex");
+    return fail(VO_ERR_INTERNAL, "walk, chain or rotation bound exceeded (err=" + std::to_string(e) + ")");
+}
+
+int run_exact(vo_tri* t, int nq) {
+    if (nq <= 0) return 0;
+    nq = std::min(nq, t->qcap);
+    const int threads = 64;
+    const int blocks = std::max(1, std::min((nq + threads - 1) / threads, t->ws_threads / threads));
+    VO_HIP(hipEventRecord(t->ev[4], t->s));
+    if (t->D == 3)
+        hipLaunchKernelGGL(k_exact<4>, dim3(blocks), dim3(threads), 0, t->s, t->q, nq, t->X64, t->ws, t->ws_cap, t->cache,
+                           t->cmask, t->st, t->cnt);
+    else
+        hipLaunchKernelGGL(k_exact<2>, dim3(blocks), dim3(threads), 0, t->s, t->q, nq, t->X64, t->ws, t->ws_cap, t->cache,
+                           t->cmask, t->st, t->cnt);
+    VO_HIP(hipGetLastError());
+    VO_HIP(hipEventRecord(t->ev[5], t->s));
+    VO_HIP(hipEventSynchronize(t->ev[5]));
+    float ms = 0;
+    hipEventElapsedTime(&ms, t->ev[4], t->ev[5]);
+    t->ms_ex += ms;
+    return 0;
+}
+
+}  // namespace
+
+// =================================================================== C ABI
+extern "C" {
+
+const char* vo_last_error(void) { return g_err.c_str(); }
+int vo_version(void) { return 1; }
+void vo_free(void* p) { std::free(p); }
+
+int vo_tri_create(int hip_device, int dim, const double* pts, int64_t n, vo_tri** out) {
+    if (!out) return fail(VO_ERR_INVALID, "out is null");
+    *out = nullptr;
+    if (dim != 2 && dim != 3) return fail(VO_ERR_INVALID, "dim must be 2 or 3");
+    if (n < 0 || (n > 0 && !pts)) return fail(VO_ERR_INVALID, "bad point buffer");
+    for (int64_t i = 0; i < n * dim; ++i)
+        if (!std::isfinite(pts[i])) return fail(VO_ERR_INVALID, "non-finite coordinate");
+    int ndev = 0;
+    VO_HIP(hipGetDeviceCount(&ndev));
+    if (hip_device < 0 || hip_device >= ndev) return fail(VO_ERR_INVALID, "no such HIP device");
+    VO_HIP(hipSetDevice(hip_device));
+    vo_tri* t = new vo_tri();
+    t->dev = hip_device;
+    t->D = dim;
+    t->M = dim + 1;
+    t->S = dim == 3 ? 4 : 2;
+    int rc = 0;
+    auto bail = [&](int code) {
+        free_all(t);
+        delete t;
+        return code;
+    };
+    if (hipStreamCreateWithFlags(&t->s, hipStreamNonBlocking) != hipSuccess) return bail(fail(VO_ERR_HIP, "stream"));
+    for (auto& e : t->ev)
+        if (hipEventCreate(&e) != hipSuccess) return bail(fail(VO_ERR_HIP, "event"));
+    if (n > 0) {
+        bounding_sphere(dim, pts, n, t->c, &t->r);
+    } else {
+        t->r = 1.0;
+    }
+    t->R = t->r * 10.0;
+    super_vertices(dim, t->c, t->R, t->sv);
+    // positive orientation of the super simplex (well conditioned: plain f64 suffices)
+    if (dim == 3) {
+        double u[3], v[3], w[3];
+        for (int k = 0; k < 3; ++k) {
+            u[k] = t->sv[1][k] - t->sv[0][k];
+            v[k] = t->sv[2][k] - t->sv[0][k];
+            w[k] = t->sv[3][k] - t->sv[0][k];
+        }
+        const double det = u[0] * (v[1] * w[2] - v[2] * w[1]) - u[1] * (v[0] * w[2] - v[2] * w[0]) +
+                           u[2] * (v[0] * w[1] - v[1] * w[0]);
+        if (det < 0) { t->tet0[1] = 2; t->tet0[2] = 1; }
+    } else {
+        const double det = (t->sv[1][0] - t->sv[0][0]) * (t->sv[2][1] - t->sv[0][1]) -
+                           (t->sv[1][1] - t->sv[0][1]) * (t->sv[2][0] - t->sv[0][0]);
+        if (det < 0) { t->tet0[1] = 2; t->tet0[2] = 1; }
+        t->tet0[3] = -1;
+    }
+    // windows: capacity and per-point cavity buffers
+    t->win_cap = dim == 3 ? (1 << 20) : (1 << 21);
+    t->kcap = dim == 3 ? 256 : 96;
+    t->fcap = dim == 3 ? 384 : 128;
+    t->qcap = 1 << 20;
+    t->cmask = (1u << 22) - 1;
+    t->ws_threads = 1024;
+    t->ws_cap = 1 << 14;
+    t->gamma = dim == 3 ? 0.2 : 0.6;
+    if ((rc = dev_alloc(&t->win, t->win_cap)) || (rc = dev_alloc(&t->pend[0], t->win_cap)) ||
+        (rc = dev_alloc(&t->pend[1], t->win_cap)) || (rc = dev_alloc(&t->cav, (size_t)t->win_cap * t->kcap)) ||
+        (rc = dev_alloc(&t->fac, (size_t)t->win_cap * t->fcap)) || (rc = dev_alloc(&t->nk, t->win_cap)) ||
+        (rc = dev_alloc(&t->nf, t->win_cap)) || (rc = dev_alloc(&t->status, t->win_cap)) ||
+        (rc = dev_alloc(&t->st, 1)) || (rc = dev_alloc(&t->cnt, 1)) || (rc = dev_alloc(&t->q, t->qcap)) ||
+        (rc = dev_alloc(&t->cache, (size_t)t->cmask + 1)) || (rc = dev_alloc(&t->ws, (size_t)t->ws_threads * t->ws_cap)))
+        return bail(rc);
+    if (hipHostMalloc((void**)&t->st_host, sizeof(RoundState)) != hipSuccess) return bail(fail(VO_ERR_OOM, "pinned"));
+    hipMemsetAsync(t->st, 0, sizeof(RoundState), t->s);
+    hipMemsetAsync(t->cnt, 0, sizeof(Counters), t->s);
+    hipMemsetAsync(t->cache, 0xFF, sizeof(CEntry) * ((size_t)t->cmask + 1), t->s);
+    if ((rc = grow_points(t, std::max<int64_t>(n, 16)))) return bail(rc);
+    if ((rc = upload_super(t))) return bail(rc);
+    if ((rc = grow_tets(t, 1024))) return bail(rc);
+    hipLaunchKernelGGL(k_init_first, dim3(1), dim3(1), 0, t->s, t->tv, t->ta, t->tet0[0], t->tet0[1], t->tet0[2],
+                       t->tet0[3]);
+    RoundState init = {};
+    init.n_tets = 1;
+    if (hipMemcpyAsync(t->st, &init, sizeof(init), hipMemcpyHostToDevice, t->s) != hipSuccess ||
+        hipStreamSynchronize(t->s) != hipSuccess)
+        return bail(fail(VO_ERR_HIP, "init"));
+    *out = t;
+    return 0;
+}
+
+void vo_tri_destroy(vo_tri* t) {
+    if (!t) return;
+    hipSetDevice(t->dev);
+    free_all(t);
+    delete t;
+}
+
+int vo_tri_stage(vo_tri* t, const double* pts, int64_t n, uint64_t seed) {
+    if (!t) return fail(VO_ERR_INVALID, "null handle");
+    if (n < 0 || (n > 0 && !pts)) return fail(VO_ERR_INVALID, "bad point buffer");
+    if (n == 0) return 0;
+    const int D = t->D, S = t->S, M = t->M;
+    for (int64_t i = 0; i < n * D; ++i)
+        if (!std::isfinite(pts[i])) return fail(VO_ERR_INVALID, "non-finite coordinate");
+    VO_HIP(hipSetDevice(t->dev));
+    int rc;
+    if ((rc = grow_points(t, t->n_pts + n))) return rc;
+    // priority order: Fisher-Yates over splitmix64(seed)
+    std::vector<int32_t> perm(n);
+    for (int64_t i = 0; i < n; ++i) perm[i] = (int32_t)i;
+    uint64_t sm = seed;
+    for (int64_t i = n - 1; i > 0; --i) {
+        const uint64_t j = (uint64_t)(((unsigned __int128)splitmix_next(sm) * (uint64_t)(i + 1)) >> 64);
+        std::swap(perm[i], perm[j]);
+    }
+    std::vector<double> x64((size_t)n * S, 0.0);
+    std::vector<float> x32((size_t)n * S, 0.0f);
+    std::vector<int> vm(n);
+    const int64_t base_in = (int64_t)t->input_of.size();
+    t->input_of.resize(base_in + n);
+    for (int64_t i = 0; i < n; ++i) {
+        const int64_t src = perm[i];
+        for (int k = 0; k < D; ++k) {
+            const double x = pts[src * D + k];
+            x64[i * S + k] = x;
+            x32[i * S + k] = (float)(x - t->c[k]);
+        }
+        t->input_of[base_in + i] = (int32_t)(base_in + src);
+        vm[i] = (int)(2 * M + base_in + src);
+    }
+    const int64_t v0 = M + t->n_pts;
+    VO_HIP(hipMemcpyAsync(t->X64 + v0 * S, x64.data(), sizeof(double) * x64.size(), hipMemcpyHostToDevice, t->s));
+    VO_HIP(hipMemcpyAsync(t->X32 + v0 * S, x32.data(), sizeof(float) * x32.size(), hipMemcpyHostToDevice, t->s));
+    VO_HIP(hipMemcpyAsync(t->vmap + v0, vm.data(), sizeof(int) * n, hipMemcpyHostToDevice, t->s));
+    VO_HIP(hipMemsetAsync(t->cp + t->n_pts, 0xFF, sizeof(int) * n, t->s));
+    t->n_pts += n;
+    // walk-start hints: a uniform grid over the bounding sphere's box, about 4 points a cell
+    if (!t->hint) {
+        int g = (int)std::ceil(std::pow(std::max<double>((double)t->n_pts / 4.0, 1.0), 1.0 / D));
+        g = std::max(1, std::min(g, D == 3 ? 256 : 4096));
+        int64_t cells = 1;
+        for (int k = 0; k < 3; ++k) {
+            t->gn[k] = k < D ? g : 1;
+            cells *= t->gn[k];
+            t->glo[k] = (float)(-t->r);
+        }
+        t->ginv = (float)(g / (2.0 * t->r));
+        if ((rc = dev_alloc(&t->hint, cells))) return rc;
+        VO_HIP(hipMemsetAsync(t->hint, 0, sizeof(int) * cells, t->s));
+    }
+    // simplex capacity for everything staged (uniform data creates ~27 (3D) / 6 (2D) per point)
+    const int64_t want = (int64_t)t->n_tets + (D == 3 ? 34 : 8) * (t->n_pts - t->inserted - t->dups) + 4096;
+    if ((rc = grow_tets(t, want))) return rc;
+    VO_HIP(hipStreamSynchronize(t->s));
+    return 0;
+}
+
+int vo_tri_run(vo_tri* t) {
+    if (!t) return fail(VO_ERR_INVALID, "null handle");
+    VO_HIP(hipSetDevice(t->dev));
+    const int D = t->D;
+    t->ms_cav = t->ms_com = t->ms_ex = t->ms_other = 0;
+    t->n_cav = t->n_com = 0;
+    const int64_t dups0 = t->dups;
+    VO_HIP(hipEventRecord(t->ev[0], t->s));
+    int rc = 0;
+    int stalled = 0;
+    while (t->head < t->n_pts || t->npend > 0) {
+        // window plan
+        const double target = std::max(8.0, t->gamma * (double)std::max<int64_t>(t->inserted, 1));
+        int nwin = (int)std::min<double>(target, t->win_cap);
+        nwin = std::max(nwin, t->npend);
+        const int nfresh = (int)std::min<int64_t>(t->n_pts - t->head, nwin - t->npend);
+        nwin = t->npend + nfresh;
+        const int64_t need = (int64_t)t->n_tets + (int64_t)nwin * (D == 3 ? 64 : 16) + 1024;
+        if (need > t->tcap && (rc = grow_tets(t, std::max<int64_t>(need, t->tcap * 3 / 2)))) return rc;
+        const int grid = (nwin + 255) / 256;
+        hipLaunchKernelGGL(k_window, dim3(grid), dim3(256), 0, t->s, t->win, t->pend[t->cur], t->npend, (int)t->head,
+                           nfresh);
+        hipLaunchKernelGGL(k_reset_round, dim3(1), dim3(1), 0, t->s, t->st);
+        const unsigned long long key_hi = (unsigned long long)(0xFFFFFFFFu - (uint32_t)t->rounds) << 32;
+        Ctx c = make_ctx(t);
+        VO_HIP(hipEventRecord(t->ev[2], t->s));
+        if (D == 3) hipLaunchKernelGGL(k_cavity<3>, dim3(grid), dim3(256), 0, t->s, c, nwin, key_hi);
+        else hipLaunchKernelGGL(k_cavity<2>, dim3(grid), dim3(256), 0, t->s, c, nwin, key_hi);
+        VO_HIP(hipGetLastError());
+        VO_HIP(hipEventRecord(t->ev[3], t->s));
+        if (D == 3) hipLaunchKernelGGL(k_commit<3>, dim3(grid), dim3(256), 0, t->s, c, nwin, key_hi);
+        else hipLaunchKernelGGL(k_commit<2>, dim3(grid), dim3(256), 0, t->s, c, nwin, key_hi);
+        VO_HIP(hipGetLastError());
+        VO_HIP(hipEventRecord(t->ev[1], t->s));
+        if ((rc = sync_state(t))) return rc;
+        float a = 0, b = 0;
+        hipEventElapsedTime(&a, t->ev[2], t->ev[3]);
+        hipEventElapsedTime(&b, t->ev[3], t->ev[1]);
+        t->ms_cav += a;
+        t->ms_com += b;
+        t->n_cav++;
+        t->n_com++;
+        if ((rc = check_err(t))) return rc;
+        const RoundState& r = *t->st_host;
+        if (r.n_q > 0) {
+            if ((rc = run_exact(t, r.n_q))) return rc;
+            if ((rc = sync_state(t)) || (rc = check_err(t))) return rc;
+        }
+        t->n_tets = r.n_tets;
+        t->n_live += r.n_live_delta;
+        t->inserted += r.n_winners;
+        t->windows_total += nwin;
+        const int64_t consumed = (int64_t)nwin - r.n_next_pending - r.n_winners;   // duplicates
+        t->dups += consumed;
+        const double frac = nwin ? (double)r.n_winners / nwin : 1.0;
+        if (frac > 0.6) t->gamma = std::min(t->gamma * 1.5, 4.0);
+        else if (frac < 0.35) t->gamma = std::max(t->gamma * 0.7, 0.02);
+        // every round either inserts, drops a duplicate, or resolves queued predicates
+        stalled = (r.n_winners == 0 && consumed == 0 && r.n_q == 0) ? stalled + 1 : 0;
+        if (stalled > 64) return fail(VO_ERR_INTERNAL, "no progress for 64 rounds");
+        t->npend = r.n_next_pending;
+        t->cur ^= 1;
+        t->head += nfresh;
+        t->rounds++;
+    }
+    VO_HIP(hipEventRecord(t->ev[1], t->s));
+    VO_HIP(hipEventSynchronize(t->ev[1]));
+    float ms = 0;
+    hipEventElapsedTime(&ms, t->ev[0], t->ev[1]);
+    t->ms_run = ms;
+    t->ms_other = ms - t->ms_cav - t->ms_com - t->ms_ex;
+    if (t->dups > dups0) return fail(VO_ERR_DUPLICATE, std::to_string(t->dups - dups0) + " duplicate points skipped");
+    return 0;
+}
+
+int vo_tri_insert(vo_tri* t, const double* pts, int64_t n, uint64_t seed) {
+    int rc = vo_tri_stage(t, pts, n, seed);
+    return rc ? rc : vo_tri_run(t);
+}
+
+int vo_tri_counts(const vo_tri* t, int64_t* n_vertices, int64_t* n_simplices, int64_t* max_simplex_id) {
+    if (!t) return fail(VO_ERR_INVALID, "null handle");
+    if (n_vertices) *n_vertices = t->M + t->inserted;
+    if (n_simplices) *n_simplices = t->n_live;
+    if (max_simplex_id) *max_simplex_id = (int64_t)t->n_tets - 1 + t->M;
+    return 0;
+}
+
+static int export_device(const vo_tri* tc, int32_t** d_simp, int32_t** d_nbr, int64_t* n_live_out) {
+    vo_tri* t = const_cast<vo_tri*>(tc);
+    const int n = t->n_tets;
+    const int nb = (n + 255) / 256;
+    int *flag, *pos, *bsum;
+    int rc;
+    if ((rc = dev_alloc(&flag, n)) || (rc = dev_alloc(&pos, n)) || (rc = dev_alloc(&bsum, nb))) return rc;
+    hipLaunchKernelGGL(k_live_flags, dim3(nb), dim3(256), 0, t->s, t->tv, n, flag);
+    hipLaunchKernelGGL(k_block_scan, dim3(nb), dim3(256), 0, t->s, flag, pos, bsum, n);
+    std::vector<int> hb(nb);
+    VO_HIP(hipMemcpyAsync(hb.data(), bsum, sizeof(int) * nb, hipMemcpyDeviceToHost, t->s));
+    VO_HIP(hipStreamSynchronize(t->s));
+    int64_t acc = 0;
+    for (int i = 0; i < nb; ++i) { int v = hb[i]; hb[i] = (int)acc; acc += v; }
+    VO_HIP(hipMemcpyAsync(bsum, hb.data(), sizeof(int) * nb, hipMemcpyHostToDevice, t->s));
+    hipLaunchKernelGGL(k_add_offsets, dim3(nb), dim3(256), 0, t->s, pos, bsum, n);
+    int32_t *simp = nullptr, *nbr = nullptr;
+    const int M = t->M;
+    if ((rc = dev_alloc(&simp, acc * M)) || (rc = dev_alloc(&nbr, acc * M))) return rc;
+    if (t->D == 3)
+        hipLaunchKernelGGL(k_export<3>, dim3(nb), dim3(256), 0, t->s, t->tv, t->ta, pos, n, t->vmap, simp, nbr);
+    else
+        hipLaunchKernelGGL(k_export<2>, dim3(nb), dim3(256), 0, t->s, t->tv, t->ta, pos, n, t->vmap, simp, nbr);
+    VO_HIP(hipGetLastError());
+    VO_HIP(hipStreamSynchronize(t->s));
+    hipFree(flag); hipFree(pos); hipFree(bsum);
+    *d_simp = simp;
+    *d_nbr = nbr;
+    *n_live_out = acc;
+    return 0;
+}
+
+int vo_tri_export(const vo_tri* t, int32_t* simplices, int32_t* neighbors) {
+    if (!t) return fail(VO_ERR_INVALID, "null handle");
+    VO_HIP(hipSetDevice(t->dev));
+    int32_t *ds, *dn;
+    int64_t nl;
+    int rc = export_device(t, &ds, &dn, &nl);
+    if (rc) return rc;
+    const size_t bytes = sizeof(int32_t) * (size_t)nl * t->M;
+    if (simplices) VO_HIP(hipMemcpy(simplices, ds, bytes, hipMemcpyDeviceToHost));
+    if (neighbors) VO_HIP(hipMemcpy(neighbors, dn, bytes, hipMemcpyDeviceToHost));
+    hipFree(ds);
+    hipFree(dn);
+    return 0;
+}
+
+int vo_tri_canonical(const vo_tri* t, int32_t** T, int64_t* n_T, int32_t** E, int64_t* n_E) {
+    if (!t || !T || !n_T || !E || !n_E) return fail(VO_ERR_INVALID, "null argument");
+    VO_HIP(hipSetDevice(t->dev));
+    int32_t *ds, *dn;
+    int64_t nl;
+    int rc = export_device(t, &ds, &dn, &nl);
+    if (rc) return rc;
+    const int M = t->M;
+    std::vector<int32_t> s((size_t)nl * M);
+    VO_HIP(hipMemcpy(s.data(), ds, sizeof(int32_t) * s.size(), hipMemcpyDeviceToHost));
+    hipFree(ds);
+    hipFree(dn);
+    const int32_t first = 2 * M;
+    std::vector<std::array<int32_t, 4>> tris;
+    std::vector<std::array<int32_t, 2>> edges;
+    tris.reserve(nl);
+    edges.reserve((size_t)nl * (M == 4 ? 6 : 3));
+    for (int64_t i = 0; i < nl; ++i) {
+        std::array<int32_t, 4> r = {INT32_MAX, INT32_MAX, INT32_MAX, INT32_MAX};
+        bool real = true;
+        for (int k = 0; k < M; ++k) {
+            const int32_t v = s[i * M + k];
+            r[k] = v >= first ? v - first : -1;
+            real &= v >= first;
+        }
+        for (int a = 0; a < M; ++a)
+            for (int b = a + 1; b < M; ++b)
+                if (r[a] >= 0 && r[b] >= 0) edges.push_back({std::min(r[a], r[b]), std::max(r[a], r[b])});
+        if (real) {
+            std::sort(r.begin(), r.begin() + M);
+            tris.push_back(r);
+        }
+    }
+    std::sort(tris.begin(), tris.end());
+    std::sort(edges.begin(), edges.end());
+    edges.erase(std::unique(edges.begin(), edges.end()), edges.end());
+    *T = (int32_t*)std::malloc(sizeof(int32_t) * std::max<size_t>(1, tris.size() * M));
+    *E = (int32_t*)std::malloc(sizeof(int32_t) * std::max<size_t>(1, edges.size() * 2));
+    if (!*T || !*E) return fail(VO_ERR_OOM, "host allocation");
+    for (size_t i = 0; i < tris.size(); ++i)
+        for (int k = 0; k < M; ++k) (*T)[i * M + k] = tris[i][k];
+    for (size_t i = 0; i < edges.size(); ++i) { (*E)[2 * i] = edges[i][0]; (*E)[2 * i + 1] = edges[i][1]; }
+    *n_T = (int64_t)tris.size();
+    *n_E = (int64_t)edges.size();
+    return 0;
+}
+
+int vo_tri_check(vo_tri* t, int64_t* n_violations) {
+    if (!t || !n_violations) return fail(VO_ERR_INVALID, "null argument");
+    VO_HIP(hipSetDevice(t->dev));
+    unsigned long long* bad;
+    int* unres;
+    int rc;
+    if ((rc = dev_alloc(&bad, 1)) || (rc = dev_alloc(&unres, 1))) return rc;
+    const int n = t->n_tets;
+    for (int attempt = 0; attempt < 16; ++attempt) {
+        VO_HIP(hipMemsetAsync(bad, 0, sizeof(unsigned long long), t->s));
+        VO_HIP(hipMemsetAsync(unres, 0, sizeof(int), t->s));
+        hipLaunchKernelGGL(k_reset_round, dim3(1), dim3(1), 0, t->s, t->st);
+        Ctx c = make_ctx(t);
+        if (t->D == 3) hipLaunchKernelGGL(k_check<3>, dim3((n + 255) / 256), dim3(256), 0, t->s, c, n, bad, unres);
+        else hipLaunchKernelGGL(k_check<2>, dim3((n + 255) / 256), dim3(256), 0, t->s, c, n, bad, unres);
+        VO_HIP(hipGetLastError());
+        unsigned long long hb = 0;
+        int hu = 0;
+        VO_HIP(hipMemcpyAsync(&hb, bad, sizeof(hb), hipMemcpyDeviceToHost, t->s));
+        VO_HIP(hipMemcpyAsync(&hu, unres, sizeof(hu), hipMemcpyDeviceToHost, t->s));
+        if ((rc = sync_state(t))) return rc;
+        if (hu == 0) {
+            hipFree(bad);
+            hipFree(unres);
+            *n_violations = (int64_t)hb;
+            return check_err(t);
+        }
+        if ((rc = run_exact(t, t->st_host->n_q))) return rc;
+    }
+    hipFree(bad);
+    hipFree(unres);
+    return fail(VO_ERR_INTERNAL, "check: predicates stayed unresolved");
+}
+
+int vo_tri_stats(const vo_tri* t, vo_stats* out) {
+    if (!t || !out) return fail(VO_ERR_INVALID, "null argument");
+    VO_HIP(hipSetDevice(t->dev));
+    Counters h;
+    VO_HIP(hipMemcpy(&h, t->cnt, sizeof(h), hipMemcpyDeviceToHost));
+    std::memset(out, 0, sizeof(*out));
+    out->rounds = t->rounds;
+    out->points_inserted = t->inserted;
+    out->points_duplicate = t->dups;
+    out->windows_total = t->windows_total;
+    out->winners_total = (int64_t)h.winners;
+    for (int i = 0; i < 2; ++i) {
+        out->tier1[i] = (int64_t)h.t1[i];
+        out->tier2[i] = (int64_t)h.t2[i];
+        out->tier3[i] = (int64_t)h.t3[i];
+        out->deferred[i] = (int64_t)h.def[i];
+    }
+    out->exact_evaluated = (int64_t)h.exact_evaluated;
+    out->walk_steps = (int64_t)h.walk_steps;
+    out->chain_hops = (int64_t)h.chain_hops;
+    out->killed = (int64_t)h.killed;
+    out->created = (int64_t)h.created;
+    out->max_killed = (int64_t)h.max_killed;
+    out->max_facets = (int64_t)h.max_facets;
+    out->ms_run = t->ms_run;
+    out->ms_locate_cavity = t->ms_cav;
+    out->ms_commit = t->ms_com;
+    out->ms_exact = t->ms_ex;
+    out->ms_other = t->ms_other;
+    out->launches_locate_cavity = t->n_cav;
+    out->launches_commit = t->n_com;
+    return 0;
+}
+
+int vo_tri_vertices(const vo_tri* t, double* coords) {
+    if (!t || !coords) return fail(VO_ERR_INVALID, "null argument");
+    VO_HIP(hipSetDevice(t->dev));
+    const int D = t->D, M = t->M, S = t->S;
+    for (int v = 0; v < 2 * M; ++v) {
+        const int src = v < M ? v : (D == 3 ? (v == 2 * M - 1 ? 1 : 0) : v - M);   // ghosts copy super vertices
+        for (int k = 0; k < D; ++k) coords[v * D + k] = t->sv[src][k];
+    }
+    std::vector<double> x((size_t)t->n_pts * S);
+    if (t->n_pts)
+        VO_HIP(hipMemcpy(x.data(), t->X64 + (size_t)M * S, sizeof(double) * x.size(), hipMemcpyDeviceToHost));
+    for (int64_t i = 0; i < t->n_pts; ++i) {
+        const int64_t in = t->input_of[i];
+        for (int k = 0; k < D; ++k) coords[(2 * M + in) * D + k] = x[i * S + k];
+    }
+    return 0;
+}
+
+}  // extern "C"
