@@ -84,6 +84,11 @@ int vo_tri_check(vo_tri* t, int64_t* n_violations);
 
 int vo_tri_stats(const vo_tri* t, vo_stats* out);
 
+/* tuning and test knobs (no counterpart in the reference): "filters" = mask of the FP32 (1)
+ * and FP64-filter (2) predicate tiers, 3 by default; 0 sends every predicate to the exact
+ * expansion tier.  "window_gamma_milli" = initial window size per inserted point x 1000. */
+int vo_tri_config(vo_tri* t, const char* key, int64_t value);
+
 /* input vertex coordinates in the reference id space (for PyVertex.point): (2(dim+1) + n) x dim */
 int vo_tri_vertices(const vo_tri* t, double* coords);
 

@@ -142,8 +142,21 @@ def test_exact_tier_decides_what_fp64_cannot():
         ref = O.Tree.delaunay(P, O.MODE_EXACT).canonical()
         assert np.array_equal(T, ref.T) and np.array_equal(E, ref.E), dim
         assert t.stats()["exact_evaluated"] > 0, dim
-        # the check re-evaluates every face's in-sphere test: undecided ones are queued,
-        # evaluated exactly, then read back from the cache
         assert t.violations() == 0
-        st = t.stats()
-        assert st["tier3"][0] + st["tier3"][1] > 0, (dim, st)
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_exact_tier_alone_builds_the_same_triangulation(dim):
+    # filters off: every predicate is queued, evaluated by the expansion kernel, cached, and
+    # read back from the cache on the point's next attempt
+    P = datasets.uniform(300, dim, seed=80 + dim)
+    t = vo.DelaunayTree(P)
+    t.config("filters", 0)
+    t.add_points_to_tree(P, seed=1)
+    T, E = t.canonical()
+    ref = O.Tree.delaunay(P, O.MODE_EXACT).canonical()
+    assert np.array_equal(T, ref.T) and np.array_equal(E, ref.E)
+    st = t.stats()
+    assert st["tier1"] == [0, 0] and st["tier2"] == [0, 0]
+    assert st["tier3"][0] > 0 and st["tier3"][1] > 0 and st["exact_evaluated"] > 0
+    assert t.violations() == 0

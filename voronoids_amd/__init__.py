@@ -88,6 +88,10 @@ class DelaunayTree:
         except Exception:
             pass
 
+    def config(self, key: str, value: int) -> None:
+        """tuning / test knobs, see vo_tri_config in include/voronoids.h"""
+        _lib.check(_lib.lib().vo_tri_config(self._h, key.encode(), int(value)))
+
     # -- construction -----------------------------------------------------------------
     def add_points_to_tree(self, vertices, seed: int = 0) -> None:
         p = _as_points(vertices, self.dim)

@@ -21,7 +21,7 @@ VO_ERR_INTERNAL = -7
 
 EXPORTED = [
     "vo_tri_create", "vo_tri_destroy", "vo_tri_insert", "vo_tri_stage", "vo_tri_run", "vo_tri_counts",
-    "vo_tri_export", "vo_tri_canonical", "vo_free", "vo_tri_check", "vo_tri_stats", "vo_tri_vertices",
+    "vo_tri_export", "vo_tri_canonical", "vo_free", "vo_tri_check", "vo_tri_stats", "vo_tri_config", "vo_tri_vertices",
     "vo_last_error", "vo_version",
 ]
 
@@ -80,6 +80,7 @@ def lib() -> ctypes.CDLL:
         L.vo_tri_check.argtypes = [P, i64p]
         L.vo_tri_stats.argtypes = [P, ctypes.POINTER(VoStats)]
         L.vo_tri_vertices.argtypes = [P, dp]
+        L.vo_tri_config.argtypes = [P, ctypes.c_char_p, ctypes.c_int64]
         L.vo_last_error.restype = ctypes.c_char_p
         L.vo_version.restype = ctypes.c_int
         _lib = L

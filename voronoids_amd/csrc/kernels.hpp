@@ -64,16 +64,17 @@ struct Ctx {
     int* nk;
     int* nf;
     int* status;
-    int* hint;
-    float glo[3];
-    float ginv;
-    int gn[3];
+    int* hint;             // multi-level grid: level l has 2^l cells per axis, -1 = unset
+    float glo[3];          // grid origin (translated frame)
+    float ginv;            // 1 / grid extent
+    int hlevels;           // finest level
     RoundState* st;
     Counters* cnt;
     QEntry* q;
     int qcap;
     CEntry* cache;
     uint32_t cmask;
+    int filters;           // bit 0: FP32 tier on, bit 1: FP64 filter on (both on in production)
     int M;                 // vertex id of point p is M + p
     int win_cap, kcap, fcap;
     int tet_cap;
@@ -189,20 +190,24 @@ __device__ __forceinline__ void enqueue(const Ctx& c, uint32_t type, const uint3
 template <int D> __device__ __forceinline__ int conflict(const Ctx& c, int4 v, int pv,
                                                          typename VT<D>::F pf, LocalCnt& lc) {
     using V = VT<D>;
-    int s;
-    if constexpr (D == 3) {
-        s = t1_insphere(V::ld(c.X32, v.x), V::ld(c.X32, v.y), V::ld(c.X32, v.z), V::ld(c.X32, v.w), pf);
-    } else {
-        s = t1_incircle(V::ld(c.X32, v.x), V::ld(c.X32, v.y), V::ld(c.X32, v.z), pf);
+    int s = SIGN_UNDECIDED;
+    if (c.filters & 1) {
+        if constexpr (D == 3) {
+            s = t1_insphere(V::ld(c.X32, v.x), V::ld(c.X32, v.y), V::ld(c.X32, v.z), V::ld(c.X32, v.w), pf);
+        } else {
+            s = t1_incircle(V::ld(c.X32, v.x), V::ld(c.X32, v.y), V::ld(c.X32, v.z), pf);
+        }
+        if (s != SIGN_UNDECIDED) { lc.t1[1]++; return s; }
     }
-    if (s != SIGN_UNDECIDED) { lc.t1[1]++; return s; }
     const double* X = c.X64;
     constexpr int S = V::S;
-    if (D == 3)
-        s = t2_insphere(X + S * v.x, X + S * v.y, X + S * v.z, X + S * v.w, X + S * pv);
-    else
-        s = t2_incircle(X + S * v.x, X + S * v.y, X + S * v.z, X + S * pv);
-    if (s != SIGN_UNDECIDED) { lc.t2[1]++; return s; }
+    if (c.filters & 2) {
+        if (D == 3)
+            s = t2_insphere(X + S * v.x, X + S * v.y, X + S * v.z, X + S * v.w, X + S * pv);
+        else
+            s = t2_incircle(X + S * v.x, X + S * v.y, X + S * v.z, X + S * pv);
+        if (s != SIGN_UNDECIDED) { lc.t2[1]++; return s; }
+    }
     uint32_t ids[5];
     uint32_t type = D == 3 ? PRED_INSPHERE : PRED_INCIRCLE;
     if (D == 3) { ids[0] = v.x; ids[1] = v.y; ids[2] = v.z; ids[3] = v.w; ids[4] = pv; }
@@ -222,15 +227,19 @@ template <int D> __device__ __forceinline__ int orient_sub(const Ctx& c, int4 v,
     int ids[4] = {v.x, v.y, v.z, v.w};
     ids[k] = pv;
     for (int i = 0; i < D + 1; ++i) a[i] = (i == k) ? pf : V::ld(c.X32, ids[i]);
-    int s;
-    if constexpr (D == 3) s = t1_orient3d(a[0], a[1], a[2], a[3]);
-    else s = t1_orient2d(a[0], a[1], a[2]);
-    if (s != SIGN_UNDECIDED) { lc.t1[0]++; return s; }
+    int s = SIGN_UNDECIDED;
+    if (c.filters & 1) {
+        if constexpr (D == 3) s = t1_orient3d(a[0], a[1], a[2], a[3]);
+        else s = t1_orient2d(a[0], a[1], a[2]);
+        if (s != SIGN_UNDECIDED) { lc.t1[0]++; return s; }
+    }
     const double* X = c.X64;
     constexpr int S = V::S;
-    if (D == 3) s = t2_orient3d(X + S * ids[0], X + S * ids[1], X + S * ids[2], X + S * ids[3]);
-    else s = t2_orient2d(X + S * ids[0], X + S * ids[1], X + S * ids[2]);
-    if (s != SIGN_UNDECIDED) { lc.t2[0]++; return s; }
+    if (c.filters & 2) {
+        if (D == 3) s = t2_orient3d(X + S * ids[0], X + S * ids[1], X + S * ids[2], X + S * ids[3]);
+        else s = t2_orient2d(X + S * ids[0], X + S * ids[1], X + S * ids[2]);
+        if (s != SIGN_UNDECIDED) { lc.t2[0]++; return s; }
+    }
     uint32_t u[5] = {(uint32_t)ids[0], (uint32_t)ids[1], (uint32_t)ids[2], D == 3 ? (uint32_t)ids[3] : kEmpty, kEmpty};
     uint32_t type = D == 3 ? PRED_ORIENT3 : PRED_ORIENT2;
     s = cache_lookup(c.cache, c.cmask, type, u);
@@ -240,12 +249,30 @@ template <int D> __device__ __forceinline__ int orient_sub(const Ctx& c, int4 v,
     return SIGN_UNDECIDED;
 }
 
-template <int D> __device__ __forceinline__ int grid_cell(const Ctx& c, typename VT<D>::F pf) {
-    int ix = (int)fminf(fmaxf((pf.x - c.glo[0]) * c.ginv, 0.0f), (float)(c.gn[0] - 1));
-    int iy = (int)fminf(fmaxf((pf.y - c.glo[1]) * c.ginv, 0.0f), (float)(c.gn[1] - 1));
+// cell of pf at level l of the hint grid (levels stored coarse to fine, one after another)
+template <int D> __device__ __forceinline__ int hint_cell(const Ctx& c, typename VT<D>::F pf, int l) {
+    const int g = 1 << l;
+    const float fg = (float)g;
+    const int off = D == 3 ? (((1 << (3 * l)) - 1) / 7) : (((1 << (2 * l)) - 1) / 3);
+    int ix = (int)fminf(fmaxf((pf.x - c.glo[0]) * c.ginv * fg, 0.0f), fg - 1.0f);
+    int iy = (int)fminf(fmaxf((pf.y - c.glo[1]) * c.ginv * fg, 0.0f), fg - 1.0f);
     int iz = 0;
-    if constexpr (D == 3) iz = (int)fminf(fmaxf((pf.z - c.glo[2]) * c.ginv, 0.0f), (float)(c.gn[2] - 1));
-    return (iz * c.gn[1] + iy) * c.gn[0] + ix;
+    if constexpr (D == 3) iz = (int)fminf(fmaxf((pf.z - c.glo[2]) * c.ginv * fg, 0.0f), fg - 1.0f);
+    return off + (iz * g + iy) * g + ix;
+}
+
+// walk start for a fresh point: the finest level whose cell has been visited
+template <int D> __device__ __forceinline__ int hint_lookup(const Ctx& c, typename VT<D>::F pf) {
+    int best = 0;
+    for (int l = 0; l <= c.hlevels; ++l) {
+        const int h = c.hint[hint_cell<D>(c, pf, l)];
+        if (h >= 0) best = h;
+    }
+    return best;
+}
+
+template <int D> __device__ __forceinline__ void hint_store(const Ctx& c, typename VT<D>::F pf, int t) {
+    for (int l = 1; l <= c.hlevels; ++l) c.hint[hint_cell<D>(c, pf, l)] = t;
 }
 
 __device__ __forceinline__ uint32_t mix32(uint32_t x) {
@@ -261,7 +288,10 @@ template <int D>
 __global__ void __launch_bounds__(256) k_cavity(Ctx c, int nwin, unsigned long long key_hi) {
     using V = VT<D>;
     constexpr int M = D + 1;
+    constexpr int KL = 32;                       // cavity entries mirrored in LDS
+    __shared__ int s_cav[KL][256];
     const int slot = blockIdx.x * blockDim.x + threadIdx.x;
+    const int lt = threadIdx.x;
     LocalCnt lc = {};
     if (slot < nwin) {
         const int p = c.win[slot];
@@ -269,9 +299,8 @@ __global__ void __launch_bounds__(256) k_cavity(Ctx c, int nwin, unsigned long l
         const typename V::F pf = V::ld(c.X32, pv);
         const int W = c.win_cap;
         int st = ST_OK;
-        const int cell = grid_cell<D>(c, pf);
         int t = c.cp[p];
-        if (t < 0) t = c.hint[cell];
+        if (t < 0) t = hint_lookup<D>(c, pf);
         int4 tt = c.tv[t];
         for (int hops = 0; tt.x < 0; ++hops) {
             if (hops > kChainLimit) { atomicOr(&c.st->err, ERR_CHAIN); st = ST_BROKEN; break; }
@@ -308,7 +337,7 @@ __global__ void __launch_bounds__(256) k_cavity(Ctx c, int nwin, unsigned long l
             if (!moved) break;
         }
         c.cp[p] = t;
-        c.hint[cell] = t;
+        hint_store<D>(c, pf, t);
         int nk = 0, nf = 0;
         if (st == ST_OK) {
             const int s0 = conflict<D>(c, tt, pv, pf, lc);
@@ -317,22 +346,26 @@ __global__ void __launch_bounds__(256) k_cavity(Ctx c, int nwin, unsigned long l
         }
         if (st == ST_OK) {
             c.cav[slot] = t;
+            s_cav[0][lt] = t;
             nk = 1;
             for (int qi = 0; qi < nk && st == ST_OK; ++qi) {
-                const int ct = c.cav[qi * W + slot];
+                const int ct = qi < KL ? s_cav[qi][lt] : c.cav[qi * W + slot];
                 const int4 aa = c.ta[ct];
                 for (int f = 0; f < M; ++f) {
                     const int nb = comp(aa, f);
                     if (nb >= 0) {
                         const int n = nb >> 2;
                         bool seen = false;
-                        for (int j = 0; j < nk && !seen; ++j) seen = c.cav[j * W + slot] == n;
+                        const int nl = nk < KL ? nk : KL;
+                        for (int j = 0; j < nl && !seen; ++j) seen = s_cav[j][lt] == n;
+                        for (int j = KL; j < nk && !seen; ++j) seen = c.cav[j * W + slot] == n;
                         if (seen) continue;
                         const int s = conflict<D>(c, c.tv[n], pv, pf, lc);
                         if (s == SIGN_UNDECIDED) { st = ST_DEFER; break; }
                         if (s > 0) {
                             if (nk == c.kcap) { st = ST_OVERFLOW; break; }
                             c.cav[nk * W + slot] = n;
+                            if (nk < KL) s_cav[nk][lt] = n;
                             nk++;
                             continue;
                         }
@@ -410,41 +443,48 @@ __global__ void __launch_bounds__(256) k_commit(Ctx c, int nwin, unsigned long l
     }
     if (win) {
         const int pv = c.M + p;
+        // phase 1: the new simplex of facet j takes slot base + j; its face 0 faces the outside
+        // neighbor, which is patched to point back; the cavity side of the facet is overwritten
+        // with the marker -2 - (base + j) so that ridge rotations find it without a search
+        for (int j = 0; j < nf; ++j) {
+            const int fe = c.fac[j * W + slot];
+            const int t = fe >> 2, i = fe & 3;
+            int* slot_ptr = comp_ptr(&c.ta[t], i);
+            const int outside = *slot_ptr;
+            const int N = base + j;
+            *comp_ptr(&c.ta[N], 0) = outside;
+            if (outside >= 0) *comp_ptr(&c.ta[outside >> 2], outside & 3) = N << 2;
+            *slot_ptr = -2 - N;
+        }
+        // phase 2: internal faces.  Face k (k >= 1) of the new simplex contains p and the ridge
+        // nv \ {p, nv[k]}; rotating around that ridge through the cavity reaches the other
+        // boundary facet containing it, whose marker names the neighbor.
         for (int j = 0; j < nf; ++j) {
             const int fe = c.fac[j * W + slot];
             const int t = fe >> 2, i = fe & 3;
             const int4 tvt = c.tv[t];
             const int4 tat = c.ta[t];
             const int4 nv = replace_first<D>(tvt, i, pv);
-            int4 na = make_int4(-1, -1, -1, -1);
-            const int outside = comp(tat, i);
-            na.x = outside;
             const int N = base + j;
-            if (outside >= 0) *comp_ptr(&c.ta[outside >> 2], outside & 3) = N << 2;
             for (int k = 1; k < M; ++k) {
-                // ridge = nv minus {p, nv[k]}; rotate around it through the cavity
                 const int cv = comp(nv, k);
-                int T = t;
                 int4 TV = tvt, TA = tat;
                 int e = index_of<D>(TV, cv);
                 int y = comp(tvt, i);
-                for (int guard = 0;; ++guard) {
-                    const int nbe = comp(TA, e);
-                    if (nbe < 0 || c.kres[nbe >> 2] != key) break;
+                int nbe = comp(TA, e);
+                for (int guard = 0; nbe >= 0; ++guard) {
                     if (guard > 4096) { atomicOr(&c.st->err, ERR_WALK); break; }
                     const int T2 = nbe >> 2;
                     const int4 TV2 = c.tv[T2];
                     const int w = comp(TV2, nbe & 3);
                     e = index_of<D>(TV2, y);
                     y = w;
-                    T = T2;
                     TV = TV2;
                     TA = c.ta[T2];
+                    nbe = comp(TA, e);
                 }
-                const int target = (T << 2) | e;
-                int jj = -1;
-                for (int q = 0; q < nf; ++q)
-                    if (c.fac[q * W + slot] == target) { jj = q; break; }
+                if (nbe > -2) { atomicOr(&c.st->err, ERR_WALK); continue; }
+                const int N2 = -2 - nbe;
                 const int4 nv2 = replace_first<D>(TV, e, pv);
                 int kk = 1;
                 for (int m = 1; m < M; ++m) {
@@ -454,12 +494,12 @@ __global__ void __launch_bounds__(256) k_commit(Ctx c, int nwin, unsigned long l
                         if (r != k && comp(nv, r) == x) in_ridge = true;
                     if (!in_ridge) { kk = m; break; }
                 }
-                set_comp(na, k, jj < 0 ? -1 : (((base + jj) << 2) | kk));
-                if (jj < 0) atomicOr(&c.st->err, ERR_WALK);
+                *comp_ptr(&c.ta[N], k) = (N2 << 2) | kk;
             }
+            if (D == 2) *comp_ptr(&c.ta[N], 3) = -1;
             c.tv[N] = nv;
-            c.ta[N] = na;
         }
+        // phase 3: killed simplices die with a forward pointer into the star
         for (int j = 0; j < nk; ++j) c.tv[c.cav[j * W + slot]] = make_int4(-1, base, 0, 0);
         for (int j = 0; j < nf; ++j) c.tv[c.fac[j * W + slot] >> 2].y = base + j;
         lc.killed = nk;

@@ -135,7 +135,7 @@ struct vo_tri {
     int* status = nullptr;
     // hints
     int* hint = nullptr;
-    int gn[3] = {1, 1, 1};
+    int hlevels = 0;
     float glo[3] = {0, 0, 0};
     float ginv = 1;
     // control
@@ -150,6 +150,7 @@ struct vo_tri {
     int ws_cap = 0, ws_threads = 0;
     hipEvent_t ev[6] = {};
     double gamma = 0.25;
+    int filters = 3;
     int64_t inserted = 0, dups = 0, rounds = 0, windows_total = 0;
     double ms_run = 0, ms_cav = 0, ms_com = 0, ms_ex = 0, ms_other = 0;
     int64_t n_cav = 0, n_com = 0;
@@ -175,8 +176,9 @@ Ctx make_ctx(vo_tri* t) {
     c.nf = t->nf;
     c.status = t->status;
     c.hint = t->hint;
-    for (int i = 0; i < 3; ++i) { c.glo[i] = t->glo[i]; c.gn[i] = t->gn[i]; }
+    for (int i = 0; i < 3; ++i) c.glo[i] = t->glo[i];
     c.ginv = t->ginv;
+    c.hlevels = t->hlevels;
     c.st = t->st;
     c.cnt = t->cnt;
     c.q = t->q;
@@ -184,6 +186,7 @@ Ctx make_ctx(vo_tri* t) {
     c.cache = t->cache;
     c.cmask = t->cmask;
     c.M = t->M;
+    c.filters = t->filters;
     c.win_cap = t->win_cap;
     c.kcap = t->kcap;
     c.fcap = t->fcap;
@@ -457,19 +460,19 @@ int vo_tri_stage(vo_tri* t, const double* pts, int64_t n, uint64_t seed) {
     VO_HIP(hipMemcpyAsync(t->vmap + v0, vm.data(), sizeof(int) * n, hipMemcpyHostToDevice, t->s));
     VO_HIP(hipMemsetAsync(t->cp + t->n_pts, 0xFF, sizeof(int) * n, t->s));
     t->n_pts += n;
-    // walk-start hints: a uniform grid over the bounding sphere's box, about 4 points a cell
+    // walk-start hints: grids of 2^l cells per axis over the bounding sphere's box, finest
+    // level about 4 points a cell; level 0 holds the initial simplex
     if (!t->hint) {
-        int g = (int)std::ceil(std::pow(std::max<double>((double)t->n_pts / 4.0, 1.0), 1.0 / D));
-        g = std::max(1, std::min(g, D == 3 ? 256 : 4096));
-        int64_t cells = 1;
-        for (int k = 0; k < 3; ++k) {
-            t->gn[k] = k < D ? g : 1;
-            cells *= t->gn[k];
-            t->glo[k] = (float)(-t->r);
-        }
-        t->ginv = (float)(g / (2.0 * t->r));
+        const double target = std::max<double>((double)t->n_pts / 4.0, 1.0);
+        int L = (int)std::floor(std::log2(target) / D + 0.5);
+        L = std::max(0, std::min(L, D == 3 ? 8 : 12));
+        t->hlevels = L;
+        const int64_t cells = D == 3 ? (((int64_t)1 << (3 * (L + 1))) - 1) / 7 : (((int64_t)1 << (2 * (L + 1))) - 1) / 3;
+        for (int k = 0; k < 3; ++k) t->glo[k] = (float)(-t->r);
+        t->ginv = (float)(1.0 / (2.0 * t->r));
         if ((rc = dev_alloc(&t->hint, cells))) return rc;
-        VO_HIP(hipMemsetAsync(t->hint, 0, sizeof(int) * cells, t->s));
+        VO_HIP(hipMemsetAsync(t->hint, 0xFF, sizeof(int) * cells, t->s));
+        VO_HIP(hipMemsetAsync(t->hint, 0, sizeof(int), t->s));
     }
     // simplex capacity for everything staged (uniform data creates ~27 (3D) / 6 (2D) per point)
     const int64_t want = (int64_t)t->n_tets + (D == 3 ? 34 : 8) * (t->n_pts - t->inserted - t->dups) + 4096;
@@ -725,6 +728,21 @@ int vo_tri_stats(const vo_tri* t, vo_stats* out) {
     out->ms_other = t->ms_other;
     out->launches_locate_cavity = t->n_cav;
     out->launches_commit = t->n_com;
+    return 0;
+}
+
+int vo_tri_config(vo_tri* t, const char* key, int64_t value) {
+    if (!t || !key) return fail(VO_ERR_INVALID, "null argument");
+    const std::string k(key);
+    if (k == "filters") {
+        if (value < 0 || value > 3) return fail(VO_ERR_INVALID, "filters must be a mask of 1 (fp32) and 2 (fp64)");
+        t->filters = (int)value;
+    } else if (k == "window_gamma_milli") {
+        if (value <= 0) return fail(VO_ERR_INVALID, "window_gamma_milli must be positive");
+        t->gamma = (double)value / 1000.0;
+    } else {
+        return fail(VO_ERR_INVALID, "unknown option " + k);
+    }
     return 0;
 }
 
