@@ -44,6 +44,10 @@ typedef struct {
     /* device time of the last vo_tri_run, ms (hipEvent) and per kernel kind */
     double ms_run, ms_locate_cavity, ms_commit, ms_exact, ms_other;
     int64_t launches_locate_cavity, launches_commit;
+    /* points whose cavity outgrew the per-point LDS lists and were inserted alone */
+    int64_t solo_runs;
+    /* points found to coincide with an existing vertex, counted on the device */
+    int64_t duplicates_detected;
 } vo_stats;
 
 /* DelaunayTree::<3,4>::new / ::<2,3>::new (src/delaunay_tree.rs:389-510, :544-640):
@@ -86,7 +90,9 @@ int vo_tri_stats(const vo_tri* t, vo_stats* out);
 
 /* tuning and test knobs (no counterpart in the reference): "filters" = mask of the FP32 (1)
  * and FP64-filter (2) predicate tiers, 3 by default; 0 sends every predicate to the exact
- * expansion tier.  "window_gamma_milli" = initial window size per inserted point x 1000. */
+ * expansion tier.  "group_lanes" = lanes cooperating on one window point in the round
+ * kernels (1, 4, 8, 16, 32; 0 = the default, 16 in 3D and 8 in 2D).  "window_gamma_milli" =
+ * initial window size per inserted point x 1000. */
 int vo_tri_config(vo_tri* t, const char* key, int64_t value);
 
 /* input vertex coordinates in the reference id space (for PyVertex.point): (2(dim+1) + n) x dim */

@@ -133,6 +133,7 @@ def test_exact_tier_decides_what_fp64_cannot():
     # level, the FP64 filter fails and the expansion tier decides (no exact degeneracy:
     # the rounding errors are generic, so the triangulation is still unique)
     rng = np.random.default_rng(72)
+    solo = 0
     for dim in (3, 2):
         u = rng.normal(size=(400, dim))
         S = 0.5 + 0.3 * u / np.linalg.norm(u, axis=1)[:, None]
@@ -141,8 +142,11 @@ def test_exact_tier_decides_what_fp64_cannot():
         T, E = t.canonical()
         ref = O.Tree.delaunay(P, O.MODE_EXACT).canonical()
         assert np.array_equal(T, ref.T) and np.array_equal(E, ref.E), dim
-        assert t.stats()["exact_evaluated"] > 0, dim
+        st = t.stats()
+        assert st["exact_evaluated"] > 0, dim
+        solo += st["solo_runs"]
         assert t.violations() == 0
+    assert solo > 0          # some cavities outgrew the per-point LDS lists and went solo
 
 
 @pytest.mark.parametrize("dim", [2, 3])
@@ -159,4 +163,17 @@ def test_exact_tier_alone_builds_the_same_triangulation(dim):
     st = t.stats()
     assert st["tier1"] == [0, 0] and st["tier2"] == [0, 0]
     assert st["tier3"][0] > 0 and st["tier3"][1] > 0 and st["exact_evaluated"] > 0
+    assert t.violations() == 0
+
+
+@pytest.mark.parametrize("lanes", [1, 4, 8, 16, 32])
+@pytest.mark.parametrize("dim", [2, 3])
+def test_every_lane_group_size_matches_oracle(dim, lanes):
+    P = datasets.uniform(20000, dim, seed=90 + dim)
+    t = vo.DelaunayTree(P)
+    t.config("group_lanes", lanes)
+    t.add_points_to_tree(P, seed=7)
+    T, E = t.canonical()
+    ref = O.Tree.delaunay(P, O.MODE_EXACT).canonical()
+    assert np.array_equal(T, ref.T) and np.array_equal(E, ref.E)
     assert t.violations() == 0

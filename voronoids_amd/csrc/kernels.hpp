@@ -46,7 +46,8 @@ struct RoundState {
     int n_q;               // exact-queue fill (may exceed capacity: excess dropped)
     unsigned int err;      // sticky ERR_* bits
     int n_live_delta;      // sum over winners of (created - killed) this round
-    int pad[2];
+    int n_big;             // points whose cavity outgrew the group kernel's LDS lists
+    int pad;
 };
 
 struct Ctx {
@@ -59,6 +60,7 @@ struct Ctx {
     int* cp;
     const int* win;
     int* pend_next;
+    int* big;              // points to re-run alone with large cavity buffers
     int* cav;              // killed lists, interleaved: cav[k * win_cap + slot]
     int* fac;              // boundary facets (t << 2 | face), interleaved likewise
     int* nk;
@@ -75,6 +77,7 @@ struct Ctx {
     CEntry* cache;
     uint32_t cmask;
     int filters;           // bit 0: FP32 tier on, bit 1: FP64 filter on (both on in production)
+    int solo;              // the window holds one point: cavity membership is stamped in kres
     int M;                 // vertex id of point p is M + p
     int win_cap, kcap, fcap;
     int tet_cap;
@@ -344,9 +347,11 @@ __global__ void __launch_bounds__(256) k_cavity(Ctx c, int nwin, unsigned long l
             if (s0 == SIGN_UNDECIDED) st = ST_DEFER;
             else if (s0 <= 0) st = ST_DUP;   // on the circumsphere of its own simplex: a vertex
         }
+        const unsigned long long skey = key_hi | (unsigned)p;
         if (st == ST_OK) {
             c.cav[slot] = t;
             s_cav[0][lt] = t;
+            if (c.solo) c.kres[t] = skey;
             nk = 1;
             for (int qi = 0; qi < nk && st == ST_OK; ++qi) {
                 const int ct = qi < KL ? s_cav[qi][lt] : c.cav[qi * W + slot];
@@ -356,9 +361,13 @@ __global__ void __launch_bounds__(256) k_cavity(Ctx c, int nwin, unsigned long l
                     if (nb >= 0) {
                         const int n = nb >> 2;
                         bool seen = false;
-                        const int nl = nk < KL ? nk : KL;
-                        for (int j = 0; j < nl && !seen; ++j) seen = s_cav[j][lt] == n;
-                        for (int j = KL; j < nk && !seen; ++j) seen = c.cav[j * W + slot] == n;
+                        if (c.solo) {
+                            seen = c.kres[n] == skey;
+                        } else {
+                            const int nl = nk < KL ? nk : KL;
+                            for (int j = 0; j < nl && !seen; ++j) seen = s_cav[j][lt] == n;
+                            for (int j = KL; j < nk && !seen; ++j) seen = c.cav[j * W + slot] == n;
+                        }
                         if (seen) continue;
                         const int s = conflict<D>(c, c.tv[n], pv, pf, lc);
                         if (s == SIGN_UNDECIDED) { st = ST_DEFER; break; }
@@ -366,6 +375,7 @@ __global__ void __launch_bounds__(256) k_cavity(Ctx c, int nwin, unsigned long l
                             if (nk == c.kcap) { st = ST_OVERFLOW; break; }
                             c.cav[nk * W + slot] = n;
                             if (nk < KL) s_cav[nk][lt] = n;
+                            if (c.solo) c.kres[n] = skey;
                             nk++;
                             continue;
                         }

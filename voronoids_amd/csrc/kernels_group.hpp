@@ -1,0 +1,354 @@
+// kernels_group.hpp -- lane-group versions of k_cavity / k_commit: G lanes of a wavefront
+// cooperate on one window point, NG = 64 / G points per wavefront, one wavefront per block.
+//
+// Same algorithm and state as kernels.hpp (the reference path they replace is cited there);
+// only the mapping of work to lanes changes, to shorten the dependent-load chain of a point:
+//   walk   : the D+1 face tests of a step run on D+1 lanes at once;
+//   cavity : breadth-first by levels; the (simplex, face) pairs of a level are spread over the
+//            G lanes, the cavity and facet lists live in LDS, new members are deduplicated and
+//            appended with group ballots and popcount prefixes;
+//   commit : one lane per boundary facet builds the new simplex, one lane per (facet, ridge)
+//            rotates around the ridge and finds the neighboring facet by an LDS scan.
+#pragma once
+#include "kernels.hpp"
+
+namespace vo {
+
+template <int G> struct Group {
+    static_assert(G == 4 || G == 8 || G == 16 || G == 32 || G == 64, "group size divides 64");
+    static constexpr int NG = 64 / G;
+    int lane, gid, gl;
+    __device__ Group() {
+        lane = (int)__lane_id();
+        gid = lane / G;
+        gl = lane % G;
+    }
+    __device__ __forceinline__ unsigned long long ballot(bool p) const {
+        const unsigned long long b = __ballot(p);
+        if constexpr (G == 64) return b;
+        else return (b >> (gid * G)) & ((1ull << G) - 1ull);
+    }
+    __device__ __forceinline__ int below(unsigned long long m) const {
+        return __popcll(m & ((1ull << gl) - 1ull));
+    }
+    __device__ __forceinline__ int bcast(int v, int k) const { return __shfl(v, gid * G + k, 64); }
+};
+
+template <int D> constexpr int kGroupKC = D == 3 ? 192 : 64;    // cavity entries per point
+template <int D> constexpr int kGroupFC = D == 3 ? 256 : 96;    // boundary facets per point
+
+__device__ __forceinline__ int ta_get(const int4* ta, int t, int f) {
+    return reinterpret_cast<const int*>(ta)[(size_t)t * 4 + f];
+}
+
+template <int D, int G>
+__global__ void __launch_bounds__(64) k_cavity_g(Ctx c, int nwin, unsigned long long key_hi) {
+    using V = VT<D>;
+    using F = typename V::F;
+    constexpr int M = D + 1;
+    constexpr int NG = Group<G>::NG;
+    constexpr int KC = kGroupKC<D>;
+    constexpr int FC = kGroupFC<D>;
+    __shared__ int s_cav[NG][KC];
+    __shared__ int s_fac[NG][FC];
+    __shared__ int s_tmp[64];
+    const Group<G> g;
+    const int slot = blockIdx.x * NG + g.gid;
+    const int W = c.win_cap;
+    const bool lead = g.gl == 0;
+    LocalCnt lc = {};
+    int st = slot < nwin ? ST_OK : ST_IDLE;
+    int p = 0, pv = 0, t = 0;
+    F pf{};
+    int4 tt = make_int4(0, 0, 0, 0);
+    unsigned long long key = 0;
+    if (st == ST_OK) {
+        p = c.win[slot];
+        pv = c.M + p;
+        key = key_hi | (unsigned)p;
+        pf = V::ld(c.X32, pv);
+        t = c.cp[p];
+        if (t < 0) t = hint_lookup<D>(c, pf);
+        tt = c.tv[t];
+        for (int hops = 0; tt.x < 0; ++hops) {
+            if (hops > kChainLimit) { if (lead) atomicOr(&c.st->err, ERR_CHAIN); st = ST_BROKEN; break; }
+            t = tt.y;
+            tt = c.tv[t];
+            if (lead) lc.chain++;
+        }
+    }
+    // ---- walk: lane k of the group tests face (r + k) mod M
+    bool walking = st == ST_OK;
+    int came = -1;
+    uint32_t rng = mix32((uint32_t)p * 0x9E3779B9u + 0x632BE5ABu);
+    for (int step = 0; __any(walking); ++step) {
+        int s = 1, nb = -1;
+        bool test = false;
+        if (walking) {
+            const int4 aa = c.ta[t];
+            rng = rng * 1664525u + 1013904223u;
+            const int r = (int)((rng >> 16) % M);
+            int k = r + g.gl;
+            if (k >= M) k -= M;
+            if (g.gl < M) {
+                nb = comp(aa, k);
+                test = !(nb >= 0 && (nb >> 2) == came);
+                if (test) s = orient_sub<D>(c, tt, k, pv, pf, lc);
+            }
+        }
+        const unsigned long long und = g.ballot(test && s == SIGN_UNDECIDED);
+        const unsigned long long neg = g.ballot(test && s != SIGN_UNDECIDED && s < 0);
+        const int first = neg ? __ffsll((unsigned long long)neg) - 1 : 0;
+        const int nb_sel = g.bcast(nb, first);
+        if (walking) {
+            if (lead) lc.walk++;
+            if (und) {
+                st = ST_DEFER;
+                walking = false;
+            } else if (!neg) {
+                walking = false;
+            } else if (nb_sel < 0 || step > kWalkLimit) {
+                if (lead) atomicOr(&c.st->err, nb_sel < 0 ? ERR_HULL : ERR_WALK);
+                st = ST_BROKEN;
+                walking = false;
+            } else {
+                came = t;
+                t = nb_sel >> 2;
+                tt = c.tv[t];
+            }
+        }
+    }
+    if (st == ST_OK || st == ST_DEFER) {
+        if (lead) {
+            c.cp[p] = t;
+            hint_store<D>(c, pf, t);
+        }
+    }
+    // ---- the containing simplex must conflict (else p repeats one of its vertices)
+    int s0 = 0;
+    if (st == ST_OK && lead) s0 = conflict<D>(c, tt, pv, pf, lc);
+    s0 = g.bcast(s0, 0);
+    if (st == ST_OK) {
+        if (s0 == SIGN_UNDECIDED) st = ST_DEFER;
+        else if (s0 <= 0) { st = ST_DUP; if (lead) lc.dups++; }
+    }
+    // ---- cavity: level-synchronous breadth-first search
+    int nk = 0, nf = 0, lo = 0, hi = 0, ci = 0;
+    bool bfs = st == ST_OK;
+    if (bfs) {
+        if (lead) s_cav[g.gid][0] = t;
+        nk = 1;
+        hi = 1;
+    }
+    __builtin_amdgcn_wave_barrier();
+    while (__any(bfs)) {
+        const int items = bfs ? (hi - lo) * M : 0;
+        const int i = ci + g.gl;
+        const bool valid = bfs && i < items;
+        int tet = 0, face = 0, n = -1, s = 0;
+        bool hull = false, test = false;
+        if (valid) {
+            tet = s_cav[g.gid][lo + i / M];
+            face = i % M;
+            const int nb = ta_get(c.ta, tet, face);
+            hull = nb < 0;
+            if (!hull) {
+                n = nb >> 2;
+                bool inC = false;
+                for (int j = 0; j < nk && !inC; ++j) inC = s_cav[g.gid][j] == n;
+                test = !inC;
+                if (test) s = conflict<D>(c, c.tv[n], pv, pf, lc);
+            }
+        }
+        const unsigned long long und = g.ballot(test && s == SIGN_UNDECIDED);
+        bool isnew = test && s != SIGN_UNDECIDED && s > 0;
+        s_tmp[g.lane] = isnew ? n : -1;
+        __builtin_amdgcn_wave_barrier();
+        if (isnew)
+            for (int j = 0; j < g.gl; ++j)
+                if (s_tmp[g.gid * G + j] == n) { isnew = false; break; }
+        const bool isfac = valid && (hull || (test && s != SIGN_UNDECIDED && s <= 0));
+        const unsigned long long newm = g.ballot(isnew), facm = g.ballot(isfac);
+        const int cnew = __popcll(newm), cfac = __popcll(facm);
+        if (bfs) {
+            if (und) {
+                st = ST_DEFER;
+                bfs = false;
+            } else if (nk + cnew > KC || nf + cfac > FC) {
+                st = ST_OVERFLOW;          // re-run alone with global-memory lists
+                bfs = false;
+            } else {
+                if (isnew) s_cav[g.gid][nk + g.below(newm)] = n;
+                if (isfac) {
+                    s_fac[g.gid][nf + g.below(facm)] = (tet << 2) | face;
+                    if (!hull) atomicMin(&c.nres[n], key);
+                }
+                nk += cnew;
+                nf += cfac;
+                ci += G;
+                if (ci >= items) {
+                    lo = hi;
+                    hi = nk;
+                    ci = 0;
+                    if (lo == hi) bfs = false;
+                }
+            }
+        }
+        __builtin_amdgcn_wave_barrier();
+    }
+    if (st == ST_OK) {
+        for (int j = g.gl; j < nk; j += G) {
+            const int x = s_cav[g.gid][j];
+            atomicMin(&c.kres[x], key);
+            c.cav[j * W + slot] = x;
+        }
+        for (int j = g.gl; j < nf; j += G) c.fac[j * W + slot] = s_fac[g.gid][j];
+    }
+    if (slot < nwin && lead) {
+        c.nk[slot] = nk;
+        c.nf[slot] = nf;
+        c.status[slot] = st;
+    }
+    flush_counters(c.cnt, lc);
+}
+
+template <int D, int G>
+__global__ void __launch_bounds__(64) k_commit_g(Ctx c, int nwin, unsigned long long key_hi) {
+    constexpr int M = D + 1;
+    constexpr int NG = Group<G>::NG;
+    constexpr int FC = kGroupFC<D>;
+    __shared__ int s_fac[NG][FC];
+    const Group<G> g;
+    const int slot = blockIdx.x * NG + g.gid;
+    const int W = c.win_cap;
+    const bool lead = g.gl == 0;
+    LocalCnt lc = {};
+    int st = ST_IDLE, p = 0, nk = 0, nf = 0;
+    unsigned long long key = 0;
+    bool ok = true;
+    if (slot < nwin) {
+        st = c.status[slot];
+        p = c.win[slot];
+        key = key_hi | (unsigned)p;
+        if (st == ST_OK) {
+            nk = c.nk[slot];
+            nf = c.nf[slot];
+            for (int j = g.gl; j < nk && ok; j += G) {
+                const int x = c.cav[j * W + slot];
+                ok = c.kres[x] == key && c.nres[x] > key;
+            }
+            for (int j = g.gl; j < nf && ok; j += G) {
+                const int fe = c.fac[j * W + slot];
+                s_fac[g.gid][j] = fe;
+                const int nb = ta_get(c.ta, fe >> 2, fe & 3);
+                if (nb >= 0) ok = c.kres[nb >> 2] > key;
+            }
+        }
+    }
+    const bool win = st == ST_OK && g.ballot(!ok) == 0;
+    // one simplex slot range per winning group, one atomicAdd per wavefront
+    int total = 0;
+    const int off = wave_excl_scan(win && lead ? nf : 0, total);
+    int base0 = 0;
+    if (g.lane == 0 && total > 0) base0 = atomicAdd(&c.st->n_tets, total);
+    base0 = __shfl(base0, 0, 64);
+    const int base = base0 + g.bcast(off, 0);
+    bool build = win;
+    if (build && base + nf > c.tet_cap) {
+        if (lead) atomicOr(&c.st->err, ERR_TETCAP);
+        build = false;
+    }
+    // a winner's check loops ran to completion, so s_fac holds its whole facet list
+    __builtin_amdgcn_wave_barrier();
+    const int pv = c.M + p;
+    if (build) {
+        // phase 1: the new simplex of facet j, its outside link and the outside neighbor's patch
+        for (int j = g.gl; j < nf; j += G) {
+            const int fe = s_fac[g.gid][j];
+            const int t = fe >> 2, i = fe & 3;
+            const int4 nv = replace_first<D>(c.tv[t], i, pv);
+            const int outside = ta_get(c.ta, t, i);
+            const int N = base + j;
+            c.tv[N] = nv;
+            *comp_ptr(&c.ta[N], 0) = outside;
+            if (D == 2) *comp_ptr(&c.ta[N], 3) = -1;
+            if (outside >= 0) *comp_ptr(&c.ta[outside >> 2], outside & 3) = N << 2;
+        }
+        // phase 2: internal faces, one lane per (facet, ridge)
+        for (int q = g.gl; q < nf * D; q += G) {
+            const int j = q / D, k = 1 + q % D;
+            const int fe = s_fac[g.gid][j];
+            const int t = fe >> 2, i = fe & 3;
+            const int4 tvt = c.tv[t];
+            const int4 nv = replace_first<D>(tvt, i, pv);
+            const int cv = comp(nv, k);
+            int T = t;
+            int4 TV = tvt;
+            int e = index_of<D>(TV, cv);
+            int y = comp(tvt, i);
+            for (int guard = 0;; ++guard) {
+                const int nbe = ta_get(c.ta, T, e);
+                if (nbe < 0 || c.kres[nbe >> 2] != key) break;
+                if (guard > 4096) { atomicOr(&c.st->err, ERR_WALK); break; }
+                const int T2 = nbe >> 2;
+                const int4 TV2 = c.tv[T2];
+                const int w = comp(TV2, nbe & 3);
+                e = index_of<D>(TV2, y);
+                y = w;
+                T = T2;
+                TV = TV2;
+            }
+            const int target = (T << 2) | e;
+            int jj = -1;
+            for (int r = 0; r < nf; ++r)
+                if (s_fac[g.gid][r] == target) { jj = r; break; }
+            const int4 nv2 = replace_first<D>(TV, e, pv);
+            int kk = 1;
+            for (int m = 1; m < M; ++m) {
+                const int x = comp(nv2, m);
+                bool in_ridge = false;
+                for (int r = 1; r < M; ++r)
+                    if (r != k && comp(nv, r) == x) in_ridge = true;
+                if (!in_ridge) { kk = m; break; }
+            }
+            if (jj < 0) atomicOr(&c.st->err, ERR_WALK);
+            *comp_ptr(&c.ta[base + j], k) = jj < 0 ? -1 : (((base + jj) << 2) | kk);
+        }
+        // phase 3: killed simplices die, forwarding to a new simplex on one of their facets
+        for (int j = g.gl; j < nk; j += G) {
+            const int t = c.cav[j * W + slot];
+            int fwd = base;
+            for (int r = 0; r < nf; ++r)
+                if ((s_fac[g.gid][r] >> 2) == t) { fwd = base + r; break; }
+            c.tv[t] = make_int4(-1, fwd, 0, 0);
+        }
+        if (lead) {
+            lc.killed = nk;
+            lc.created = nf;
+            lc.wins = 1;
+            lc.maxk = nk;
+            lc.maxf = nf;
+        }
+    }
+    // cavities too large for the LDS lists are re-run alone after the round
+    const bool big = lead && st == ST_OVERFLOW;
+    const unsigned long long bm = __ballot(big);
+    int bpos = 0;
+    if (g.lane == 0 && bm) bpos = atomicAdd(&c.st->n_big, __popcll(bm));
+    bpos = __shfl(bpos, 0, 64);
+    if (big) c.big[bpos + __popcll(bm & ((1ull << g.lane) - 1ull))] = p;
+    const bool again = lead && ((st == ST_OK && !build) || st == ST_DEFER);
+    const unsigned long long mask = __ballot(again);
+    const int cnt = __popcll(mask);
+    int pos0 = 0;
+    if (g.lane == 0 && cnt) pos0 = atomicAdd(&c.st->n_next_pending, cnt);
+    pos0 = __shfl(pos0, 0, 64);
+    if (again) c.pend_next[pos0 + __popcll(mask & ((1ull << g.lane) - 1ull))] = p;
+    const unsigned long long wm = __ballot(build && lead);
+    if (g.lane == 0 && wm) atomicAdd(&c.st->n_winners, __popcll(wm));
+    const long long delta = wave_sum_i64(build && lead ? (long long)(nf - nk) : 0ll);
+    if (g.lane == 0 && delta) atomicAdd(&c.st->n_live_delta, (int)delta);
+    flush_counters(c.cnt, lc);
+}
+
+}  // namespace vo

@@ -10,8 +10,8 @@
 //   insphere(a,b,c,d,e) > 0  iff e inside sphere(abcd) when orient3d(a,b,c,d) > 0
 //
 // Tier 1 evaluates in FP32 on coordinates stored as fl32(x - c) (c = bounding-sphere
-// center) and carries a running absolute error bound through every operation, starting
-// from the storage error |s - (x - c)| <= 2^-23 |s| + 1e-36.  Tier 2 is the FP64 stage-A
+// center): Shewchuk's stage-A expression plus a closed-form bound on the storage error
+// |s - (x - c)| <= 2^-23 |s| (derivation below).  Tier 2 is the FP64 stage-A
 // filter of Shewchuk (1997, "Adaptive Precision Floating-Point Arithmetic and Fast Robust
 // Geometric Predicates"): evaluation on the raw f64 coordinates, bound = c * eps * permanent.
 // Tier 3 is exact: FP64 expansion arithmetic (same paper, sections 2.2-2.6) on the exact
@@ -27,89 +27,129 @@ namespace vo {
 enum : int { SIGN_UNDECIDED = 2 };
 
 // ----------------------------------------------------------------------------- tier 1
-struct ef { float v, e; };   // value and a bound on |value - exact|
+// FP32 on stored coordinates s = fl32(x - c).  Each computed difference d~ = fl(s_a - s_b)
+// differs from the exact difference of the inputs by at most
+//     eps = 2^-23 (|s_a| + |s_b|) + 2^-24 |d~| + 2e-37  <=  2^-22 m + 2^-24 delta + 2e-37,
+// m = max |s|, delta = max |d~| over the call.  The determinant is then evaluated from the
+// d~ with Shewchuk's stage-A expression (its bound c * u * permanent covers the evaluation,
+// u = 2^-24), and the input error is bounded by multilinearity and Hadamard's inequality:
+//   orient2d   2      ((delta+eps)^2 - delta^2)
+//   orient3d   3 sqrt3((delta+eps)^3 - delta^3)
+//   incircle  12      ((delta+eps)^4 - delta^4)
+//   insphere  36 sqrt3((delta+eps)^5 - delta^5)
+// (rows have norm <= sqrt(d) (delta+eps), lifts <= d (delta+eps)^2).  The bound's own fp32
+// rounding is covered by a factor 1.01; magnitudes near the fp32 underflow range defer.
+constexpr float kU32 = 5.9604644775390625e-8f;    // 2^-24
+constexpr float kCcwA32 = (3.0f + 16.0f * kU32) * kU32;
+constexpr float kO3dA32 = (7.0f + 56.0f * kU32) * kU32;
+constexpr float kIccA32 = (10.0f + 96.0f * kU32) * kU32;
+constexpr float kIspA32 = (16.0f + 224.0f * kU32) * kU32;
+constexpr float kTiny32 = 1e-30f;
 
-constexpr float kU32 = 5.9604644775390625e-8f;    // 2^-24, unit roundoff of fp32
-constexpr float kEta = 1.1920928955078125e-7f;    // 2^-23, relative storage error bound
-constexpr float kFloor = 1e-36f;                  // covers underflow of any single op
+// 2^-22 m + 2^-24 delta, plus 2e-37 for coordinates that underflowed in the fp32 store
+__device__ __forceinline__ float in_err(float m, float dl) { return m * 2.384185791015625e-7f + dl * kU32 + 2e-37f; }
 
-__device__ __forceinline__ ef ef_diff(float a, float b) {
-    float d = a - b;
-    return {d, kEta * (fabsf(a) + fabsf(b)) + kU32 * fabsf(d) + kFloor};
-}
-__device__ __forceinline__ ef ef_mul(ef a, ef b) {
-    float p = a.v * b.v;
-    return {p, fabsf(a.v) * b.e + fabsf(b.v) * a.e + a.e * b.e + kU32 * fabsf(p) + kFloor};
-}
-__device__ __forceinline__ ef ef_add(ef a, ef b) {
-    float s = a.v + b.v;
-    return {s, a.e + b.e + kU32 * fabsf(s) + kFloor};
-}
-__device__ __forceinline__ ef ef_sub(ef a, ef b) {
-    float s = a.v - b.v;
-    return {s, a.e + b.e + kU32 * fabsf(s) + kFloor};
-}
-// The bound itself is computed with < 40 fp32 roundings, each relative <= 2^-24:
-// inflating it by 1/1024 covers them.
-__device__ __forceinline__ int ef_sign(ef r) {
-    float b = r.e * 1.0009765625f;
-    if (!(fabsf(r.v) < __builtin_inff()) || !(b < __builtin_inff())) return SIGN_UNDECIDED;
-    if (r.v > b) return 1;
-    if (r.v < -b) return -1;
+__device__ __forceinline__ int t1_decide(float det, float perm, float pin) {
+    if (!(perm > kTiny32) || !(perm < 1e30f)) return SIGN_UNDECIDED;   // also NaN
+    const float bound = (perm + pin) * 1.01f;
+    if (det > bound) return 1;
+    if (-det > bound) return -1;
     return SIGN_UNDECIDED;
 }
 
+__device__ __forceinline__ float amax3(float a, float b, float c) { return fmaxf(fabsf(a), fmaxf(fabsf(b), fabsf(c))); }
+
 __device__ __forceinline__ int t1_orient2d(float2 a, float2 b, float2 c) {
-    ef ux = ef_diff(b.x, a.x), uy = ef_diff(b.y, a.y), vx = ef_diff(c.x, a.x), vy = ef_diff(c.y, a.y);
-    return ef_sign(ef_sub(ef_mul(ux, vy), ef_mul(uy, vx)));
+    const float acx = a.x - c.x, bcx = b.x - c.x, acy = a.y - c.y, bcy = b.y - c.y;
+    const float l = acx * bcy, r = acy * bcx;
+    const float det = l - r;
+    const float m = fmaxf(fmaxf(fabsf(a.x), fabsf(a.y)), fmaxf(fmaxf(fabsf(b.x), fabsf(b.y)), fmaxf(fabsf(c.x), fabsf(c.y))));
+    const float dl = fmaxf(fmaxf(fabsf(acx), fabsf(bcx)), fmaxf(fabsf(acy), fabsf(bcy)));
+    const float e = in_err(m, dl);
+    const float pin = 2.0f * e * (2.0f * dl + e);
+    return t1_decide(det, kCcwA32 * (fabsf(l) + fabsf(r)), pin);
 }
 
-__device__ __forceinline__ int t1_orient3d(float4 a, float4 b, float4 c, float4 d) {
-    ef ux = ef_diff(b.x, a.x), uy = ef_diff(b.y, a.y), uz = ef_diff(b.z, a.z);
-    ef vx = ef_diff(c.x, a.x), vy = ef_diff(c.y, a.y), vz = ef_diff(c.z, a.z);
-    ef wx = ef_diff(d.x, a.x), wy = ef_diff(d.y, a.y), wz = ef_diff(d.z, a.z);
-    ef m0 = ef_sub(ef_mul(vy, wz), ef_mul(vz, wy));
-    ef m1 = ef_sub(ef_mul(vx, wz), ef_mul(vz, wx));
-    ef m2 = ef_sub(ef_mul(vx, wy), ef_mul(vy, wx));
-    ef det = ef_add(ef_sub(ef_mul(ux, m0), ef_mul(uy, m1)), ef_mul(uz, m2));
-    return ef_sign(det);
+// orient3d(a,b,c,d) = -det[a-d; b-d; c-d]
+__device__ __forceinline__ int t1_orient3d(float4 pa, float4 pb, float4 pc, float4 pd) {
+    const float adx = pa.x - pd.x, bdx = pb.x - pd.x, cdx = pc.x - pd.x;
+    const float ady = pa.y - pd.y, bdy = pb.y - pd.y, cdy = pc.y - pd.y;
+    const float adz = pa.z - pd.z, bdz = pb.z - pd.z, cdz = pc.z - pd.z;
+    const float bdxcdy = bdx * cdy, cdxbdy = cdx * bdy;
+    const float cdxady = cdx * ady, adxcdy = adx * cdy;
+    const float adxbdy = adx * bdy, bdxady = bdx * ady;
+    const float det = adz * (bdxcdy - cdxbdy) + bdz * (cdxady - adxcdy) + cdz * (adxbdy - bdxady);
+    const float perm = (fabsf(bdxcdy) + fabsf(cdxbdy)) * fabsf(adz) + (fabsf(cdxady) + fabsf(adxcdy)) * fabsf(bdz) +
+                       (fabsf(adxbdy) + fabsf(bdxady)) * fabsf(cdz);
+    const float m = fmaxf(fmaxf(amax3(pa.x, pa.y, pa.z), amax3(pb.x, pb.y, pb.z)),
+                          fmaxf(amax3(pc.x, pc.y, pc.z), amax3(pd.x, pd.y, pd.z)));
+    const float dl = fmaxf(fmaxf(amax3(adx, ady, adz), amax3(bdx, bdy, bdz)), amax3(cdx, cdy, cdz));
+    const float e = in_err(m, dl);
+    const float pin = 5.196152422706632f * e * (3.0f * dl * dl + e * (3.0f * dl + e));
+    const int s = t1_decide(det, kO3dA32 * perm, pin);
+    return s == SIGN_UNDECIDED ? s : -s;
 }
 
-__device__ __forceinline__ int t1_incircle(float2 a, float2 b, float2 c, float2 d) {
-    ef ax = ef_diff(a.x, d.x), ay = ef_diff(a.y, d.y);
-    ef bx = ef_diff(b.x, d.x), by = ef_diff(b.y, d.y);
-    ef cx = ef_diff(c.x, d.x), cy = ef_diff(c.y, d.y);
-    ef al = ef_add(ef_mul(ax, ax), ef_mul(ay, ay));
-    ef bl = ef_add(ef_mul(bx, bx), ef_mul(by, by));
-    ef cl = ef_add(ef_mul(cx, cx), ef_mul(cy, cy));
-    ef t0 = ef_mul(al, ef_sub(ef_mul(bx, cy), ef_mul(cx, by)));
-    ef t1 = ef_mul(bl, ef_sub(ef_mul(cx, ay), ef_mul(ax, cy)));
-    ef t2 = ef_mul(cl, ef_sub(ef_mul(ax, by), ef_mul(bx, ay)));
-    return ef_sign(ef_add(ef_add(t0, t1), t2));
+__device__ __forceinline__ int t1_incircle(float2 pa, float2 pb, float2 pc, float2 pd) {
+    const float adx = pa.x - pd.x, bdx = pb.x - pd.x, cdx = pc.x - pd.x;
+    const float ady = pa.y - pd.y, bdy = pb.y - pd.y, cdy = pc.y - pd.y;
+    const float bdxcdy = bdx * cdy, cdxbdy = cdx * bdy;
+    const float alift = adx * adx + ady * ady;
+    const float cdxady = cdx * ady, adxcdy = adx * cdy;
+    const float blift = bdx * bdx + bdy * bdy;
+    const float adxbdy = adx * bdy, bdxady = bdx * ady;
+    const float clift = cdx * cdx + cdy * cdy;
+    const float det = alift * (bdxcdy - cdxbdy) + blift * (cdxady - adxcdy) + clift * (adxbdy - bdxady);
+    const float perm = (fabsf(bdxcdy) + fabsf(cdxbdy)) * alift + (fabsf(cdxady) + fabsf(adxcdy)) * blift +
+                       (fabsf(adxbdy) + fabsf(bdxady)) * clift;
+    const float m = fmaxf(fmaxf(fmaxf(fabsf(pa.x), fabsf(pa.y)), fmaxf(fabsf(pb.x), fabsf(pb.y))),
+                          fmaxf(fmaxf(fabsf(pc.x), fabsf(pc.y)), fmaxf(fabsf(pd.x), fabsf(pd.y))));
+    const float dl = fmaxf(fmaxf(fmaxf(fabsf(adx), fabsf(ady)), fmaxf(fabsf(bdx), fabsf(bdy))),
+                           fmaxf(fabsf(cdx), fabsf(cdy)));
+    const float e = in_err(m, dl);
+    const float pin = 12.0f * e * (4.0f * dl * dl * dl + e * (6.0f * dl * dl + e * (4.0f * dl + e)));
+    return t1_decide(det, kIccA32 * perm, pin);
 }
 
-__device__ __forceinline__ int t1_insphere(float4 a, float4 b, float4 c, float4 d, float4 e) {
-    ef ax = ef_diff(a.x, e.x), ay = ef_diff(a.y, e.y), az = ef_diff(a.z, e.z);
-    ef bx = ef_diff(b.x, e.x), by = ef_diff(b.y, e.y), bz = ef_diff(b.z, e.z);
-    ef cx = ef_diff(c.x, e.x), cy = ef_diff(c.y, e.y), cz = ef_diff(c.z, e.z);
-    ef dx = ef_diff(d.x, e.x), dy = ef_diff(d.y, e.y), dz = ef_diff(d.z, e.z);
-    ef ab = ef_sub(ef_mul(ax, by), ef_mul(bx, ay));
-    ef bc = ef_sub(ef_mul(bx, cy), ef_mul(cx, by));
-    ef cd = ef_sub(ef_mul(cx, dy), ef_mul(dx, cy));
-    ef da = ef_sub(ef_mul(dx, ay), ef_mul(ax, dy));
-    ef ac = ef_sub(ef_mul(ax, cy), ef_mul(cx, ay));
-    ef bd = ef_sub(ef_mul(bx, dy), ef_mul(dx, by));
-    ef abc = ef_add(ef_sub(ef_mul(az, bc), ef_mul(bz, ac)), ef_mul(cz, ab));
-    ef bcd = ef_add(ef_sub(ef_mul(bz, cd), ef_mul(cz, bd)), ef_mul(dz, bc));
-    ef cda = ef_add(ef_add(ef_mul(cz, da), ef_mul(dz, ac)), ef_mul(az, cd));
-    ef dab = ef_add(ef_add(ef_mul(dz, ab), ef_mul(az, bd)), ef_mul(bz, da));
-    ef al = ef_add(ef_add(ef_mul(ax, ax), ef_mul(ay, ay)), ef_mul(az, az));
-    ef bl = ef_add(ef_add(ef_mul(bx, bx), ef_mul(by, by)), ef_mul(bz, bz));
-    ef cl = ef_add(ef_add(ef_mul(cx, cx), ef_mul(cy, cy)), ef_mul(cz, cz));
-    ef dl = ef_add(ef_add(ef_mul(dx, dx), ef_mul(dy, dy)), ef_mul(dz, dz));
-    ef det = ef_add(ef_sub(ef_mul(dl, abc), ef_mul(cl, dab)), ef_sub(ef_mul(bl, cda), ef_mul(al, bcd)));
-    int s = ef_sign(det);
-    return s == SIGN_UNDECIDED ? s : -s;   // the lifted determinant is -insphere
+__device__ __forceinline__ int t1_insphere(float4 pa, float4 pb, float4 pc, float4 pd, float4 pe) {
+    const float aex = pa.x - pe.x, bex = pb.x - pe.x, cex = pc.x - pe.x, dex = pd.x - pe.x;
+    const float aey = pa.y - pe.y, bey = pb.y - pe.y, cey = pc.y - pe.y, dey = pd.y - pe.y;
+    const float aez = pa.z - pe.z, bez = pb.z - pe.z, cez = pc.z - pe.z, dez = pd.z - pe.z;
+    const float aexbey = aex * bey, bexaey = bex * aey, ab = aexbey - bexaey;
+    const float bexcey = bex * cey, cexbey = cex * bey, bc = bexcey - cexbey;
+    const float cexdey = cex * dey, dexcey = dex * cey, cd = cexdey - dexcey;
+    const float dexaey = dex * aey, aexdey = aex * dey, da = dexaey - aexdey;
+    const float aexcey = aex * cey, cexaey = cex * aey, ac = aexcey - cexaey;
+    const float bexdey = bex * dey, dexbey = dex * bey, bd = bexdey - dexbey;
+    const float abc = aez * bc - bez * ac + cez * ab;
+    const float bcd = bez * cd - cez * bd + dez * bc;
+    const float cda = cez * da + dez * ac + aez * cd;
+    const float dab = dez * ab + aez * bd + bez * da;
+    const float alift = aex * aex + aey * aey + aez * aez;
+    const float blift = bex * bex + bey * bey + bez * bez;
+    const float clift = cex * cex + cey * cey + cez * cez;
+    const float dlift = dex * dex + dey * dey + dez * dez;
+    const float det = (dlift * abc - clift * dab) + (blift * cda - alift * bcd);
+    const float aezp = fabsf(aez), bezp = fabsf(bez), cezp = fabsf(cez), dezp = fabsf(dez);
+    const float aexbeyp = fabsf(aexbey), bexaeyp = fabsf(bexaey), bexceyp = fabsf(bexcey), cexbeyp = fabsf(cexbey);
+    const float cexdeyp = fabsf(cexdey), dexceyp = fabsf(dexcey), dexaeyp = fabsf(dexaey), aexdeyp = fabsf(aexdey);
+    const float aexceyp = fabsf(aexcey), cexaeyp = fabsf(cexaey), bexdeyp = fabsf(bexdey), dexbeyp = fabsf(dexbey);
+    const float perm =
+        ((cexdeyp + dexceyp) * bezp + (dexbeyp + bexdeyp) * cezp + (bexceyp + cexbeyp) * dezp) * alift +
+        ((dexaeyp + aexdeyp) * cezp + (aexceyp + cexaeyp) * dezp + (cexdeyp + dexceyp) * aezp) * blift +
+        ((aexbeyp + bexaeyp) * dezp + (bexdeyp + dexbeyp) * aezp + (dexaeyp + aexdeyp) * bezp) * clift +
+        ((bexceyp + cexbeyp) * aezp + (cexaeyp + aexceyp) * bezp + (aexbeyp + bexaeyp) * cezp) * dlift;
+    const float m = fmaxf(fmaxf(fmaxf(amax3(pa.x, pa.y, pa.z), amax3(pb.x, pb.y, pb.z)),
+                                fmaxf(amax3(pc.x, pc.y, pc.z), amax3(pd.x, pd.y, pd.z))),
+                          amax3(pe.x, pe.y, pe.z));
+    const float dl = fmaxf(fmaxf(amax3(aex, aey, aez), amax3(bex, bey, bez)),
+                           fmaxf(amax3(cex, cey, cez), amax3(dex, dey, dez)));
+    const float e = in_err(m, dl);
+    const float d2 = dl * dl;
+    const float pin = 62.35382907247958f * e *
+                      (5.0f * d2 * d2 + e * (10.0f * d2 * dl + e * (10.0f * d2 + e * (5.0f * dl + e))));
+    const int s = t1_decide(det, kIspA32 * perm, pin);
+    return s == SIGN_UNDECIDED ? s : -s;    // the lifted determinant is -insphere
 }
 
 // ----------------------------------------------------------------------------- tier 2

@@ -12,6 +12,7 @@
 #include <array>
 #include <cmath>
 #include <cstdint>
+#include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <string>
@@ -19,6 +20,7 @@
 
 #include "../../include/voronoids.h"
 #include "kernels.hpp"
+#include "kernels_group.hpp"
 
 #pragma clang fp contract(off)
 
@@ -133,6 +135,12 @@ struct vo_tri {
     int* nk = nullptr;
     int* nf = nullptr;
     int* status = nullptr;
+    // points re-run alone: their list, and large single-point cavity buffers
+    int* big = nullptr;
+    int* cav1 = nullptr;
+    int* fac1 = nullptr;
+    int* misc1 = nullptr;              // nk, nf, status of the single point
+    int big_cap = 1 << 20;
     // hints
     int* hint = nullptr;
     int hlevels = 0;
@@ -151,9 +159,10 @@ struct vo_tri {
     hipEvent_t ev[6] = {};
     double gamma = 0.25;
     int filters = 3;
+    int group = 0;                     // lanes per window point in the round kernels (1 = one thread)
     int64_t inserted = 0, dups = 0, rounds = 0, windows_total = 0;
     double ms_run = 0, ms_cav = 0, ms_com = 0, ms_ex = 0, ms_other = 0;
-    int64_t n_cav = 0, n_com = 0;
+    int64_t n_cav = 0, n_com = 0, n_solo = 0;
     bool dirty_dup = false;
 };
 
@@ -170,6 +179,7 @@ Ctx make_ctx(vo_tri* t) {
     c.cp = t->cp;
     c.win = t->win;
     c.pend_next = t->pend[t->cur ^ 1];
+    c.big = t->big;
     c.cav = t->cav;
     c.fac = t->fac;
     c.nk = t->nk;
@@ -187,6 +197,7 @@ Ctx make_ctx(vo_tri* t) {
     c.cmask = t->cmask;
     c.M = t->M;
     c.filters = t->filters;
+    c.solo = 0;
     c.win_cap = t->win_cap;
     c.kcap = t->kcap;
     c.fcap = t->fcap;
@@ -204,7 +215,10 @@ __global__ void k_reset_round(RoundState* st) {
     st->n_winners = 0;
     st->n_q = 0;
     st->n_live_delta = 0;
+    st->n_big = 0;
 }
+
+
 
 // grow the simplex arrays to cap (copying the first n), new reservations = max key
 int grow_tets(vo_tri* t, int64_t cap) {
@@ -275,7 +289,8 @@ int upload_super(vo_tri* t) {
 
 void free_all(vo_tri* t) {
     void* ptrs[] = {t->X64, t->X32, t->cp, t->vmap, t->tv, t->ta, t->kres, t->nres, t->win, t->pend[0], t->pend[1],
-                    t->cav, t->fac, t->nk, t->nf, t->status, t->hint, t->st, t->cnt, t->q, t->cache, t->ws};
+                    t->cav, t->fac, t->nk, t->nf, t->status, t->hint, t->st, t->cnt, t->q, t->cache, t->ws,
+                    t->big, t->cav1, t->fac1, t->misc1};
     for (void* p : ptrs)
         if (p) hipFree(p);
     if (t->st_host) hipHostFree(t->st_host);
@@ -321,6 +336,89 @@ int run_exact(vo_tri* t, int nq) {
     t->ms_ex += ms;
     return 0;
 }
+
+// Points whose cavity outgrew the group kernel's LDS lists, re-run one at a time with the
+// thread kernels over single-point lists of big_cap entries.  A point alone in its round wins
+// unless a predicate defers (then it joins the next window).  The solo runs add to the same
+// device round state (winners, live delta, pending list, exact queue) as the round itself.
+int run_solo(vo_tri* t, int n_big) {
+    int rc;
+    t->n_tets = t->st_host->n_tets;     // grow_tets below copies the first n_tets simplices
+    for (int i = 0; i < n_big; ++i) {
+        t->rounds++;
+        const unsigned long long key_hi = (unsigned long long)(0xFFFFFFFFu - (uint32_t)t->rounds) << 32;
+        Ctx c = make_ctx(t);
+        c.win = t->big + i;
+        c.cav = t->cav1;
+        c.fac = t->fac1;
+        c.nk = t->misc1;
+        c.nf = t->misc1 + 1;
+        c.status = t->misc1 + 2;
+        c.win_cap = 1;
+        c.kcap = t->big_cap;
+        c.fcap = t->big_cap;
+        c.solo = 1;
+        const int need = t->n_tets + t->big_cap + 1024;
+        if (need > t->tcap) {
+            if ((rc = grow_tets(t, std::max<int64_t>(need, t->tcap * 3 / 2)))) return rc;
+            c = make_ctx(t);
+            c.win = t->big + i; c.cav = t->cav1; c.fac = t->fac1;
+            c.nk = t->misc1; c.nf = t->misc1 + 1; c.status = t->misc1 + 2;
+            c.win_cap = 1; c.kcap = t->big_cap; c.fcap = t->big_cap; c.solo = 1;
+        }
+        if (t->D == 3) {
+            hipLaunchKernelGGL(k_cavity<3>, dim3(1), dim3(256), 0, t->s, c, 1, key_hi);
+            hipLaunchKernelGGL(k_commit<3>, dim3(1), dim3(256), 0, t->s, c, 1, key_hi);
+        } else {
+            hipLaunchKernelGGL(k_cavity<2>, dim3(1), dim3(256), 0, t->s, c, 1, key_hi);
+            hipLaunchKernelGGL(k_commit<2>, dim3(1), dim3(256), 0, t->s, c, 1, key_hi);
+        }
+        VO_HIP(hipGetLastError());
+        if ((rc = sync_state(t)) || (rc = check_err(t))) return rc;
+        t->n_tets = t->st_host->n_tets;
+    }
+    t->n_solo += n_big;
+    return 0;
+}
+
+#define VO_LAUNCH_G(DIM, GL)                                                                     \
+    do {                                                                                         \
+        const dim3 grid_((nwin + 64 / GL - 1) / (64 / GL));                                      \
+        if (which) hipLaunchKernelGGL((k_commit_g<DIM, GL>), grid_, dim3(64), 0, t->s, c, nwin, key_hi); \
+        else hipLaunchKernelGGL((k_cavity_g<DIM, GL>), grid_, dim3(64), 0, t->s, c, nwin, key_hi);       \
+    } while (0)
+
+// which = 0: locate + cavity + reservation, 1: check + commit
+int launch_round(vo_tri* t, const Ctx& c, int nwin, unsigned long long key_hi, int which) {
+    const int G = t->group ? t->group : (t->D == 3 ? 16 : 8);
+    if (G == 1) {
+        const int grid = (nwin + 255) / 256;
+        if (t->D == 3) {
+            if (which) hipLaunchKernelGGL(k_commit<3>, dim3(grid), dim3(256), 0, t->s, c, nwin, key_hi);
+            else hipLaunchKernelGGL(k_cavity<3>, dim3(grid), dim3(256), 0, t->s, c, nwin, key_hi);
+        } else {
+            if (which) hipLaunchKernelGGL(k_commit<2>, dim3(grid), dim3(256), 0, t->s, c, nwin, key_hi);
+            else hipLaunchKernelGGL(k_cavity<2>, dim3(grid), dim3(256), 0, t->s, c, nwin, key_hi);
+        }
+    } else if (t->D == 3) {
+        switch (G) {
+            case 4: VO_LAUNCH_G(3, 4); break;
+            case 8: VO_LAUNCH_G(3, 8); break;
+            case 16: VO_LAUNCH_G(3, 16); break;
+            default: VO_LAUNCH_G(3, 32); break;
+        }
+    } else {
+        switch (G) {
+            case 4: VO_LAUNCH_G(2, 4); break;
+            case 8: VO_LAUNCH_G(2, 8); break;
+            case 16: VO_LAUNCH_G(2, 16); break;
+            default: VO_LAUNCH_G(2, 32); break;
+        }
+    }
+    VO_HIP(hipGetLastError());
+    return 0;
+}
+#undef VO_LAUNCH_G
 
 }  // namespace
 
@@ -394,7 +492,9 @@ int vo_tri_create(int hip_device, int dim, const double* pts, int64_t n, vo_tri*
         (rc = dev_alloc(&t->fac, (size_t)t->win_cap * t->fcap)) || (rc = dev_alloc(&t->nk, t->win_cap)) ||
         (rc = dev_alloc(&t->nf, t->win_cap)) || (rc = dev_alloc(&t->status, t->win_cap)) ||
         (rc = dev_alloc(&t->st, 1)) || (rc = dev_alloc(&t->cnt, 1)) || (rc = dev_alloc(&t->q, t->qcap)) ||
-        (rc = dev_alloc(&t->cache, (size_t)t->cmask + 1)) || (rc = dev_alloc(&t->ws, (size_t)t->ws_threads * t->ws_cap)))
+        (rc = dev_alloc(&t->cache, (size_t)t->cmask + 1)) || (rc = dev_alloc(&t->ws, (size_t)t->ws_threads * t->ws_cap)) ||
+        (rc = dev_alloc(&t->big, t->win_cap)) || (rc = dev_alloc(&t->cav1, t->big_cap)) ||
+        (rc = dev_alloc(&t->fac1, t->big_cap)) || (rc = dev_alloc(&t->misc1, 4)))
         return bail(rc);
     if (hipHostMalloc((void**)&t->st_host, sizeof(RoundState)) != hipSuccess) return bail(fail(VO_ERR_OOM, "pinned"));
     hipMemsetAsync(t->st, 0, sizeof(RoundState), t->s);
@@ -491,6 +591,13 @@ int vo_tri_run(vo_tri* t) {
     VO_HIP(hipEventRecord(t->ev[0], t->s));
     int rc = 0;
     int stalled = 0;
+    // VO_TRACE=<path>: append one line per round (diagnostics only)
+    struct TraceFile {
+        FILE* f = nullptr;
+        ~TraceFile() { if (f) std::fclose(f); }
+    } tf;
+    if (const char* tp = std::getenv("VO_TRACE")) tf.f = std::fopen(tp, "a");
+    FILE* trace = tf.f;
     while (t->head < t->n_pts || t->npend > 0) {
         // window plan
         const double target = std::max(8.0, t->gamma * (double)std::max<int64_t>(t->inserted, 1));
@@ -507,13 +614,9 @@ int vo_tri_run(vo_tri* t) {
         const unsigned long long key_hi = (unsigned long long)(0xFFFFFFFFu - (uint32_t)t->rounds) << 32;
         Ctx c = make_ctx(t);
         VO_HIP(hipEventRecord(t->ev[2], t->s));
-        if (D == 3) hipLaunchKernelGGL(k_cavity<3>, dim3(grid), dim3(256), 0, t->s, c, nwin, key_hi);
-        else hipLaunchKernelGGL(k_cavity<2>, dim3(grid), dim3(256), 0, t->s, c, nwin, key_hi);
-        VO_HIP(hipGetLastError());
+        if ((rc = launch_round(t, c, nwin, key_hi, 0))) return rc;
         VO_HIP(hipEventRecord(t->ev[3], t->s));
-        if (D == 3) hipLaunchKernelGGL(k_commit<3>, dim3(grid), dim3(256), 0, t->s, c, nwin, key_hi);
-        else hipLaunchKernelGGL(k_commit<2>, dim3(grid), dim3(256), 0, t->s, c, nwin, key_hi);
-        VO_HIP(hipGetLastError());
+        if ((rc = launch_round(t, c, nwin, key_hi, 1))) return rc;
         VO_HIP(hipEventRecord(t->ev[1], t->s));
         if ((rc = sync_state(t))) return rc;
         float a = 0, b = 0;
@@ -524,6 +627,10 @@ int vo_tri_run(vo_tri* t) {
         t->n_cav++;
         t->n_com++;
         if ((rc = check_err(t))) return rc;
+        const int n_big = t->st_host->n_big;
+        if (n_big > 0) {
+            if ((rc = run_solo(t, n_big))) return rc;
+        }
         const RoundState& r = *t->st_host;
         if (r.n_q > 0) {
             if ((rc = run_exact(t, r.n_q))) return rc;
@@ -533,6 +640,9 @@ int vo_tri_run(vo_tri* t) {
         t->n_live += r.n_live_delta;
         t->inserted += r.n_winners;
         t->windows_total += nwin;
+        if (trace)
+            std::fprintf(trace, "%lld %d %d %d %d %.4f %.4f %d %lld %d %d\n", (long long)t->rounds, nwin, t->npend,
+                         nfresh, r.n_winners, a, b, r.n_q, (long long)t->inserted, n_big, r.n_next_pending);
         const int64_t consumed = (int64_t)nwin - r.n_next_pending - r.n_winners;   // duplicates
         t->dups += consumed;
         const double frac = nwin ? (double)r.n_winners / nwin : 1.0;
@@ -728,6 +838,8 @@ int vo_tri_stats(const vo_tri* t, vo_stats* out) {
     out->ms_other = t->ms_other;
     out->launches_locate_cavity = t->n_cav;
     out->launches_commit = t->n_com;
+    out->solo_runs = t->n_solo;
+    out->duplicates_detected = (int64_t)h.dups;
     return 0;
 }
 
@@ -737,6 +849,10 @@ int vo_tri_config(vo_tri* t, const char* key, int64_t value) {
     if (k == "filters") {
         if (value < 0 || value > 3) return fail(VO_ERR_INVALID, "filters must be a mask of 1 (fp32) and 2 (fp64)");
         t->filters = (int)value;
+    } else if (k == "group_lanes") {
+        if (value != 0 && value != 1 && value != 4 && value != 8 && value != 16 && value != 32)
+            return fail(VO_ERR_INVALID, "group_lanes must be 0 (default), 1, 4, 8, 16 or 32");
+        t->group = (int)value;
     } else if (k == "window_gamma_milli") {
         if (value <= 0) return fail(VO_ERR_INVALID, "window_gamma_milli must be positive");
         t->gamma = (double)value / 1000.0;
