@@ -91,8 +91,9 @@ int vo_tri_stats(const vo_tri* t, vo_stats* out);
 /* tuning and test knobs (no counterpart in the reference): "filters" = mask of the FP32 (1)
  * and FP64-filter (2) predicate tiers, 3 by default; 0 sends every predicate to the exact
  * expansion tier.  "group_lanes" = lanes cooperating on one window point in the round
- * kernels (1, 4, 8, 16, 32; 0 = the default, 16 in 3D and 8 in 2D).  "window_gamma_milli" =
- * initial window size per inserted point x 1000. */
+ * kernels (1, 4, 8, 16, 32; 0 = chosen per round: the largest size with window x size <=
+ * "lanes_target", default 2^20 in 3D and 2^16 in 2D, from the autotuner).  "window_gamma_milli" = initial window
+ * size per inserted point x 1000. */
 int vo_tri_config(vo_tri* t, const char* key, int64_t value);
 
 /* input vertex coordinates in the reference id space (for PyVertex.point): (2(dim+1) + n) x dim */

@@ -67,6 +67,17 @@ def _circumspheres(X: np.ndarray, S: np.ndarray):
     return center, radius
 
 
+def _tuned(dim: int) -> dict:
+    """the autotuner's best round-kernel setting for this dimension (voronoids_amd/tuning.json)"""
+    import json
+    import os
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuning.json")
+    if not os.path.exists(path):
+        return {}
+    with open(path) as f:
+        return json.load(f).get(str(dim), {}).get("setting", {})
+
+
 class DelaunayTree:
     def __init__(self, vertices, device: int = 0):
         p = _as_points(vertices)
@@ -76,6 +87,8 @@ class DelaunayTree:
         _lib.check(_lib.lib().vo_tri_create(device, self.dim, p.ctypes.data_as(_dp), p.shape[0], ctypes.byref(h)))
         self._h = h
         self._n_input = 0
+        for key, value in _tuned(self.dim).items():
+            self.config(key, value)
 
     def close(self) -> None:
         if getattr(self, "_h", None):

@@ -53,6 +53,7 @@ struct RoundState {
 struct Ctx {
     int4* tv;
     int4* ta;
+    float4* tx;            // per-simplex fp32 vertex coordinates (4 x float4 in 3D, 2 in 2D)
     unsigned long long* kres;
     unsigned long long* nres;
     const double* X64;
@@ -188,22 +189,38 @@ __device__ __forceinline__ void enqueue(const Ctx& c, uint32_t type, const uint3
     }
 }
 
-// ---------------------------------------------------------------- predicates on ids
-// p strictly inside the circumsphere of simplex v (positively oriented): > 0
-template <int D> __device__ __forceinline__ int conflict(const Ctx& c, int4 v, int pv,
-                                                         typename VT<D>::F pf, LocalCnt& lc) {
-    using V = VT<D>;
-    int s = SIGN_UNDECIDED;
-    if (c.filters & 1) {
-        if constexpr (D == 3) {
-            s = t1_insphere(V::ld(c.X32, v.x), V::ld(c.X32, v.y), V::ld(c.X32, v.z), V::ld(c.X32, v.w), pf);
-        } else {
-            s = t1_incircle(V::ld(c.X32, v.x), V::ld(c.X32, v.y), V::ld(c.X32, v.z), pf);
-        }
-        if (s != SIGN_UNDECIDED) { lc.t1[1]++; return s; }
+// ---------------------------------------------------------------- predicates
+// Tier 1 reads the simplex's own coordinate record tx[t] (the fp32 coordinates of its
+// vertices, in vertex order); tiers 2 and 3 need vertex ids and load tv[t] only then.
+template <int D> constexpr int kTxStride = D == 3 ? 4 : 2;   // float4 per simplex record
+
+template <int D> __device__ __forceinline__ void load_tx(const Ctx& c, int t, typename VT<D>::F* a) {
+    const float4* q = c.tx + (size_t)kTxStride<D> * t;
+    if constexpr (D == 3) {
+        a[0] = q[0]; a[1] = q[1]; a[2] = q[2]; a[3] = q[3];
+    } else {
+        const float4 q0 = q[0], q1 = q[1];
+        a[0] = make_float2(q0.x, q0.y);
+        a[1] = make_float2(q0.z, q0.w);
+        a[2] = make_float2(q1.x, q1.y);
     }
+}
+
+template <int D> __device__ __forceinline__ void store_tx(const Ctx& c, int t, const typename VT<D>::F* a) {
+    float4* q = c.tx + (size_t)kTxStride<D> * t;
+    if constexpr (D == 3) {
+        q[0] = a[0]; q[1] = a[1]; q[2] = a[2]; q[3] = a[3];
+    } else {
+        q[0] = make_float4(a[0].x, a[0].y, a[1].x, a[1].y);
+        q[1] = make_float4(a[2].x, a[2].y, 0.0f, 0.0f);
+    }
+}
+
+// tiers 2 and 3 of the in-sphere test of vertex pv against simplex v
+template <int D> __device__ int conflict_slow(const Ctx& c, int4 v, int pv, LocalCnt& lc) {
+    constexpr int S = VT<D>::S;
     const double* X = c.X64;
-    constexpr int S = V::S;
+    int s = SIGN_UNDECIDED;
     if (c.filters & 2) {
         if (D == 3)
             s = t2_insphere(X + S * v.x, X + S * v.y, X + S * v.z, X + S * v.w, X + S * pv);
@@ -212,7 +229,7 @@ template <int D> __device__ __forceinline__ int conflict(const Ctx& c, int4 v, i
         if (s != SIGN_UNDECIDED) { lc.t2[1]++; return s; }
     }
     uint32_t ids[5];
-    uint32_t type = D == 3 ? PRED_INSPHERE : PRED_INCIRCLE;
+    const uint32_t type = D == 3 ? PRED_INSPHERE : PRED_INCIRCLE;
     if (D == 3) { ids[0] = v.x; ids[1] = v.y; ids[2] = v.z; ids[3] = v.w; ids[4] = pv; }
     else { ids[0] = v.x; ids[1] = v.y; ids[2] = v.z; ids[3] = pv; ids[4] = kEmpty; }
     s = cache_lookup(c.cache, c.cmask, type, ids);
@@ -222,34 +239,65 @@ template <int D> __device__ __forceinline__ int conflict(const Ctx& c, int4 v, i
     return SIGN_UNDECIDED;
 }
 
-// orientation of simplex v with vertex k replaced by p: < 0 means p lies beyond face k
-template <int D> __device__ __forceinline__ int orient_sub(const Ctx& c, int4 v, int k, int pv,
-                                                           typename VT<D>::F pf, LocalCnt& lc) {
-    using V = VT<D>;
-    typename V::F a[4];
-    int ids[4] = {v.x, v.y, v.z, v.w};
-    ids[k] = pv;
-    for (int i = 0; i < D + 1; ++i) a[i] = (i == k) ? pf : V::ld(c.X32, ids[i]);
-    int s = SIGN_UNDECIDED;
-    if (c.filters & 1) {
-        if constexpr (D == 3) s = t1_orient3d(a[0], a[1], a[2], a[3]);
-        else s = t1_orient2d(a[0], a[1], a[2]);
-        if (s != SIGN_UNDECIDED) { lc.t1[0]++; return s; }
-    }
+// tiers 2 and 3 of an orientation test on vertex ids
+template <int D> __device__ int orient_slow(const Ctx& c, const int* ids, LocalCnt& lc) {
+    constexpr int S = VT<D>::S;
     const double* X = c.X64;
-    constexpr int S = V::S;
+    int s = SIGN_UNDECIDED;
     if (c.filters & 2) {
         if (D == 3) s = t2_orient3d(X + S * ids[0], X + S * ids[1], X + S * ids[2], X + S * ids[3]);
         else s = t2_orient2d(X + S * ids[0], X + S * ids[1], X + S * ids[2]);
         if (s != SIGN_UNDECIDED) { lc.t2[0]++; return s; }
     }
     uint32_t u[5] = {(uint32_t)ids[0], (uint32_t)ids[1], (uint32_t)ids[2], D == 3 ? (uint32_t)ids[3] : kEmpty, kEmpty};
-    uint32_t type = D == 3 ? PRED_ORIENT3 : PRED_ORIENT2;
+    const uint32_t type = D == 3 ? PRED_ORIENT3 : PRED_ORIENT2;
     s = cache_lookup(c.cache, c.cmask, type, u);
     if (s != SIGN_UNDECIDED) { lc.t3[0]++; return s; }
     lc.def[0]++;
     enqueue(c, type, u);
     return SIGN_UNDECIDED;
+}
+
+// p (vertex pv, coordinates pf) strictly inside the circumsphere of simplex t whose vertex
+// coordinates are a[]: > 0
+template <int D> __device__ __forceinline__ int conflict_at(const Ctx& c, const typename VT<D>::F* a, int t, int pv,
+                                                            typename VT<D>::F pf, LocalCnt& lc) {
+    if (c.filters & 1) {
+        int s;
+        if constexpr (D == 3) s = t1_insphere(a[0], a[1], a[2], a[3], pf);
+        else s = t1_incircle(a[0], a[1], a[2], pf);
+        if (s != SIGN_UNDECIDED) { lc.t1[1]++; return s; }
+    }
+    return conflict_slow<D>(c, c.tv[t], pv, lc);
+}
+
+// orientation of simplex t (coordinates a[]) with vertex k replaced by p: < 0 means p lies
+// beyond face k
+template <int D> __device__ __forceinline__ int orient_at(const Ctx& c, const typename VT<D>::F* a, int t, int k,
+                                                          int pv, typename VT<D>::F pf, LocalCnt& lc) {
+    if (c.filters & 1) {
+        typename VT<D>::F b[4];
+        for (int i = 0; i < D + 1; ++i) b[i] = i == k ? pf : a[i];
+        int s;
+        if constexpr (D == 3) s = t1_orient3d(b[0], b[1], b[2], b[3]);
+        else s = t1_orient2d(b[0], b[1], b[2]);
+        if (s != SIGN_UNDECIDED) { lc.t1[0]++; return s; }
+    }
+    const int4 v = c.tv[t];
+    int ids[4] = {v.x, v.y, v.z, v.w};
+    ids[k] = pv;
+    return orient_slow<D>(c, ids, lc);
+}
+
+// position in the old simplex of vertex k (k >= 1) of replace_first(v, i, p)
+template <int D> __device__ __forceinline__ int first_src(int i, int k) {
+    if (D == 3) {
+        const int code = i == 0 ? 0x321 : (i == 1 ? 0x230 : (i == 2 ? 0x310 : 0x120));
+        return (code >> (4 * (k - 1))) & 0xF;
+    } else {
+        const int code = i == 0 ? 0x21 : (i == 1 ? 0x02 : 0x10);
+        return (code >> (4 * (k - 1))) & 0xF;
+    }
 }
 
 // cell of pf at level l of the hint grid (levels stored coarse to fine, one after another)
@@ -314,6 +362,8 @@ __global__ void __launch_bounds__(256) k_cavity(Ctx c, int nwin, unsigned long l
         // remembering stochastic visibility walk
         int came = -1;
         uint32_t rng = mix32((uint32_t)p * 0x9E3779B9u + 0x632BE5ABu);
+        typename V::F a[4];
+        load_tx<D>(c, t, a);
         for (int step = 0; st == ST_OK; ++step) {
             if (step > kWalkLimit) { atomicOr(&c.st->err, ERR_WALK); st = ST_BROKEN; break; }
             const int4 aa = c.ta[t];
@@ -325,13 +375,13 @@ __global__ void __launch_bounds__(256) k_cavity(Ctx c, int nwin, unsigned long l
                 if (k >= M) k -= M;
                 const int nb = comp(aa, k);
                 if (nb >= 0 && (nb >> 2) == came) continue;
-                const int s = orient_sub<D>(c, tt, k, pv, pf, lc);
+                const int s = orient_at<D>(c, a, t, k, pv, pf, lc);
                 if (s == SIGN_UNDECIDED) { st = ST_DEFER; break; }
                 if (s < 0) {
                     if (nb < 0) { atomicOr(&c.st->err, ERR_HULL); st = ST_BROKEN; break; }
                     came = t;
                     t = nb >> 2;
-                    tt = c.tv[t];
+                    load_tx<D>(c, t, a);
                     moved = 1;
                     break;
                 }
@@ -343,7 +393,7 @@ __global__ void __launch_bounds__(256) k_cavity(Ctx c, int nwin, unsigned long l
         hint_store<D>(c, pf, t);
         int nk = 0, nf = 0;
         if (st == ST_OK) {
-            const int s0 = conflict<D>(c, tt, pv, pf, lc);
+            const int s0 = conflict_at<D>(c, a, t, pv, pf, lc);
             if (s0 == SIGN_UNDECIDED) st = ST_DEFER;
             else if (s0 <= 0) st = ST_DUP;   // on the circumsphere of its own simplex: a vertex
         }
@@ -369,7 +419,9 @@ __global__ void __launch_bounds__(256) k_cavity(Ctx c, int nwin, unsigned long l
                             for (int j = KL; j < nk && !seen; ++j) seen = c.cav[j * W + slot] == n;
                         }
                         if (seen) continue;
-                        const int s = conflict<D>(c, c.tv[n], pv, pf, lc);
+                        typename V::F an[4];
+                        load_tx<D>(c, n, an);
+                        const int s = conflict_at<D>(c, an, n, pv, pf, lc);
                         if (s == SIGN_UNDECIDED) { st = ST_DEFER; break; }
                         if (s > 0) {
                             if (nk == c.kcap) { st = ST_OVERFLOW; break; }
@@ -453,6 +505,7 @@ __global__ void __launch_bounds__(256) k_commit(Ctx c, int nwin, unsigned long l
     }
     if (win) {
         const int pv = c.M + p;
+        const typename VT<D>::F pf = VT<D>::ld(c.X32, pv);
         // phase 1: the new simplex of facet j takes slot base + j; its face 0 faces the outside
         // neighbor, which is patched to point back; the cavity side of the facet is overwritten
         // with the marker -2 - (base + j) so that ridge rotations find it without a search
@@ -508,6 +561,11 @@ __global__ void __launch_bounds__(256) k_commit(Ctx c, int nwin, unsigned long l
             }
             if (D == 2) *comp_ptr(&c.ta[N], 3) = -1;
             c.tv[N] = nv;
+            typename VT<D>::F ox[4], nx[4];
+            load_tx<D>(c, t, ox);
+            nx[0] = pf;
+            for (int k = 1; k < M; ++k) nx[k] = ox[first_src<D>(i, k)];
+            store_tx<D>(c, N, nx);
         }
         // phase 3: killed simplices die with a forward pointer into the star
         for (int j = 0; j < nk; ++j) c.tv[c.cav[j * W + slot]] = make_int4(-1, base, 0, 0);
@@ -629,7 +687,15 @@ __global__ void __launch_bounds__(256) k_check(Ctx c, int n, unsigned long long*
         const int4 v = c.tv[t];
         if (v.x >= 0) {
             const int4 a = c.ta[t];
-            int s = orient_sub<D>(c, v, 0, v.x, V::ld(c.X32, v.x), lc);
+            typename V::F x[4], x2[4];
+            load_tx<D>(c, t, x);
+            bool coords_ok = true;                   // the record matches the vertex table
+            for (int k = 0; k < M; ++k) {
+                const typename V::F g = V::ld(c.X32, comp(v, k));
+                coords_ok &= g.x == x[k].x && g.y == x[k].y;
+            }
+            if (!coords_ok) nbad++;
+            int s = orient_at<D>(c, x, t, 0, v.x, x[0], lc);
             if (s == SIGN_UNDECIDED) nun++;
             else if (s <= 0) nbad++;
             for (int f = 0; f < M; ++f) {
@@ -639,7 +705,8 @@ __global__ void __launch_bounds__(256) k_check(Ctx c, int n, unsigned long long*
                 const int4 v2 = c.tv[n2];
                 if (v2.x < 0 || comp(c.ta[n2], f2) != ((t << 2) | f)) { nbad++; continue; }
                 const int w = comp(v2, f2);
-                const int cs = conflict<D>(c, v, w, V::ld(c.X32, w), lc);
+                load_tx<D>(c, n2, x2);
+                const int cs = conflict_at<D>(c, x, t, w, x2[f2], lc);
                 if (cs == SIGN_UNDECIDED) nun++;
                 else if (cs > 0) nbad++;
             }

@@ -81,6 +81,8 @@ __global__ void __launch_bounds__(64) k_cavity_g(Ctx c, int nwin, unsigned long 
     bool walking = st == ST_OK;
     int came = -1;
     uint32_t rng = mix32((uint32_t)p * 0x9E3779B9u + 0x632BE5ABu);
+    F a[4];
+    if (walking) load_tx<D>(c, t, a);
     for (int step = 0; __any(walking); ++step) {
         int s = 1, nb = -1;
         bool test = false;
@@ -93,7 +95,7 @@ __global__ void __launch_bounds__(64) k_cavity_g(Ctx c, int nwin, unsigned long 
             if (g.gl < M) {
                 nb = comp(aa, k);
                 test = !(nb >= 0 && (nb >> 2) == came);
-                if (test) s = orient_sub<D>(c, tt, k, pv, pf, lc);
+                if (test) s = orient_at<D>(c, a, t, k, pv, pf, lc);
             }
         }
         const unsigned long long und = g.ballot(test && s == SIGN_UNDECIDED);
@@ -114,7 +116,7 @@ __global__ void __launch_bounds__(64) k_cavity_g(Ctx c, int nwin, unsigned long 
             } else {
                 came = t;
                 t = nb_sel >> 2;
-                tt = c.tv[t];
+                load_tx<D>(c, t, a);
             }
         }
     }
@@ -126,7 +128,7 @@ __global__ void __launch_bounds__(64) k_cavity_g(Ctx c, int nwin, unsigned long 
     }
     // ---- the containing simplex must conflict (else p repeats one of its vertices)
     int s0 = 0;
-    if (st == ST_OK && lead) s0 = conflict<D>(c, tt, pv, pf, lc);
+    if (st == ST_OK && lead) s0 = conflict_at<D>(c, a, t, pv, pf, lc);
     s0 = g.bcast(s0, 0);
     if (st == ST_OK) {
         if (s0 == SIGN_UNDECIDED) st = ST_DEFER;
@@ -157,7 +159,11 @@ __global__ void __launch_bounds__(64) k_cavity_g(Ctx c, int nwin, unsigned long 
                 bool inC = false;
                 for (int j = 0; j < nk && !inC; ++j) inC = s_cav[g.gid][j] == n;
                 test = !inC;
-                if (test) s = conflict<D>(c, c.tv[n], pv, pf, lc);
+                if (test) {
+                    F an[4];
+                    load_tx<D>(c, n, an);
+                    s = conflict_at<D>(c, an, n, pv, pf, lc);
+                }
             }
         }
         const unsigned long long und = g.ballot(test && s == SIGN_UNDECIDED);
@@ -262,6 +268,7 @@ __global__ void __launch_bounds__(64) k_commit_g(Ctx c, int nwin, unsigned long 
     __builtin_amdgcn_wave_barrier();
     const int pv = c.M + p;
     if (build) {
+        const typename VT<D>::F pf = VT<D>::ld(c.X32, pv);
         // phase 1: the new simplex of facet j, its outside link and the outside neighbor's patch
         for (int j = g.gl; j < nf; j += G) {
             const int fe = s_fac[g.gid][j];
@@ -270,6 +277,11 @@ __global__ void __launch_bounds__(64) k_commit_g(Ctx c, int nwin, unsigned long 
             const int outside = ta_get(c.ta, t, i);
             const int N = base + j;
             c.tv[N] = nv;
+            typename VT<D>::F ox[4], nx[4];
+            load_tx<D>(c, t, ox);
+            nx[0] = pf;
+            for (int k = 1; k < M; ++k) nx[k] = ox[first_src<D>(i, k)];
+            store_tx<D>(c, N, nx);
             *comp_ptr(&c.ta[N], 0) = outside;
             if (D == 2) *comp_ptr(&c.ta[N], 3) = -1;
             if (outside >= 0) *comp_ptr(&c.ta[outside >> 2], outside & 3) = N << 2;

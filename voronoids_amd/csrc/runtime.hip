@@ -120,6 +120,7 @@ struct vo_tri {
     // simplices
     int4* tv = nullptr;
     int4* ta = nullptr;
+    float4* tx = nullptr;              // per-simplex fp32 vertex coordinates
     unsigned long long* kres = nullptr;
     unsigned long long* nres = nullptr;
     int64_t tcap = 0;
@@ -159,7 +160,8 @@ struct vo_tri {
     hipEvent_t ev[6] = {};
     double gamma = 0.25;
     int filters = 3;
-    int group = 0;                     // lanes per window point in the round kernels (1 = one thread)
+    int group = 0;                     // lanes per window point (0 = chosen per round, 1 = one thread)
+    int64_t lanes_target = 0;          // auto group size: largest G with window * G <= this (0 = default)
     int64_t inserted = 0, dups = 0, rounds = 0, windows_total = 0;
     double ms_run = 0, ms_cav = 0, ms_com = 0, ms_ex = 0, ms_other = 0;
     int64_t n_cav = 0, n_com = 0, n_solo = 0;
@@ -172,6 +174,7 @@ Ctx make_ctx(vo_tri* t) {
     Ctx c;
     c.tv = t->tv;
     c.ta = t->ta;
+    c.tx = t->tx;
     c.kres = t->kres;
     c.nres = t->nres;
     c.X64 = t->X64;
@@ -225,9 +228,12 @@ int grow_tets(vo_tri* t, int64_t cap) {
     if (cap <= t->tcap) return 0;
     if (cap > (int64_t)1 << 30) return fail(VO_ERR_OOM, "simplex capacity above 2^30");
     int4 *tv, *ta;
+    float4* tx;
     unsigned long long *kr, *nr;
     int rc;
-    if ((rc = dev_alloc(&tv, cap)) || (rc = dev_alloc(&ta, cap)) || (rc = dev_alloc(&kr, cap)) || (rc = dev_alloc(&nr, cap)))
+    const int txs = t->D == 3 ? 4 : 2;
+    if ((rc = dev_alloc(&tv, cap)) || (rc = dev_alloc(&ta, cap)) || (rc = dev_alloc(&kr, cap)) ||
+        (rc = dev_alloc(&nr, cap)) || (rc = dev_alloc(&tx, (size_t)cap * txs)))
         return rc;
     VO_HIP(hipMemsetAsync(kr, 0xFF, sizeof(unsigned long long) * cap, t->s));
     VO_HIP(hipMemsetAsync(nr, 0xFF, sizeof(unsigned long long) * cap, t->s));
@@ -236,10 +242,11 @@ int grow_tets(vo_tri* t, int64_t cap) {
         VO_HIP(hipMemcpyAsync(ta, t->ta, sizeof(int4) * t->n_tets, hipMemcpyDeviceToDevice, t->s));
         VO_HIP(hipMemcpyAsync(kr, t->kres, sizeof(unsigned long long) * t->n_tets, hipMemcpyDeviceToDevice, t->s));
         VO_HIP(hipMemcpyAsync(nr, t->nres, sizeof(unsigned long long) * t->n_tets, hipMemcpyDeviceToDevice, t->s));
+        VO_HIP(hipMemcpyAsync(tx, t->tx, sizeof(float4) * txs * t->n_tets, hipMemcpyDeviceToDevice, t->s));
         VO_HIP(hipStreamSynchronize(t->s));
-        hipFree(t->tv); hipFree(t->ta); hipFree(t->kres); hipFree(t->nres);
+        hipFree(t->tv); hipFree(t->ta); hipFree(t->kres); hipFree(t->nres); hipFree(t->tx);
     }
-    t->tv = tv; t->ta = ta; t->kres = kr; t->nres = nr;
+    t->tv = tv; t->ta = ta; t->kres = kr; t->nres = nr; t->tx = tx;
     t->tcap = cap;
     return 0;
 }
@@ -288,7 +295,7 @@ int upload_super(vo_tri* t) {
 }
 
 void free_all(vo_tri* t) {
-    void* ptrs[] = {t->X64, t->X32, t->cp, t->vmap, t->tv, t->ta, t->kres, t->nres, t->win, t->pend[0], t->pend[1],
+    void* ptrs[] = {t->X64, t->X32, t->cp, t->vmap, t->tv, t->ta, t->tx, t->kres, t->nres, t->win, t->pend[0], t->pend[1],
                     t->cav, t->fac, t->nk, t->nf, t->status, t->hint, t->st, t->cnt, t->q, t->cache, t->ws,
                     t->big, t->cav1, t->fac1, t->misc1};
     for (void* p : ptrs)
@@ -389,8 +396,21 @@ int run_solo(vo_tri* t, int n_big) {
     } while (0)
 
 // which = 0: locate + cavity + reservation, 1: check + commit
+// lanes per point for this round: small windows are latency-bound (more lanes per point
+// shorten each point's dependent-load chain), large windows are throughput-bound (one lane
+// per point keeps every SIMD lane busy)
+int auto_group(const vo_tri* t, int nwin) {
+    if (t->group) return t->group;
+    const int64_t target = t->lanes_target ? t->lanes_target : (t->D == 3 ? (1 << 20) : (1 << 16));
+    static const int sizes[] = {32, 16, 8, 4};
+    const int gmax = t->D == 3 ? 32 : 16;
+    for (int g : sizes)
+        if (g <= gmax && (int64_t)nwin * g <= target) return g;
+    return 1;
+}
+
 int launch_round(vo_tri* t, const Ctx& c, int nwin, unsigned long long key_hi, int which) {
-    const int G = t->group ? t->group : (t->D == 3 ? 16 : 8);
+    const int G = auto_group(t, nwin);
     if (G == 1) {
         const int grid = (nwin + 255) / 256;
         if (t->D == 3) {
@@ -505,6 +525,19 @@ int vo_tri_create(int hip_device, int dim, const double* pts, int64_t n, vo_tri*
     if ((rc = grow_tets(t, 1024))) return bail(rc);
     hipLaunchKernelGGL(k_init_first, dim3(1), dim3(1), 0, t->s, t->tv, t->ta, t->tet0[0], t->tet0[1], t->tet0[2],
                        t->tet0[3]);
+    {
+        float rec[16] = {};
+        for (int k = 0; k < t->M; ++k) {
+            const int v = t->tet0[k];
+            for (int j = 0; j < dim; ++j) {
+                const float x = (float)(t->sv[v][j] - t->c[j]);
+                if (dim == 3) rec[4 * k + j] = x;
+                else rec[2 * k + j] = x;
+            }
+        }
+        if (hipMemcpyAsync(t->tx, rec, sizeof(float) * (dim == 3 ? 16 : 8), hipMemcpyHostToDevice, t->s) != hipSuccess)
+            return bail(fail(VO_ERR_HIP, "init record"));
+    }
     RoundState init = {};
     init.n_tets = 1;
     if (hipMemcpyAsync(t->st, &init, sizeof(init), hipMemcpyHostToDevice, t->s) != hipSuccess ||
@@ -851,8 +884,11 @@ int vo_tri_config(vo_tri* t, const char* key, int64_t value) {
         t->filters = (int)value;
     } else if (k == "group_lanes") {
         if (value != 0 && value != 1 && value != 4 && value != 8 && value != 16 && value != 32)
-            return fail(VO_ERR_INVALID, "group_lanes must be 0 (default), 1, 4, 8, 16 or 32");
+            return fail(VO_ERR_INVALID, "group_lanes must be 0 (per round), 1, 4, 8, 16 or 32");
         t->group = (int)value;
+    } else if (k == "lanes_target") {
+        if (value < 0) return fail(VO_ERR_INVALID, "lanes_target must be >= 0");
+        t->lanes_target = value;
     } else if (k == "window_gamma_milli") {
         if (value <= 0) return fail(VO_ERR_INVALID, "window_gamma_milli must be positive");
         t->gamma = (double)value / 1000.0;
