@@ -47,6 +47,9 @@ WORKLOADS = {
 }
 
 
+TRAFFIC_PROFILES = {"3d4m": "traffic_round_kernels_3d4m_uniform.md"}
+
+
 def make_points(w: dict) -> np.ndarray:
     from voronoids_amd import datasets
     if w["kind"] == "uniform":
@@ -86,6 +89,40 @@ def cpu_baseline(dim: int, threads: int) -> dict:
                       f"{dt:.1f} s wall"}
 
 
+def measured_traffic(workload: str, kernel: str):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 counter summary of this
+    workload (FETCH_SIZE and WRITE_SIZE passes, each scaled by what the counter reads for a
+    device-to-device copy of known size), or None when no summary exists."""
+    path = os.path.join(ROOT, "profiles", TRAFFIC_PROFILES.get(workload, "-"))
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("traffic_json: "):
+            rec = json.loads(line[len("traffic_json: "):])
+            k = rec["kernels"].get(kernel)
+            return None if k is None else {"bytes_per_launch": k["traffic_per_launch"],
+                                           "fetch_factor": rec["fetch_factor"], "write_factor": rec["write_factor"],
+                                           "source": "profiles/" + TRAFFIC_PROFILES[workload]}
+    return None
+
+
+def copy_calibration(nbytes: int, reps: int = 3) -> None:
+    """Device-to-device copies of a known size (the runtime's copy kernel), so that rocprofv3's
+    FETCH_SIZE / WRITE_SIZE can be scaled to bytes."""
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so")
+    src, dst = ctypes.c_void_p(), ctypes.c_void_p()
+    assert hip.hipMalloc(ctypes.byref(src), ctypes.c_size_t(nbytes)) == 0
+    assert hip.hipMalloc(ctypes.byref(dst), ctypes.c_size_t(nbytes)) == 0
+    assert hip.hipMemset(src, 1, ctypes.c_size_t(nbytes)) == 0
+    for _ in range(reps):
+        assert hip.hipMemcpy(dst, src, ctypes.c_size_t(nbytes), 3) == 0      # hipMemcpyDeviceToDevice
+    assert hip.hipDeviceSynchronize() == 0
+    hip.hipFree(src)
+    hip.hipFree(dst)
+    print(json.dumps({"copy_calibration_bytes": nbytes, "reps": reps}), flush=True)
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -94,10 +131,15 @@ def main() -> int:
     ap.add_argument("--workload", default="3d4m", choices=sorted(WORKLOADS))
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--check", action="store_true", help="run the exact device check after the last step")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
+                    help="only run device-to-device copies of BYTES (for rocprofv3 counter scaling)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration)
+        return 0
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (asked for {args.gpus}); "
-              "the multi-GPU sharding is a design in DESIGN.md, not a measured path", file=sys.stderr)
+              "multi-GPU sharding is designed but not built or measured", file=sys.stderr)
         return 2
 
     import voronoids_amd as vo
@@ -138,6 +180,7 @@ def main() -> int:
     launches = {"k_cavity": st["launches_locate_cavity"], "k_commit": st["launches_commit"]}
     dom = max(kt, key=kt.get)
     achieved = alg[dom] / (kt[dom] / 1e3) / 1e9
+    traffic = measured_traffic(args.workload, dom)
     tot_calls = [st["tier1"][i] + st["tier2"][i] + st["tier3"][i] + st["deferred"][i] for i in range(2)]
     tiers = {name: {"fp32": st["tier1"][i] / max(tot_calls[i], 1), "fp64_filter": st["tier2"][i] / max(tot_calls[i], 1),
                     "fp64_expansion": st["tier3"][i] / max(tot_calls[i], 1),
@@ -150,7 +193,9 @@ def main() -> int:
         "dtype": "f64", "data": "synthetic",
         "config": {"workload": w["label"], "n": n, "dim": dim, "timed": "vo_tri_run, inputs resident in HBM"},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": None if traffic is None else traffic["bytes_per_launch"],
+                     "traffic_source": traffic,
                      "alg_bytes_per_launch": alg[dom] / max(launches[dom], 1),
                      "avg_launch_ms": kt[dom] / max(launches[dom], 1), "launches": launches[dom]},
         "kernels_ms": kt, "rounds": st["rounds"], "tiers": tiers,
