@@ -102,6 +102,82 @@ int vo_tri_vertices(const vo_tri* t, double* coords);
 const char* vo_last_error(void);
 int vo_version(void);
 
+/* ------------------------------------------------------------------------------------------
+ * Several independent point sets on one device (batched and served modes).
+ *
+ * No counterpart in the reference, whose unit of work is one DelaunayTree: each set here is
+ * the same construction as vo_tri_create + vo_tri_insert(pts, n, seed) on its own points
+ * (its own bounding sphere and super simplex, its own priority order from seed) and yields the
+ * same simplices; the sets share the device arrays and every round's kernels.  A vo_engine
+ * holds sets of one dimension; a vo_server runs a 2D and a 3D engine behind a request queue.
+ * ------------------------------------------------------------------------------------------ */
+typedef struct vo_engine vo_engine;
+
+typedef struct {
+    int64_t tag;            /* the caller's tag given at submission */
+    int32_t dim;
+    int32_t status;         /* VO_OK, or VO_ERR_DUPLICATE (coincident points skipped, rest inserted) */
+    int64_t n_points;
+    int64_t points_duplicate;
+    int64_t n_simplices;
+    int32_t* simplices;     /* n_simplices x (dim+1) live simplices, reference id space (input
+                               point i = 2(dim+1) + i, super vertices 0..dim); release with vo_free */
+    double t_submit, t_admit, t_done;   /* seconds on vo_clock(): queued, resident, result ready */
+    int64_t rounds;         /* engine rounds the set was resident for */
+} vo_set_result;
+
+typedef struct {
+    int64_t sets_submitted, sets_admitted, sets_done, sets_resident, sets_queued;
+    int64_t points_inserted, rounds, replays, solo_runs, range_extensions, admission_waits;
+    int64_t vertex_capacity, simplex_capacity, simplex_in_use, max_sets_resident;
+    double ms_replays, ms_boundary;      /* device time inside graph replays; host time between them */
+} vo_engine_stats;
+
+/* max_points: vertex-arena capacity (points resident at once, super vertices included);
+ * max_simplices: simplex-arena capacity (< 2^29; 0 = 30 (3D) / 8 (2D) per point).  Device
+ * memory is about 48 B per vertex plus 112 B (3D) / 80 B (2D) per simplex, allocated once. */
+int vo_engine_create(int hip_device, int dim, int64_t max_points, int64_t max_simplices, vo_engine** out);
+void vo_engine_destroy(vo_engine* e);
+/* queue one set (copied); it becomes resident when its footprint fits */
+int vo_engine_submit(vo_engine* e, const double* pts, int64_t n, uint64_t seed, int64_t tag);
+/* admit what fits, replay the captured rounds once, re-run oversized cavities, extend full
+ * simplex ranges, export and free completed sets.  n_left = sets queued or resident after. */
+int vo_engine_step(vo_engine* e, int64_t* n_left);
+/* the same step split in two: launch returns once the replay is queued on the engine's stream
+ * (so that engines on other streams can overlap it); finish waits for it and does the rest */
+int vo_engine_launch(vo_engine* e);
+int vo_engine_finish(vo_engine* e, int64_t* n_left);
+/* move up to max completed results into out (ownership of simplices passes to the caller) */
+int vo_engine_results(vo_engine* e, vo_set_result* out, int64_t max, int64_t* n_out);
+int vo_engine_stats_get(const vo_engine* e, vo_engine_stats* out);
+/* knobs: "rounds_per_replay" (1..64, default 8), "lanes_target" (as vo_tri_config),
+ * "group_lanes" (0 = per replay, or 1/4/8/16/32), "min_window" (least fresh points per set
+ * and round, default 8), "window_cap" (points per round, up to the allocated 2^20/2^21),
+ * "tets_per_point_milli" (first simplex range per point x 1000, default 30000 (3D) /
+ * 7000 (2D)), "window_gamma_milli" (initial window per inserted point x 1000), "filters" (as
+ * vo_tri_config) */
+int vo_engine_config(vo_engine* e, const char* key, int64_t value);
+/* batched mode: n_sets sets, set k = points set_offsets[k] .. set_offsets[k+1]-1 of pts
+ * (row-major, dim columns), seed of set k = seed + k; runs to completion and writes results[k]
+ * for every set (tag = k).  Returns the first error; results of completed sets are valid. */
+int vo_batch_triangulate(vo_engine* e, const double* pts, const int64_t* set_offsets, int64_t n_sets,
+                         uint64_t seed, vo_set_result* results);
+/* canonical T / E (as vo_tri_canonical) of a result's simplices; arrays released with vo_free */
+int vo_result_canonical(const vo_set_result* r, int32_t** T, int64_t* n_T, int32_t** E, int64_t* n_E);
+double vo_clock(void);
+
+/* served mode: requests of either dimension arrive from any thread; a driver thread admits
+ * them into a 2D and a 3D engine by device-memory footprint as slots free up, replays both
+ * engines' rounds on two streams, and queues results as sets complete (continuous batching). */
+typedef struct vo_server vo_server;
+int vo_server_create(int hip_device, int64_t max_points_2d, int64_t max_points_3d, vo_server** out);
+int vo_server_submit(vo_server* s, int dim, const double* pts, int64_t n, uint64_t seed, int64_t tag);
+/* completed results, up to max; waits up to timeout_ms for at least one when none is ready */
+int vo_server_results(vo_server* s, vo_set_result* out, int64_t max, int64_t* n_out, int timeout_ms);
+int vo_server_stats_get(const vo_server* s, int dim, vo_engine_stats* out);
+/* stops the driver after the work in hand; results not collected are freed */
+void vo_server_destroy(vo_server* s);
+
 #ifdef __cplusplus
 }
 #endif

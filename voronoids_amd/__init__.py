@@ -22,8 +22,10 @@ import numpy as np
 
 from . import _lib
 from ._lib import VoError, VoStats
+from .batch import Engine, Server, SetResult, batch_triangulate
 
-__all__ = ["DelaunayTree", "PyDelauanyTree", "PyVertex", "PySimplex", "delaunay", "VoError"]
+__all__ = ["DelaunayTree", "PyDelauanyTree", "PyVertex", "PySimplex", "delaunay", "VoError", "Engine", "Server",
+           "SetResult", "batch_triangulate"]
 
 _dp = ctypes.POINTER(ctypes.c_double)
 _i32p = ctypes.POINTER(ctypes.c_int32)

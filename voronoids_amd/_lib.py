@@ -23,7 +23,32 @@ EXPORTED = [
     "vo_tri_create", "vo_tri_destroy", "vo_tri_insert", "vo_tri_stage", "vo_tri_run", "vo_tri_counts",
     "vo_tri_export", "vo_tri_canonical", "vo_free", "vo_tri_check", "vo_tri_stats", "vo_tri_config", "vo_tri_vertices",
     "vo_last_error", "vo_version",
+    "vo_engine_create", "vo_engine_destroy", "vo_engine_submit", "vo_engine_step", "vo_engine_launch",
+    "vo_engine_finish", "vo_engine_results", "vo_engine_stats_get", "vo_engine_config", "vo_batch_triangulate",
+    "vo_result_canonical", "vo_clock",
+    "vo_server_create", "vo_server_submit", "vo_server_results", "vo_server_stats_get", "vo_server_destroy",
 ]
+
+
+class VoSetResult(ctypes.Structure):
+    _fields_ = [
+        ("tag", ctypes.c_int64), ("dim", ctypes.c_int32), ("status", ctypes.c_int32),
+        ("n_points", ctypes.c_int64), ("points_duplicate", ctypes.c_int64), ("n_simplices", ctypes.c_int64),
+        ("simplices", ctypes.POINTER(ctypes.c_int32)),
+        ("t_submit", ctypes.c_double), ("t_admit", ctypes.c_double), ("t_done", ctypes.c_double),
+        ("rounds", ctypes.c_int64),
+    ]
+
+
+class VoEngineStats(ctypes.Structure):
+    _fields_ = [(name, ctypes.c_int64) for name in (
+        "sets_submitted", "sets_admitted", "sets_done", "sets_resident", "sets_queued",
+        "points_inserted", "rounds", "replays", "solo_runs", "range_extensions", "admission_waits",
+        "vertex_capacity", "simplex_capacity", "simplex_in_use", "max_sets_resident")] + [
+        ("ms_replays", ctypes.c_double), ("ms_boundary", ctypes.c_double)]
+
+    def as_dict(self) -> dict:
+        return {name: getattr(self, name) for name, _ in self._fields_}
 
 
 class VoStats(ctypes.Structure):
@@ -82,6 +107,26 @@ def lib() -> ctypes.CDLL:
         L.vo_tri_stats.argtypes = [P, ctypes.POINTER(VoStats)]
         L.vo_tri_vertices.argtypes = [P, dp]
         L.vo_tri_config.argtypes = [P, ctypes.c_char_p, ctypes.c_int64]
+        res_p = ctypes.POINTER(VoSetResult)
+        L.vo_engine_create.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int64, ctypes.c_int64, ctypes.POINTER(P)]
+        L.vo_engine_destroy.argtypes = [P]
+        L.vo_engine_destroy.restype = None
+        L.vo_engine_submit.argtypes = [P, dp, ctypes.c_int64, ctypes.c_uint64, ctypes.c_int64]
+        L.vo_engine_step.argtypes = [P, i64p]
+        L.vo_engine_launch.argtypes = [P]
+        L.vo_engine_finish.argtypes = [P, i64p]
+        L.vo_engine_results.argtypes = [P, res_p, ctypes.c_int64, i64p]
+        L.vo_engine_stats_get.argtypes = [P, ctypes.POINTER(VoEngineStats)]
+        L.vo_engine_config.argtypes = [P, ctypes.c_char_p, ctypes.c_int64]
+        L.vo_batch_triangulate.argtypes = [P, dp, i64p, ctypes.c_int64, ctypes.c_uint64, res_p]
+        L.vo_result_canonical.argtypes = [res_p, ctypes.POINTER(i32p), i64p, ctypes.POINTER(i32p), i64p]
+        L.vo_clock.restype = ctypes.c_double
+        L.vo_server_create.argtypes = [ctypes.c_int, ctypes.c_int64, ctypes.c_int64, ctypes.POINTER(P)]
+        L.vo_server_submit.argtypes = [P, ctypes.c_int, dp, ctypes.c_int64, ctypes.c_uint64, ctypes.c_int64]
+        L.vo_server_results.argtypes = [P, res_p, ctypes.c_int64, i64p, ctypes.c_int]
+        L.vo_server_stats_get.argtypes = [P, ctypes.c_int, ctypes.POINTER(VoEngineStats)]
+        L.vo_server_destroy.argtypes = [P]
+        L.vo_server_destroy.restype = None
         L.vo_last_error.restype = ctypes.c_char_p
         L.vo_version.restype = ctypes.c_int
         _lib = L

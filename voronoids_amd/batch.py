@@ -1,0 +1,212 @@
+"""Batched and served modes: several independent point sets on one device.
+
+``Engine`` (vo_engine_*), ``batch_triangulate`` (vo_batch_triangulate) and ``Server``
+(vo_server_*) of include/voronoids.h.  Every set gives the same simplices as
+``DelaunayTree(p)`` + ``add_points_to_tree(p, seed)`` on its own points (the reference's
+DelaunayTree::new + add_points_to_tree, src/delaunay_tree.rs:389-510 / :544-640 and :336-386);
+results are ``SetResult`` records whose ``simplices`` use the reference id space (input point
+i = 2(d+1) + i).  There is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _lib
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_i32p = ctypes.POINTER(ctypes.c_int32)
+_i64p = ctypes.POINTER(ctypes.c_int64)
+
+
+@dataclass
+class SetResult:
+    tag: int
+    dim: int
+    status: int
+    n_points: int
+    points_duplicate: int
+    simplices: np.ndarray          # (n_simplices, dim + 1) int32, reference id space
+    t_submit: float
+    t_admit: float
+    t_done: float
+    rounds: int
+
+    @property
+    def latency(self) -> float:
+        return self.t_done - self.t_submit
+
+    def canonical(self) -> tuple[np.ndarray, np.ndarray]:
+        """canonical T (all-real simplices as sorted input indices, lexicographic) and E
+        (real-real pairs, sorted, unique): the same form as DelaunayTree.canonical()"""
+        return canonical_from_simplices(self.simplices, self.dim)
+
+
+def canonical_from_simplices(simplices: np.ndarray, dim: int) -> tuple[np.ndarray, np.ndarray]:
+    M = dim + 1
+    s = np.ascontiguousarray(simplices, dtype=np.int32).reshape(-1, M)
+    r = _lib.VoSetResult()
+    r.dim = dim
+    r.n_simplices = s.shape[0]
+    r.simplices = s.ctypes.data_as(_i32p)
+    T, E = _i32p(), _i32p()
+    nT, nE = ctypes.c_int64(), ctypes.c_int64()
+    L = _lib.lib()
+    _lib.check(L.vo_result_canonical(ctypes.byref(r), ctypes.byref(T), ctypes.byref(nT), ctypes.byref(E),
+                                     ctypes.byref(nE)))
+    try:
+        Ta = np.ctypeslib.as_array(T, shape=(max(nT.value, 1) * M,))[: nT.value * M].copy()
+        Ea = np.ctypeslib.as_array(E, shape=(max(nE.value, 1) * 2,))[: nE.value * 2].copy()
+    finally:
+        L.vo_free(ctypes.cast(T, ctypes.c_void_p))
+        L.vo_free(ctypes.cast(E, ctypes.c_void_p))
+    return Ta.reshape(-1, M), Ea.reshape(-1, 2)
+
+
+def _take(r: _lib.VoSetResult) -> SetResult:
+    """copy a C result into numpy and release its buffer"""
+    M = r.dim + 1
+    n = r.n_simplices
+    if n > 0 and r.simplices:
+        s = np.ctypeslib.as_array(r.simplices, shape=(n * M,)).copy().reshape(n, M)
+    else:
+        s = np.zeros((0, M), np.int32)
+    if r.simplices:
+        _lib.lib().vo_free(ctypes.cast(r.simplices, ctypes.c_void_p))
+        r.simplices = _i32p()
+    return SetResult(r.tag, r.dim, r.status, r.n_points, r.points_duplicate, s, r.t_submit, r.t_admit, r.t_done,
+                     r.rounds)
+
+
+def clock() -> float:
+    """the engines' clock (seconds, steady)"""
+    return _lib.lib().vo_clock()
+
+
+class Engine:
+    """vo_engine: sets of one dimension sharing one device arena"""
+
+    def __init__(self, dim: int, max_points: int = 1 << 24, max_simplices: int = 0, device: int = 0, **config):
+        if dim not in (2, 3):
+            raise ValueError("dim must be 2 or 3")
+        self.dim = dim
+        h = ctypes.c_void_p()
+        _lib.check(_lib.lib().vo_engine_create(device, dim, int(max_points), int(max_simplices), ctypes.byref(h)))
+        self._h = h
+        for k, v in config.items():
+            self.config(k, v)
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            _lib.lib().vo_engine_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def config(self, key: str, value: int) -> None:
+        _lib.check(_lib.lib().vo_engine_config(self._h, key.encode(), int(value)))
+
+    def submit(self, points, seed: int = 0, tag: int = 0) -> None:
+        p = np.ascontiguousarray(points, dtype=np.float64).reshape(-1, self.dim)
+        _lib.check(_lib.lib().vo_engine_submit(self._h, p.ctypes.data_as(_dp), p.shape[0], seed, tag))
+
+    def step(self) -> int:
+        left = ctypes.c_int64()
+        _lib.check(_lib.lib().vo_engine_step(self._h, ctypes.byref(left)))
+        return left.value
+
+    def results(self) -> list[SetResult]:
+        out = []
+        buf = (_lib.VoSetResult * 256)()
+        n = ctypes.c_int64()
+        while True:
+            _lib.check(_lib.lib().vo_engine_results(self._h, buf, 256, ctypes.byref(n)))
+            out.extend(_take(buf[i]) for i in range(n.value))
+            if n.value < 256:
+                return out
+
+    def run(self) -> list[SetResult]:
+        """step until every submitted set is done; results in completion order"""
+        out = []
+        while self.step() > 0:
+            out.extend(self.results())
+        out.extend(self.results())
+        return out
+
+    def stats(self) -> dict:
+        s = _lib.VoEngineStats()
+        _lib.check(_lib.lib().vo_engine_stats_get(self._h, ctypes.byref(s)))
+        return s.as_dict()
+
+    def batch(self, sets, seed: int = 0) -> list[SetResult]:
+        """vo_batch_triangulate: every set to completion, results in set order (seed of set k
+        = seed + k)"""
+        arrs = [np.ascontiguousarray(s, dtype=np.float64).reshape(-1, self.dim) for s in sets]
+        offs = np.zeros(len(arrs) + 1, np.int64)
+        offs[1:] = np.cumsum([a.shape[0] for a in arrs])
+        pts = np.concatenate(arrs) if arrs else np.zeros((0, self.dim))
+        pts = np.ascontiguousarray(pts)
+        res = (_lib.VoSetResult * max(len(arrs), 1))()
+        rc = _lib.lib().vo_batch_triangulate(self._h, pts.ctypes.data_as(_dp), offs.ctypes.data_as(_i64p), len(arrs),
+                                             seed, res)
+        out = [_take(res[i]) for i in range(len(arrs))]
+        _lib.check(rc)
+        return out
+
+
+def batch_triangulate(sets, dim: int | None = None, seed: int = 0, device: int = 0, **config) -> list[SetResult]:
+    """many independent sets of one dimension in one engine (sized to the batch)"""
+    sets = [np.asarray(s, dtype=np.float64) for s in sets]
+    if dim is None:
+        dim = sets[0].shape[1]
+    biggest = max((s.shape[0] for s in sets), default=0)
+    total = sum(s.shape[0] for s in sets)
+    max_points = int(min(max(total + (dim + 1) * len(sets), 1 << 16), max(1 << 24, 4 * biggest)))
+    eng = Engine(dim, max_points=max_points, device=device, **config)
+    try:
+        return eng.batch(sets, seed=seed)
+    finally:
+        eng.close()
+
+
+class Server:
+    """vo_server: continuous batching of 2D and 3D requests (a driver thread in the library)"""
+
+    def __init__(self, max_points_2d: int = 1 << 24, max_points_3d: int = 1 << 24, device: int = 0):
+        h = ctypes.c_void_p()
+        _lib.check(_lib.lib().vo_server_create(device, int(max_points_2d), int(max_points_3d), ctypes.byref(h)))
+        self._h = h
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            _lib.lib().vo_server_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def submit(self, points, seed: int = 0, tag: int = 0) -> None:
+        p = np.ascontiguousarray(points, dtype=np.float64)
+        if p.ndim != 2 or p.shape[1] not in (2, 3):
+            raise ValueError("points must be (n, 2) or (n, 3)")
+        _lib.check(_lib.lib().vo_server_submit(self._h, p.shape[1], p.ctypes.data_as(_dp), p.shape[0], seed, tag))
+
+    def results(self, timeout_ms: int = 0) -> list[SetResult]:
+        buf = (_lib.VoSetResult * 256)()
+        n = ctypes.c_int64()
+        _lib.check(_lib.lib().vo_server_results(self._h, buf, 256, ctypes.byref(n), int(timeout_ms)))
+        return [_take(buf[i]) for i in range(n.value)]
+
+    def stats(self, dim: int) -> dict:
+        s = _lib.VoEngineStats()
+        _lib.check(_lib.lib().vo_server_stats_get(self._h, dim, ctypes.byref(s)))
+        return s.as_dict()

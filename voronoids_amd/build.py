@@ -10,7 +10,7 @@ import sys
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB = os.path.join(HERE, "libvoronoids_hip.so")
-SRC = [os.path.join(HERE, "csrc", "runtime.hip")]
+SRC = [os.path.join(HERE, "csrc", "runtime.hip"), os.path.join(HERE, "csrc", "engine.hip")]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 FLAGS = [

@@ -1,0 +1,152 @@
+// host_common.hpp -- host-side helpers shared by the one-set runtime and the multi-set engine:
+// thread-local error text, the reference's bounding sphere and super simplex, the priority
+// permutation and device allocation.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "../../include/voronoids.h"
+
+namespace vo_host {
+
+inline thread_local std::string g_err;
+
+inline int fail(int code, const std::string& msg) {
+    g_err = msg;
+    return code;
+}
+
+#define VO_HIP(x)                                                                              \
+    do {                                                                                       \
+        hipError_t e_ = (x);                                                                   \
+        if (e_ != hipSuccess) return ::vo_host::fail(VO_ERR_HIP, std::string(#x) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+// bounding_sphere, src/geometry.rs:99-141: per-axis min/max folds, center (hi+lo)/2, radius
+// the larger corner distance, then one pass in input order multiplying the radius by 1.5 for
+// every point the current sphere does not strictly contain (in_sphere, :91-97).
+inline void bounding_sphere(int D, const double* pts, int64_t n, double* c, double* r) {
+    double lo[3], hi[3];
+    for (int i = 0; i < D; ++i) {
+        lo[i] = INFINITY;
+        hi[i] = -INFINITY;
+        for (int64_t k = 0; k < n; ++k) {
+            lo[i] = std::fmin(lo[i], pts[k * D + i]);
+            hi[i] = std::fmax(hi[i], pts[k * D + i]);
+        }
+        c[i] = (hi[i] + lo[i]) / 2.0;
+    }
+    double su = 0.0, sl = 0.0;
+    for (int i = 0; i < D; ++i) su += (hi[i] - c[i]) * (hi[i] - c[i]);
+    for (int i = 0; i < D; ++i) sl += (lo[i] - c[i]) * (lo[i] - c[i]);
+    const double ud = std::sqrt(su), ld = std::sqrt(sl);
+    double rad = ud > ld ? ud : ld;
+    for (int64_t k = 0; k < n; ++k) {
+        double dist = 0.0;
+        for (int i = 0; i < D; ++i) dist += (c[i] - pts[k * D + i]) * (c[i] - pts[k * D + i]);
+        if (!(dist < rad * rad)) rad = rad * 1.5;
+    }
+    *r = rad;
+}
+
+// center and radius the super simplex is built from: the reference's bounding sphere, except
+// that a zero radius (one point, or all points equal), for which the reference has no
+// super simplex (all its vertices coincide), is replaced by 1
+inline void set_frame(int D, const double* pts, int64_t n, double* c, double* r) {
+    c[0] = c[1] = c[2] = 0.0;
+    *r = 1.0;
+    if (n > 0) bounding_sphere(D, pts, n, c, r);
+    if (!(*r > 0.0)) *r = 1.0;
+}
+
+// super vertices of DelaunayTree::new: 3D src/delaunay_tree.rs:393-406, 2D :548-558
+inline void super_vertices(int D, const double* c, double R, double out[4][3]) {
+    const double pi = 3.14159265358979323846;
+    const double c1 = std::cos(2. * pi / 3.), s1 = std::sin(2. * pi / 3.);
+    const double c2 = std::cos(4. * pi / 3.), s2 = std::sin(4. * pi / 3.);
+    if (D == 3) {
+        const double v[4][3] = {{c[0], c[1], c[2] + R},
+                                {c[0] + R, c[1], c[2] - R},
+                                {c[0] + R * c1, c[1] + R * s1, c[2] - R},
+                                {c[0] + R * c2, c[1] + R * s2, c[2] - R}};
+        std::memcpy(out, v, sizeof(v));
+    } else {
+        const double v[3][3] = {{c[0] + R, c[1], 0}, {c[0] + R * c1, c[1] + R * s1, 0}, {c[0] + R * c2, c[1] + R * s2, 0}};
+        std::memcpy(out, v, sizeof(v));
+    }
+}
+
+// vertex order of the first simplex giving it positive orientation (well conditioned: plain
+// f64 decides it); tet0[3] = -1 in 2D
+inline void first_simplex_order(int D, const double sv[4][3], int tet0[4]) {
+    tet0[0] = 0; tet0[1] = 1; tet0[2] = 2; tet0[3] = 3;
+    if (D == 3) {
+        double u[3], v[3], w[3];
+        for (int k = 0; k < 3; ++k) {
+            u[k] = sv[1][k] - sv[0][k];
+            v[k] = sv[2][k] - sv[0][k];
+            w[k] = sv[3][k] - sv[0][k];
+        }
+        const double det = u[0] * (v[1] * w[2] - v[2] * w[1]) - u[1] * (v[0] * w[2] - v[2] * w[0]) +
+                           u[2] * (v[0] * w[1] - v[1] * w[0]);
+        if (det < 0) { tet0[1] = 2; tet0[2] = 1; }
+    } else {
+        const double det = (sv[1][0] - sv[0][0]) * (sv[2][1] - sv[0][1]) - (sv[1][1] - sv[0][1]) * (sv[2][0] - sv[0][0]);
+        if (det < 0) { tet0[1] = 2; tet0[2] = 1; }
+        tet0[3] = -1;
+    }
+}
+
+inline uint64_t splitmix_next(uint64_t& s) {
+    uint64_t z = (s += 0x9E3779B97F4A7C15ull);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+// insertion priority order: Fisher-Yates over splitmix64(seed); perm[i] = input index of the
+// point with priority i
+inline std::vector<int32_t> priority_order(int64_t n, uint64_t seed) {
+    std::vector<int32_t> perm(n);
+    for (int64_t i = 0; i < n; ++i) perm[i] = (int32_t)i;
+    uint64_t sm = seed;
+    for (int64_t i = n - 1; i > 0; --i) {
+        const uint64_t j = (uint64_t)(((unsigned __int128)splitmix_next(sm) * (uint64_t)(i + 1)) >> 64);
+        std::swap(perm[i], perm[j]);
+    }
+    return perm;
+}
+
+// finest hint-grid level for n points: about 4 points a cell
+inline int hint_levels(int D, int64_t n) {
+    const double target = n / 4.0 > 1.0 ? n / 4.0 : 1.0;
+    int L = (int)std::floor(std::log2(target) / D + 0.5);
+    const int lmax = D == 3 ? 8 : 12;
+    return L < 0 ? 0 : (L > lmax ? lmax : L);
+}
+inline int64_t hint_cells(int D, int L) {
+    return D == 3 ? (((int64_t)1 << (3 * (L + 1))) - 1) / 7 : (((int64_t)1 << (2 * (L + 1))) - 1) / 3;
+}
+
+template <class T>
+int dev_alloc(T** p, size_t count) {
+    *p = nullptr;
+    if (count == 0) count = 1;
+    hipError_t e = hipMalloc((void**)p, sizeof(T) * count);
+    if (e != hipSuccess) return fail(VO_ERR_OOM, std::string("hipMalloc: ") + hipGetErrorString(e));
+    return 0;
+}
+
+// canonical output from live simplices in the reference id space (input point i = 2M + i):
+// T = all-real simplices as sorted input indices, lexicographic; E = real-real pairs of any
+// simplex, sorted, unique.  Arrays are malloc'd.
+int canonical_from_simplices(int M, const int32_t* s, int64_t nl, int32_t** T, int64_t* n_T, int32_t** E,
+                             int64_t* n_E);
+
+}  // namespace vo_host

@@ -28,7 +28,7 @@ namespace vo {
 
 enum : int { ST_OK = 0, ST_DEFER = 1, ST_DUP = 2, ST_OVERFLOW = 3, ST_BROKEN = 4, ST_IDLE = 5 };
 enum : uint32_t { ERR_OVERFLOW = 1, ERR_WALK = 2, ERR_CHAIN = 4, ERR_TETCAP = 8, ERR_HULL = 16, ERR_CACHE = 32,
-                  ERR_PRECISION = 64 };
+                  ERR_PRECISION = 64, ERR_PENDING = 128 };
 
 constexpr int kWalkLimit = 1 << 16;
 constexpr int kChainLimit = 1 << 20;
@@ -40,14 +40,45 @@ struct Counters {                    // device-side, 64-bit, reduced per wave
 };
 
 struct RoundState {
-    int n_tets;            // simplices allocated (append-only high-water mark)
+    int n_tets;            // simplices allocated (append-only high-water mark; one set only)
     int n_next_pending;    // losers and deferred points appended for the next window
     int n_winners;
     int n_q;               // exact-queue fill (may exceed capacity: excess dropped)
     unsigned int err;      // sticky ERR_* bits
     int n_live_delta;      // sum over winners of (created - killed) this round
     int n_big;             // points whose cavity outgrew the group kernel's LDS lists
-    int pad;
+    int n_nospace;         // several sets: winners whose set had no simplex space left
+};
+
+// Several independent point sets sharing one device arena (batched and served modes).
+// Set s owns a contiguous vertex block (its d+1 super vertices, then its points in priority
+// order), one or more simplex ranges handed out by the host, and its own hint grid.
+enum : int { SET_ACTIVE = 1, SET_NEED_SPACE = 2, SET_DONE = 4 };
+struct SetRec {
+    int cursor, end;       // simplex allocation in the current range: next free, one past last
+    int inserted, dups;    // points inserted / dropped as duplicates
+    int head, last;        // fresh (never windowed) points: head .. last-1 (point indices)
+    int pend;              // points of the set sent to the pending list in the current round
+    int flags;             // SET_*
+    int hint_off, hlevels; // hint grid of the set: offset in the hint arena, finest level
+    float glo[3], ginv;    // grid origin and 1 / extent in the set's translated frame
+    int live;              // live simplices of the set
+    int n;                 // points of the set
+    int fresh, woff;       // planner: fresh points windowed this round and their offset
+};
+
+// round control planned on the device when several sets share the arrays: the kernels read
+// the window size and the reservation key of the round from here
+struct BatchCtl {
+    int nwin, npend;       // window size; pending points copied into the window
+    int tail;              // pending points beyond the window's capacity, carried to the next list
+    float gamma;
+    unsigned int round;
+    unsigned long long key_hi;
+    int n_done;            // sets completed since the host last collected them
+    int n_active;
+    int min_window;        // least fresh window of a set per round
+    int pad[2];
 };
 
 struct Ctx {
@@ -82,7 +113,46 @@ struct Ctx {
     int M;                 // vertex id of point p is M + p
     int win_cap, kcap, fcap;
     int tet_cap;
+    // several sets (null / 0 with one set)
+    SetRec* sets;
+    const int* pset;       // set of point p
+    const BatchCtl* ctl;   // window size and key of the round
+    int big_cap;           // capacity of the big list
+    int pend_cap;          // capacity of the pending list
 };
+
+// window size and reservation key of the round: kernel arguments with one set, planned on
+// the device with several
+__device__ __forceinline__ void round_params(const Ctx& c, int& nwin, unsigned long long& key_hi) {
+    if (c.ctl) {
+        nwin = c.ctl->nwin;
+        key_hi = c.ctl->key_hi;
+    }
+}
+
+// hint grid of a point's set
+struct HintView {
+    int* h;
+    float glo[3];
+    float ginv;
+    int levels;
+};
+__device__ __forceinline__ HintView hint_view(const Ctx& c, int p) {
+    HintView v;
+    if (c.sets) {
+        const SetRec& r = c.sets[c.pset[p]];
+        v.h = c.hint + r.hint_off;
+        v.glo[0] = r.glo[0]; v.glo[1] = r.glo[1]; v.glo[2] = r.glo[2];
+        v.ginv = r.ginv;
+        v.levels = r.hlevels;
+    } else {
+        v.h = c.hint;
+        v.glo[0] = c.glo[0]; v.glo[1] = c.glo[1]; v.glo[2] = c.glo[2];
+        v.ginv = c.ginv;
+        v.levels = c.hlevels;
+    }
+    return v;
+}
 
 template <int D> struct VT;
 template <> struct VT<3> {
@@ -301,29 +371,30 @@ template <int D> __device__ __forceinline__ int first_src(int i, int k) {
 }
 
 // cell of pf at level l of the hint grid (levels stored coarse to fine, one after another)
-template <int D> __device__ __forceinline__ int hint_cell(const Ctx& c, typename VT<D>::F pf, int l) {
+template <int D> __device__ __forceinline__ int hint_cell(const HintView& v, typename VT<D>::F pf, int l) {
     const int g = 1 << l;
     const float fg = (float)g;
     const int off = D == 3 ? (((1 << (3 * l)) - 1) / 7) : (((1 << (2 * l)) - 1) / 3);
-    int ix = (int)fminf(fmaxf((pf.x - c.glo[0]) * c.ginv * fg, 0.0f), fg - 1.0f);
-    int iy = (int)fminf(fmaxf((pf.y - c.glo[1]) * c.ginv * fg, 0.0f), fg - 1.0f);
+    int ix = (int)fminf(fmaxf((pf.x - v.glo[0]) * v.ginv * fg, 0.0f), fg - 1.0f);
+    int iy = (int)fminf(fmaxf((pf.y - v.glo[1]) * v.ginv * fg, 0.0f), fg - 1.0f);
     int iz = 0;
-    if constexpr (D == 3) iz = (int)fminf(fmaxf((pf.z - c.glo[2]) * c.ginv * fg, 0.0f), fg - 1.0f);
+    if constexpr (D == 3) iz = (int)fminf(fmaxf((pf.z - v.glo[2]) * v.ginv * fg, 0.0f), fg - 1.0f);
     return off + (iz * g + iy) * g + ix;
 }
 
-// walk start for a fresh point: the finest level whose cell has been visited
-template <int D> __device__ __forceinline__ int hint_lookup(const Ctx& c, typename VT<D>::F pf) {
-    int best = 0;
-    for (int l = 0; l <= c.hlevels; ++l) {
-        const int h = c.hint[hint_cell<D>(c, pf, l)];
+// walk start for a fresh point: the finest level whose cell has been visited (level 0 holds
+// the set's first simplex)
+template <int D> __device__ __forceinline__ int hint_lookup(const HintView& v, typename VT<D>::F pf) {
+    int best = v.h[0];
+    for (int l = 1; l <= v.levels; ++l) {
+        const int h = v.h[hint_cell<D>(v, pf, l)];
         if (h >= 0) best = h;
     }
     return best;
 }
 
-template <int D> __device__ __forceinline__ void hint_store(const Ctx& c, typename VT<D>::F pf, int t) {
-    for (int l = 1; l <= c.hlevels; ++l) c.hint[hint_cell<D>(c, pf, l)] = t;
+template <int D> __device__ __forceinline__ void hint_store(const HintView& v, typename VT<D>::F pf, int t) {
+    for (int l = 1; l <= v.levels; ++l) v.h[hint_cell<D>(v, pf, l)] = t;
 }
 
 __device__ __forceinline__ uint32_t mix32(uint32_t x) {
@@ -341,17 +412,20 @@ __global__ void __launch_bounds__(256) k_cavity(Ctx c, int nwin, unsigned long l
     constexpr int M = D + 1;
     constexpr int KL = 32;                       // cavity entries mirrored in LDS
     __shared__ int s_cav[KL][256];
-    const int slot = blockIdx.x * blockDim.x + threadIdx.x;
     const int lt = threadIdx.x;
     LocalCnt lc = {};
+    round_params(c, nwin, key_hi);
+    for (int wbase = blockIdx.x * blockDim.x; wbase < nwin; wbase += gridDim.x * blockDim.x) {
+    const int slot = wbase + threadIdx.x;
     if (slot < nwin) {
         const int p = c.win[slot];
         const int pv = c.M + p;
         const typename V::F pf = V::ld(c.X32, pv);
+        const HintView hv = hint_view(c, p);
         const int W = c.win_cap;
         int st = ST_OK;
         int t = c.cp[p];
-        if (t < 0) t = hint_lookup<D>(c, pf);
+        if (t < 0) t = hint_lookup<D>(hv, pf);
         int4 tt = c.tv[t];
         for (int hops = 0; tt.x < 0; ++hops) {
             if (hops > kChainLimit) { atomicOr(&c.st->err, ERR_CHAIN); st = ST_BROKEN; break; }
@@ -390,7 +464,7 @@ __global__ void __launch_bounds__(256) k_cavity(Ctx c, int nwin, unsigned long l
             if (!moved) break;
         }
         c.cp[p] = t;
-        hint_store<D>(c, pf, t);
+        hint_store<D>(hv, pf, t);
         int nk = 0, nf = 0;
         if (st == ST_OK) {
             const int s0 = conflict_at<D>(c, a, t, pv, pf, lc);
@@ -447,11 +521,13 @@ __global__ void __launch_bounds__(256) k_cavity(Ctx c, int nwin, unsigned long l
                 if (nb >= 0) atomicMin(&c.nres[nb >> 2], key);
             }
         }
-        if (st == ST_OVERFLOW) atomicOr(&c.st->err, ERR_OVERFLOW);
+        // a window point whose cavity outgrows the lists is re-run alone; alone, it is an error
+        if (st == ST_OVERFLOW && c.solo) atomicOr(&c.st->err, ERR_OVERFLOW);
         if (st == ST_DUP) lc.dups++;
         c.nk[slot] = nk;
         c.nf[slot] = nf;
         c.status[slot] = st;
+    }
     }
     flush_counters(c.cnt, lc);
 }
@@ -468,9 +544,11 @@ __global__ void __launch_bounds__(256) k_cavity(Ctx c, int nwin, unsigned long l
 template <int D>
 __global__ void __launch_bounds__(256) k_commit(Ctx c, int nwin, unsigned long long key_hi) {
     constexpr int M = D + 1;
-    const int slot = blockIdx.x * blockDim.x + threadIdx.x;
     const int W = c.win_cap;
     LocalCnt lc = {};
+    round_params(c, nwin, key_hi);
+    for (int wbase = blockIdx.x * blockDim.x; wbase < nwin; wbase += gridDim.x * blockDim.x) {
+    const int slot = wbase + threadIdx.x;
     int st = ST_IDLE, p = 0, nk = 0, nf = 0;
     bool win = false;
     unsigned long long key = 0;
@@ -493,15 +571,29 @@ __global__ void __launch_bounds__(256) k_commit(Ctx c, int nwin, unsigned long l
             }
         }
     }
-    int total = 0;
-    const int off = wave_excl_scan(win ? nf : 0, total);
-    int base0 = 0;
-    if (__lane_id() == 0 && total > 0) base0 = atomicAdd(&c.st->n_tets, total);
-    base0 = __shfl(base0, 0, 64);
-    const int base = base0 + off;
-    if (win && base + nf > c.tet_cap) {
-        atomicOr(&c.st->err, ERR_TETCAP);
-        win = false;
+    int base = 0;
+    if (c.sets) {
+        // several sets: slots from the set's own range; a full range sends the point back
+        if (win) {
+            SetRec& r = c.sets[c.pset[p]];
+            base = atomicAdd(&r.cursor, nf);
+            if (base + nf > r.end) {
+                atomicOr(&r.flags, SET_NEED_SPACE);
+                atomicAdd(&c.st->n_nospace, 1);
+                win = false;
+            }
+        }
+    } else {
+        int total = 0;
+        const int off = wave_excl_scan(win ? nf : 0, total);
+        int base0 = 0;
+        if (__lane_id() == 0 && total > 0) base0 = atomicAdd(&c.st->n_tets, total);
+        base0 = __shfl(base0, 0, 64);
+        base = base0 + off;
+        if (win && base + nf > c.tet_cap) {
+            atomicOr(&c.st->err, ERR_TETCAP);
+            win = false;
+        }
     }
     if (win) {
         const int pv = c.M + p;
@@ -570,22 +662,46 @@ __global__ void __launch_bounds__(256) k_commit(Ctx c, int nwin, unsigned long l
         // phase 3: killed simplices die with a forward pointer into the star
         for (int j = 0; j < nk; ++j) c.tv[c.cav[j * W + slot]] = make_int4(-1, base, 0, 0);
         for (int j = 0; j < nf; ++j) c.tv[c.fac[j * W + slot] >> 2].y = base + j;
-        lc.killed = nk;
-        lc.created = nf;
-        lc.wins = 1;
-        lc.maxk = nk;
-        lc.maxf = nf;
+        lc.killed += nk;
+        lc.created += nf;
+        lc.wins += 1;
+        lc.maxk = nk > (int)lc.maxk ? nk : lc.maxk;
+        lc.maxf = nf > (int)lc.maxf ? nf : lc.maxf;
+    }
+    // cavities too large for the per-point lists are re-run alone after the round
+    const bool big = st == ST_OVERFLOW && !c.solo;
+    bool again = (st == ST_OK && !win) || st == ST_DEFER;
+    {
+        const unsigned long long bm = __ballot(big);
+        int bpos = 0;
+        if (__lane_id() == 0 && bm) bpos = atomicAdd(&c.st->n_big, __popcll(bm));
+        bpos = __shfl(bpos, 0, 64) + __popcll(bm & ((1ull << __lane_id()) - 1ull));
+        if (big) {
+            if (bpos < c.big_cap) c.big[bpos] = p;
+            else again = true;             // list full: retry in the next window
+        }
+    }
+    if (c.sets && st != ST_IDLE) {
+        SetRec& r = c.sets[c.pset[p]];
+        if (win) {
+            atomicAdd(&r.inserted, 1);
+            atomicAdd(&r.live, nf - nk);
+        } else if (again) {
+            atomicAdd(&r.pend, 1);
+        } else if (st == ST_DUP) {
+            atomicAdd(&r.dups, 1);
+        }
     }
     // next window: losers (valid cavity, lost a reservation) and deferred points
-    const bool again = (st == ST_OK && !win) || st == ST_DEFER;
     const unsigned long long mask = __ballot(again);
     const int cnt = __popcll(mask);
     int pos0 = 0;
     if (__lane_id() == 0 && cnt) pos0 = atomicAdd(&c.st->n_next_pending, cnt);
     pos0 = __shfl(pos0, 0, 64);
     if (again) {
-        const int before = __popcll(mask & ((1ull << __lane_id()) - 1ull));
-        c.pend_next[pos0 + before] = p;
+        const int pos = pos0 + __popcll(mask & ((1ull << __lane_id()) - 1ull));
+        if (pos < c.pend_cap) c.pend_next[pos] = p;
+        else atomicOr(&c.st->err, ERR_PENDING);
     }
     const unsigned long long wm = __ballot(win);
     if (__lane_id() == 0) {
@@ -594,22 +710,17 @@ __global__ void __launch_bounds__(256) k_commit(Ctx c, int nwin, unsigned long l
     }
     const long long delta = wave_sum_i64(win ? (long long)(nf - nk) : 0ll);
     if (__lane_id() == 0 && delta) atomicAdd(&c.st->n_live_delta, (int)delta);
+    }
     flush_counters(c.cnt, lc);
-}
-
-// next window = pending losers followed by fresh points head .. head + nfresh - 1
-__global__ void k_window(int* win, const int* pend, int npend, int head, int nfresh) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < npend) win[i] = pend[i];
-    else if (i < npend + nfresh) win[i] = head + (i - npend);
 }
 
 // ---------------------------------------------------------------- k_exact (tier 3)
 template <int S>
 __global__ void __launch_bounds__(64) k_exact(const QEntry* q, int nq, const double* X64, double* ws, int ws_cap,
-                                              CEntry* cache, uint32_t cmask, RoundState* st, Counters* cnt) {
+                                              CEntry* cache, uint32_t cmask, RoundState* st, Counters* cnt, int qcap) {
     const int tid = blockIdx.x * blockDim.x + threadIdx.x;
     const int nth = gridDim.x * blockDim.x;
+    if (nq < 0) nq = min(st->n_q, qcap);     // queue fill of the round, read on the device
     for (int i = tid; i < nq; i += nth) {
         Ws w{ws + (size_t)tid * ws_cap, ws_cap, 0, 0};
         const QEntry e = q[i];
@@ -629,32 +740,6 @@ __global__ void __launch_bounds__(64) k_exact(const QEntry* q, int nq, const dou
 }
 
 // ---------------------------------------------------------------- export / check
-__global__ void k_live_flags(const int4* tv, int n, int* flag) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) flag[i] = tv[i].x >= 0;
-}
-
-// exclusive scan of flags within each 256-block (block sums out), then add block offsets
-__global__ void k_block_scan(const int* in, int* out, int* bsum, int n) {
-    __shared__ int s[256];
-    const int i = blockIdx.x * 256 + threadIdx.x;
-    const int v = i < n ? in[i] : 0;
-    s[threadIdx.x] = v;
-    __syncthreads();
-    for (int o = 1; o < 256; o <<= 1) {
-        int add = threadIdx.x >= (unsigned)o ? s[threadIdx.x - o] : 0;
-        __syncthreads();
-        s[threadIdx.x] += add;
-        __syncthreads();
-    }
-    if (i < n) out[i] = s[threadIdx.x] - v;
-    if (threadIdx.x == 255) bsum[blockIdx.x] = s[255];
-}
-__global__ void k_add_offsets(int* out, const int* boff, int n) {
-    const int i = blockIdx.x * 256 + threadIdx.x;
-    if (i < n) out[i] += boff[blockIdx.x];
-}
-
 // compacted export: vertex ids mapped to the reference id space, neighbors to export indices
 template <int D>
 __global__ void k_export(const int4* tv, const int4* ta, const int* pos, int n, const int* vmap, int32_t* simp,

@@ -53,10 +53,12 @@ __global__ void __launch_bounds__(64) k_cavity_g(Ctx c, int nwin, unsigned long 
     __shared__ int s_fac[NG][FC];
     __shared__ int s_tmp[64];
     const Group<G> g;
-    const int slot = blockIdx.x * NG + g.gid;
     const int W = c.win_cap;
     const bool lead = g.gl == 0;
     LocalCnt lc = {};
+    round_params(c, nwin, key_hi);
+    for (int wbase = blockIdx.x * NG; wbase < nwin; wbase += gridDim.x * NG) {
+    const int slot = wbase + g.gid;
     int st = slot < nwin ? ST_OK : ST_IDLE;
     int p = 0, pv = 0, t = 0;
     F pf{};
@@ -68,7 +70,7 @@ __global__ void __launch_bounds__(64) k_cavity_g(Ctx c, int nwin, unsigned long 
         key = key_hi | (unsigned)p;
         pf = V::ld(c.X32, pv);
         t = c.cp[p];
-        if (t < 0) t = hint_lookup<D>(c, pf);
+        if (t < 0) t = hint_lookup<D>(hint_view(c, p), pf);
         tt = c.tv[t];
         for (int hops = 0; tt.x < 0; ++hops) {
             if (hops > kChainLimit) { if (lead) atomicOr(&c.st->err, ERR_CHAIN); st = ST_BROKEN; break; }
@@ -123,7 +125,7 @@ __global__ void __launch_bounds__(64) k_cavity_g(Ctx c, int nwin, unsigned long 
     if (st == ST_OK || st == ST_DEFER) {
         if (lead) {
             c.cp[p] = t;
-            hint_store<D>(c, pf, t);
+            hint_store<D>(hint_view(c, p), pf, t);
         }
     }
     // ---- the containing simplex must conflict (else p repeats one of its vertices)
@@ -215,6 +217,8 @@ __global__ void __launch_bounds__(64) k_cavity_g(Ctx c, int nwin, unsigned long 
         c.nf[slot] = nf;
         c.status[slot] = st;
     }
+    __builtin_amdgcn_wave_barrier();       // the LDS lists are reused by the next slots
+    }
     flush_counters(c.cnt, lc);
 }
 
@@ -225,10 +229,12 @@ __global__ void __launch_bounds__(64) k_commit_g(Ctx c, int nwin, unsigned long 
     constexpr int FC = kGroupFC<D>;
     __shared__ int s_fac[NG][FC];
     const Group<G> g;
-    const int slot = blockIdx.x * NG + g.gid;
     const int W = c.win_cap;
     const bool lead = g.gl == 0;
     LocalCnt lc = {};
+    round_params(c, nwin, key_hi);
+    for (int wbase = blockIdx.x * NG; wbase < nwin; wbase += gridDim.x * NG) {
+    const int slot = wbase + g.gid;
     int st = ST_IDLE, p = 0, nk = 0, nf = 0;
     unsigned long long key = 0;
     bool ok = true;
@@ -252,17 +258,35 @@ __global__ void __launch_bounds__(64) k_commit_g(Ctx c, int nwin, unsigned long 
         }
     }
     const bool win = st == ST_OK && g.ballot(!ok) == 0;
-    // one simplex slot range per winning group, one atomicAdd per wavefront
-    int total = 0;
-    const int off = wave_excl_scan(win && lead ? nf : 0, total);
-    int base0 = 0;
-    if (g.lane == 0 && total > 0) base0 = atomicAdd(&c.st->n_tets, total);
-    base0 = __shfl(base0, 0, 64);
-    const int base = base0 + g.bcast(off, 0);
+    int base = 0;
     bool build = win;
-    if (build && base + nf > c.tet_cap) {
-        if (lead) atomicOr(&c.st->err, ERR_TETCAP);
-        build = false;
+    if (c.sets) {
+        // several sets: one range request per winning point on its set's cursor; a full
+        // range sends the point back to the window
+        int b = 0, room = 1;
+        if (win && lead) {
+            SetRec& r = c.sets[c.pset[p]];
+            b = atomicAdd(&r.cursor, nf);
+            room = b + nf <= r.end;
+            if (!room) {
+                atomicOr(&r.flags, SET_NEED_SPACE);
+                atomicAdd(&c.st->n_nospace, 1);
+            }
+        }
+        base = g.bcast(b, 0);
+        build = win && g.bcast(room, 0);
+    } else {
+        // one simplex slot range per winning group, one atomicAdd per wavefront
+        int total = 0;
+        const int off = wave_excl_scan(win && lead ? nf : 0, total);
+        int base0 = 0;
+        if (g.lane == 0 && total > 0) base0 = atomicAdd(&c.st->n_tets, total);
+        base0 = __shfl(base0, 0, 64);
+        base = base0 + g.bcast(off, 0);
+        if (build && base + nf > c.tet_cap) {
+            if (lead) atomicOr(&c.st->err, ERR_TETCAP);
+            build = false;
+        }
     }
     // a winner's check loops ran to completion, so s_fac holds its whole facet list
     __builtin_amdgcn_wave_barrier();
@@ -335,31 +359,53 @@ __global__ void __launch_bounds__(64) k_commit_g(Ctx c, int nwin, unsigned long 
             c.tv[t] = make_int4(-1, fwd, 0, 0);
         }
         if (lead) {
-            lc.killed = nk;
-            lc.created = nf;
-            lc.wins = 1;
-            lc.maxk = nk;
-            lc.maxf = nf;
+            lc.killed += nk;
+            lc.created += nf;
+            lc.wins += 1;
+            lc.maxk = nk > (int)lc.maxk ? nk : lc.maxk;
+            lc.maxf = nf > (int)lc.maxf ? nf : lc.maxf;
         }
     }
     // cavities too large for the LDS lists are re-run alone after the round
     const bool big = lead && st == ST_OVERFLOW;
-    const unsigned long long bm = __ballot(big);
-    int bpos = 0;
-    if (g.lane == 0 && bm) bpos = atomicAdd(&c.st->n_big, __popcll(bm));
-    bpos = __shfl(bpos, 0, 64);
-    if (big) c.big[bpos + __popcll(bm & ((1ull << g.lane) - 1ull))] = p;
-    const bool again = lead && ((st == ST_OK && !build) || st == ST_DEFER);
+    bool again = lead && ((st == ST_OK && !build) || st == ST_DEFER);
+    {
+        const unsigned long long bm = __ballot(big);
+        int bpos = 0;
+        if (g.lane == 0 && bm) bpos = atomicAdd(&c.st->n_big, __popcll(bm));
+        bpos = __shfl(bpos, 0, 64) + __popcll(bm & ((1ull << g.lane) - 1ull));
+        if (big) {
+            if (bpos < c.big_cap) c.big[bpos] = p;
+            else again = true;             // list full: retry in the next window
+        }
+    }
+    if (c.sets && lead && st != ST_IDLE) {
+        SetRec& r = c.sets[c.pset[p]];
+        if (build) {
+            atomicAdd(&r.inserted, 1);
+            atomicAdd(&r.live, nf - nk);
+        } else if (again) {
+            atomicAdd(&r.pend, 1);
+        } else if (st == ST_DUP) {
+            atomicAdd(&r.dups, 1);
+        }
+    }
     const unsigned long long mask = __ballot(again);
     const int cnt = __popcll(mask);
     int pos0 = 0;
     if (g.lane == 0 && cnt) pos0 = atomicAdd(&c.st->n_next_pending, cnt);
     pos0 = __shfl(pos0, 0, 64);
-    if (again) c.pend_next[pos0 + __popcll(mask & ((1ull << g.lane) - 1ull))] = p;
+    if (again) {
+        const int pos = pos0 + __popcll(mask & ((1ull << g.lane) - 1ull));
+        if (pos < c.pend_cap) c.pend_next[pos] = p;
+        else atomicOr(&c.st->err, ERR_PENDING);
+    }
     const unsigned long long wm = __ballot(build && lead);
     if (g.lane == 0 && wm) atomicAdd(&c.st->n_winners, __popcll(wm));
     const long long delta = wave_sum_i64(build && lead ? (long long)(nf - nk) : 0ll);
     if (g.lane == 0 && delta) atomicAdd(&c.st->n_live_delta, (int)delta);
+    __builtin_amdgcn_wave_barrier();       // s_fac is reused by the next slots
+    }
     flush_counters(c.cnt, lc);
 }
 

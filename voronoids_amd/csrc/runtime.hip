@@ -19,6 +19,7 @@
 #include <vector>
 
 #include "../../include/voronoids.h"
+#include "host_common.hpp"
 #include "kernels.hpp"
 #include "kernels_group.hpp"
 
@@ -26,83 +27,47 @@
 
 using namespace vo;
 
-namespace {
+using namespace vo_host;
 
-thread_local std::string g_err;
-
-int fail(int code, const std::string& msg) {
-    g_err = msg;
-    return code;
-}
-
-#define VO_HIP(x)                                                                              \
-    do {                                                                                       \
-        hipError_t e_ = (x);                                                                   \
-        if (e_ != hipSuccess) return fail(VO_ERR_HIP, std::string(#x) + ": " + hipGetErrorString(e_)); \
-    } while (0)
-
-// ------------------------------------------------------------------ reference geometry
-// bounding_sphere, src/geometry.rs:99-141: per-axis min/max folds, center (hi+lo)/2, radius
-// the larger corner distance, then one pass in input order multiplying the radius by 1.5 for
-// every point the current sphere does not strictly contain (in_sphere, :91-97).
-void bounding_sphere(int D, const double* pts, int64_t n, double* c, double* r) {
-    double lo[3], hi[3];
-    for (int i = 0; i < D; ++i) {
-        lo[i] = INFINITY;
-        hi[i] = -INFINITY;
-        for (int64_t k = 0; k < n; ++k) {
-            lo[i] = std::fmin(lo[i], pts[k * D + i]);
-            hi[i] = std::fmax(hi[i], pts[k * D + i]);
+namespace vo_host {
+int canonical_from_simplices(int M, const int32_t* s, int64_t nl, int32_t** T, int64_t* n_T, int32_t** E,
+                             int64_t* n_E) {
+    const int32_t first = 2 * M;
+    std::vector<std::array<int32_t, 4>> tris;
+    std::vector<std::array<int32_t, 2>> edges;
+    tris.reserve(nl);
+    edges.reserve((size_t)nl * (M == 4 ? 6 : 3));
+    for (int64_t i = 0; i < nl; ++i) {
+        std::array<int32_t, 4> r = {INT32_MAX, INT32_MAX, INT32_MAX, INT32_MAX};
+        bool real = true;
+        for (int k = 0; k < M; ++k) {
+            const int32_t v = s[i * M + k];
+            r[k] = v >= first ? v - first : -1;
+            real &= v >= first;
         }
-        c[i] = (hi[i] + lo[i]) / 2.0;
+        for (int a = 0; a < M; ++a)
+            for (int b = a + 1; b < M; ++b)
+                if (r[a] >= 0 && r[b] >= 0) edges.push_back({std::min(r[a], r[b]), std::max(r[a], r[b])});
+        if (real) {
+            std::sort(r.begin(), r.begin() + M);
+            tris.push_back(r);
+        }
     }
-    double su = 0.0, sl = 0.0;
-    for (int i = 0; i < D; ++i) su += (hi[i] - c[i]) * (hi[i] - c[i]);
-    for (int i = 0; i < D; ++i) sl += (lo[i] - c[i]) * (lo[i] - c[i]);
-    const double ud = std::sqrt(su), ld = std::sqrt(sl);
-    double rad = ud > ld ? ud : ld;
-    for (int64_t k = 0; k < n; ++k) {
-        double dist = 0.0;
-        for (int i = 0; i < D; ++i) dist += (c[i] - pts[k * D + i]) * (c[i] - pts[k * D + i]);
-        if (!(dist < rad * rad)) rad = rad * 1.5;
-    }
-    *r = rad;
-}
-
-// super vertices of DelaunayTree::new: 3D src/delaunay_tree.rs:393-406, 2D :548-558
-void super_vertices(int D, const double* c, double R, double out[4][3]) {
-    const double pi = 3.14159265358979323846;
-    const double c1 = std::cos(2. * pi / 3.), s1 = std::sin(2. * pi / 3.);
-    const double c2 = std::cos(4. * pi / 3.), s2 = std::sin(4. * pi / 3.);
-    if (D == 3) {
-        const double v[4][3] = {{c[0], c[1], c[2] + R},
-                                {c[0] + R, c[1], c[2] - R},
-                                {c[0] + R * c1, c[1] + R * s1, c[2] - R},
-                                {c[0] + R * c2, c[1] + R * s2, c[2] - R}};
-        std::memcpy(out, v, sizeof(v));
-    } else {
-        const double v[3][3] = {{c[0] + R, c[1], 0}, {c[0] + R * c1, c[1] + R * s1, 0}, {c[0] + R * c2, c[1] + R * s2, 0}};
-        std::memcpy(out, v, sizeof(v));
-    }
-}
-
-uint64_t splitmix_next(uint64_t& s) {
-    uint64_t z = (s += 0x9E3779B97F4A7C15ull);
-    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
-    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
-    return z ^ (z >> 31);
-}
-
-template <class T>
-int dev_alloc(T** p, size_t count) {
-    *p = nullptr;
-    if (count == 0) count = 1;
-    hipError_t e = hipMalloc((void**)p, sizeof(T) * count);
-    if (e != hipSuccess) return fail(VO_ERR_OOM, std::string("hipMalloc: ") + hipGetErrorString(e));
+    std::sort(tris.begin(), tris.end());
+    std::sort(edges.begin(), edges.end());
+    edges.erase(std::unique(edges.begin(), edges.end()), edges.end());
+    *T = (int32_t*)std::malloc(sizeof(int32_t) * std::max<size_t>(1, tris.size() * M));
+    *E = (int32_t*)std::malloc(sizeof(int32_t) * std::max<size_t>(1, edges.size() * 2));
+    if (!*T || !*E) return fail(VO_ERR_OOM, "host allocation");
+    for (size_t i = 0; i < tris.size(); ++i)
+        for (int k = 0; k < M; ++k) (*T)[i * M + k] = tris[i][k];
+    for (size_t i = 0; i < edges.size(); ++i) { (*E)[2 * i] = edges[i][0]; (*E)[2 * i + 1] = edges[i][1]; }
+    *n_T = (int64_t)tris.size();
+    *n_E = (int64_t)edges.size();
     return 0;
 }
+}  // namespace vo_host
 
-}  // namespace
 
 struct vo_tri {
     int dev = 0, D = 3, M = 4, S = 4;
@@ -205,7 +170,45 @@ Ctx make_ctx(vo_tri* t) {
     c.kcap = t->kcap;
     c.fcap = t->fcap;
     c.tet_cap = (int)t->tcap;
+    c.sets = nullptr;
+    c.pset = nullptr;
+    c.ctl = nullptr;
+    c.big_cap = t->win_cap;
+    c.pend_cap = t->win_cap;
     return c;
+}
+
+// next window = pending losers followed by fresh points head .. head + nfresh - 1
+__global__ void k_window(int* win, const int* pend, int npend, int head, int nfresh) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < npend) win[i] = pend[i];
+    else if (i < npend + nfresh) win[i] = head + (i - npend);
+}
+
+__global__ void k_live_flags(const int4* tv, int n, int* flag) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) flag[i] = tv[i].x >= 0;
+}
+
+// exclusive scan of flags within each 256-block (block sums out), then add block offsets
+__global__ void k_block_scan(const int* in, int* out, int* bsum, int n) {
+    __shared__ int s[256];
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    const int v = i < n ? in[i] : 0;
+    s[threadIdx.x] = v;
+    __syncthreads();
+    for (int o = 1; o < 256; o <<= 1) {
+        int add = threadIdx.x >= (unsigned)o ? s[threadIdx.x - o] : 0;
+        __syncthreads();
+        s[threadIdx.x] += add;
+        __syncthreads();
+    }
+    if (i < n) out[i] = s[threadIdx.x] - v;
+    if (threadIdx.x == 255) bsum[blockIdx.x] = s[255];
+}
+__global__ void k_add_offsets(int* out, const int* boff, int n) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i < n) out[i] += boff[blockIdx.x];
 }
 
 __global__ void k_init_first(int4* tv, int4* ta, int a, int b, int c, int d) {
@@ -226,7 +229,7 @@ __global__ void k_reset_round(RoundState* st) {
 // grow the simplex arrays to cap (copying the first n), new reservations = max key
 int grow_tets(vo_tri* t, int64_t cap) {
     if (cap <= t->tcap) return 0;
-    if (cap > (int64_t)1 << 30) return fail(VO_ERR_OOM, "simplex capacity above 2^30");
+    if (cap > (int64_t)1 << 29) return fail(VO_ERR_OOM, "simplex capacity above 2^29 (adjacency packs index << 2)");
     int4 *tv, *ta;
     float4* tx;
     unsigned long long *kr, *nr;
@@ -331,10 +334,10 @@ int run_exact(vo_tri* t, int nq) {
     VO_HIP(hipEventRecord(t->ev[4], t->s));
     if (t->D == 3)
         hipLaunchKernelGGL(k_exact<4>, dim3(blocks), dim3(threads), 0, t->s, t->q, nq, t->X64, t->ws, t->ws_cap, t->cache,
-                           t->cmask, t->st, t->cnt);
+                           t->cmask, t->st, t->cnt, t->qcap);
     else
         hipLaunchKernelGGL(k_exact<2>, dim3(blocks), dim3(threads), 0, t->s, t->q, nq, t->X64, t->ws, t->ws_cap, t->cache,
-                           t->cmask, t->st, t->cnt);
+                           t->cmask, t->st, t->cnt, t->qcap);
     VO_HIP(hipGetLastError());
     VO_HIP(hipEventRecord(t->ev[5], t->s));
     VO_HIP(hipEventSynchronize(t->ev[5]));
@@ -445,7 +448,7 @@ int launch_round(vo_tri* t, const Ctx& c, int nwin, unsigned long long key_hi, i
 // =================================================================== C ABI
 extern "C" {
 
-const char* vo_last_error(void) { return g_err.c_str(); }
+const char* vo_last_error(void) { return vo_host::g_err.c_str(); }
 int vo_version(void) { return 1; }
 void vo_free(void* p) { std::free(p); }
 
@@ -474,32 +477,17 @@ int vo_tri_create(int hip_device, int dim, const double* pts, int64_t n, vo_tri*
     if (hipStreamCreateWithFlags(&t->s, hipStreamNonBlocking) != hipSuccess) return bail(fail(VO_ERR_HIP, "stream"));
     for (auto& e : t->ev)
         if (hipEventCreate(&e) != hipSuccess) return bail(fail(VO_ERR_HIP, "event"));
-    if (n > 0) {
-        bounding_sphere(dim, pts, n, t->c, &t->r);
-    } else {
-        t->r = 1.0;
-    }
+    set_frame(dim, pts, n, t->c, &t->r);
     t->R = t->r * 10.0;
     super_vertices(dim, t->c, t->R, t->sv);
-    // positive orientation of the super simplex (well conditioned: plain f64 suffices)
-    if (dim == 3) {
-        double u[3], v[3], w[3];
-        for (int k = 0; k < 3; ++k) {
-            u[k] = t->sv[1][k] - t->sv[0][k];
-            v[k] = t->sv[2][k] - t->sv[0][k];
-            w[k] = t->sv[3][k] - t->sv[0][k];
-        }
-        const double det = u[0] * (v[1] * w[2] - v[2] * w[1]) - u[1] * (v[0] * w[2] - v[2] * w[0]) +
-                           u[2] * (v[0] * w[1] - v[1] * w[0]);
-        if (det < 0) { t->tet0[1] = 2; t->tet0[2] = 1; }
-    } else {
-        const double det = (t->sv[1][0] - t->sv[0][0]) * (t->sv[2][1] - t->sv[0][1]) -
-                           (t->sv[1][1] - t->sv[0][1]) * (t->sv[2][0] - t->sv[0][0]);
-        if (det < 0) { t->tet0[1] = 2; t->tet0[2] = 1; }
-        t->tet0[3] = -1;
-    }
+    first_simplex_order(dim, t->sv, t->tet0);
     // windows: capacity and per-point cavity buffers
-    t->win_cap = dim == 3 ? (1 << 20) : (1 << 21);
+    // a window never holds more than the set's points
+    {
+        int wc = 4096;
+        while (wc < n && wc < (dim == 3 ? (1 << 20) : (1 << 21))) wc <<= 1;
+        t->win_cap = wc;
+    }
     t->kcap = dim == 3 ? 256 : 96;
     t->fcap = dim == 3 ? 384 : 128;
     t->qcap = 1 << 20;
@@ -564,14 +552,7 @@ int vo_tri_stage(vo_tri* t, const double* pts, int64_t n, uint64_t seed) {
     VO_HIP(hipSetDevice(t->dev));
     int rc;
     if ((rc = grow_points(t, t->n_pts + n))) return rc;
-    // priority order: Fisher-Yates over splitmix64(seed)
-    std::vector<int32_t> perm(n);
-    for (int64_t i = 0; i < n; ++i) perm[i] = (int32_t)i;
-    uint64_t sm = seed;
-    for (int64_t i = n - 1; i > 0; --i) {
-        const uint64_t j = (uint64_t)(((unsigned __int128)splitmix_next(sm) * (uint64_t)(i + 1)) >> 64);
-        std::swap(perm[i], perm[j]);
-    }
+    const std::vector<int32_t> perm = priority_order(n, seed);
     std::vector<double> x64((size_t)n * S, 0.0);
     std::vector<float> x32((size_t)n * S, 0.0f);
     std::vector<int> vm(n);
@@ -596,11 +577,9 @@ int vo_tri_stage(vo_tri* t, const double* pts, int64_t n, uint64_t seed) {
     // walk-start hints: grids of 2^l cells per axis over the bounding sphere's box, finest
     // level about 4 points a cell; level 0 holds the initial simplex
     if (!t->hint) {
-        const double target = std::max<double>((double)t->n_pts / 4.0, 1.0);
-        int L = (int)std::floor(std::log2(target) / D + 0.5);
-        L = std::max(0, std::min(L, D == 3 ? 8 : 12));
+        const int L = hint_levels(D, t->n_pts);
         t->hlevels = L;
-        const int64_t cells = D == 3 ? (((int64_t)1 << (3 * (L + 1))) - 1) / 7 : (((int64_t)1 << (2 * (L + 1))) - 1) / 3;
+        const int64_t cells = hint_cells(D, L);
         for (int k = 0; k < 3; ++k) t->glo[k] = (float)(-t->r);
         t->ginv = (float)(1.0 / (2.0 * t->r));
         if ((rc = dev_alloc(&t->hint, cells))) return rc;
@@ -635,6 +614,7 @@ int vo_tri_run(vo_tri* t) {
         // window plan
         const double target = std::max(8.0, t->gamma * (double)std::max<int64_t>(t->inserted, 1));
         int nwin = (int)std::min<double>(target, t->win_cap);
+        if (t->npend > t->win_cap) return fail(VO_ERR_INTERNAL, "pending list beyond the window capacity");
         nwin = std::max(nwin, t->npend);
         const int nfresh = (int)std::min<int64_t>(t->n_pts - t->head, nwin - t->npend);
         nwin = t->npend + nfresh;
@@ -771,39 +751,7 @@ int vo_tri_canonical(const vo_tri* t, int32_t** T, int64_t* n_T, int32_t** E, in
     VO_HIP(hipMemcpy(s.data(), ds, sizeof(int32_t) * s.size(), hipMemcpyDeviceToHost));
     hipFree(ds);
     hipFree(dn);
-    const int32_t first = 2 * M;
-    std::vector<std::array<int32_t, 4>> tris;
-    std::vector<std::array<int32_t, 2>> edges;
-    tris.reserve(nl);
-    edges.reserve((size_t)nl * (M == 4 ? 6 : 3));
-    for (int64_t i = 0; i < nl; ++i) {
-        std::array<int32_t, 4> r = {INT32_MAX, INT32_MAX, INT32_MAX, INT32_MAX};
-        bool real = true;
-        for (int k = 0; k < M; ++k) {
-            const int32_t v = s[i * M + k];
-            r[k] = v >= first ? v - first : -1;
-            real &= v >= first;
-        }
-        for (int a = 0; a < M; ++a)
-            for (int b = a + 1; b < M; ++b)
-                if (r[a] >= 0 && r[b] >= 0) edges.push_back({std::min(r[a], r[b]), std::max(r[a], r[b])});
-        if (real) {
-            std::sort(r.begin(), r.begin() + M);
-            tris.push_back(r);
-        }
-    }
-    std::sort(tris.begin(), tris.end());
-    std::sort(edges.begin(), edges.end());
-    edges.erase(std::unique(edges.begin(), edges.end()), edges.end());
-    *T = (int32_t*)std::malloc(sizeof(int32_t) * std::max<size_t>(1, tris.size() * M));
-    *E = (int32_t*)std::malloc(sizeof(int32_t) * std::max<size_t>(1, edges.size() * 2));
-    if (!*T || !*E) return fail(VO_ERR_OOM, "host allocation");
-    for (size_t i = 0; i < tris.size(); ++i)
-        for (int k = 0; k < M; ++k) (*T)[i * M + k] = tris[i][k];
-    for (size_t i = 0; i < edges.size(); ++i) { (*E)[2 * i] = edges[i][0]; (*E)[2 * i + 1] = edges[i][1]; }
-    *n_T = (int64_t)tris.size();
-    *n_E = (int64_t)edges.size();
-    return 0;
+    return canonical_from_simplices(M, s.data(), nl, T, n_T, E, n_E);
 }
 
 int vo_tri_check(vo_tri* t, int64_t* n_violations) {
