@@ -106,6 +106,177 @@ def measured_traffic(workload: str, kernel: str):
     return None
 
 
+ENGINE_STAT_KEYS = ("rounds", "replays", "solo_runs", "range_extensions", "admission_waits", "max_sets_resident",
+                    "ms_replays", "ms_boundary")
+
+
+def run_batched(seed: int = 0, plain_stride: int = 8) -> dict:
+    """Batching: 4,096 independent 3D sets, sizes floor(exp(U[ln 64, ln 65536])), points i.i.d.
+    uniform in [0,1)^3, through one engine (vo_batch_triangulate: device-planned rounds
+    replayed as HIP graphs, sets admitted by arena footprint as others complete).  Timed from
+    host points to every set's simplices on the host.  The plain path (vo_tri create + insert +
+    export + destroy, one set at a time) is timed on every plain_stride-th set, and each of
+    those sets' simplex sets is compared with the batched result (order-free hash)."""
+    from voronoids_amd import batch as B
+    from voronoids_amd import datasets
+    sizes = datasets.batch_sizes(4096, 64, 65536, seed=seed)
+    P, off = datasets.batch_sets(sizes, 3, seed=seed)
+    eng = B.Engine(3, max_points=18_000_000, max_simplices=(1 << 29) - 4096)
+    try:
+        eng.batch_flat(P[: off[64]], off[:65], seed)          # warm-up: graph capture, arena touch
+        base = eng.stats()
+        t0 = time.perf_counter()
+        res = eng.batch_flat(P, off, seed)
+        dt = time.perf_counter() - t0
+        st = eng.stats()
+    finally:
+        eng.close()
+    print(f"# batched: {len(res)} sets, {int(off[-1])} points in {dt:.2f} s", file=sys.stderr, flush=True)
+    idx = list(range(0, len(sizes), plain_stride))
+    t0 = time.perf_counter()
+    hashes = []
+    for k in idx:
+        s, _ = B.plain_simplices(P[off[k]:off[k + 1]], seed + k)
+        hashes.append(B.simplex_set_hash(s))
+    dtp = time.perf_counter() - t0                            # includes the hashing: subtracted below
+    t_hash = time.perf_counter()
+    for k in idx:
+        B.simplex_set_hash(res[k].simplices)
+    dth = time.perf_counter() - t_hash
+    dtp = max(dtp - dth, 1e-9)
+    same = sum(B.simplex_set_hash(res[k].simplices) == h for k, h in zip(idx, hashes))
+    pts_s = int(sum(sizes[k] for k in idx))
+    print(f"# plain path: {len(idx)} sets, {pts_s} points in {dtp:.2f} s; identical {same}/{len(idx)}",
+          file=sys.stderr, flush=True)
+    ok = all(r.status == 0 for r in res)
+    return {
+        "workload": "4,096 3D sets, sizes log-uniform [64, 65536], i.i.d. uniform [0,1)^3 f64",
+        "sets": len(res), "points": int(off[-1]), "s": dt, "sets_per_s": len(res) / dt, "pts_per_s": int(off[-1]) / dt,
+        "plain": {"sets_timed": len(idx), "points_timed": pts_s, "s": dtp, "sets_per_s": len(idx) / dtp,
+                  "pts_per_s": pts_s / dtp, "sample": f"every {plain_stride}th set"},
+        "speedup_sets_per_s": (len(res) / dt) / (len(idx) / dtp),
+        "speedup_pts_per_s": (int(off[-1]) / dt) / (pts_s / dtp),
+        "identical_output": {"compared_sets": len(idx), "identical": int(same), "all_status_ok": ok},
+        "engine": {k: st[k] - (base[k] if k not in ("max_sets_resident",) else 0) for k in ENGINE_STAT_KEYS},
+    }
+
+
+def _percentiles(x) -> dict:
+    x = np.asarray(x, dtype=np.float64)
+    return {"p50_ms": float(np.percentile(x, 50) * 1e3), "p99_ms": float(np.percentile(x, 99) * 1e3),
+            "mean_ms": float(x.mean() * 1e3), "max_ms": float(x.max() * 1e3)}
+
+
+def run_served(seed: int = 0, load: float = 0.8, target_s: float = 8.0) -> dict:
+    """Serving: Poisson arrivals at load / E[plain service time]; each request 2D or 3D (equal
+    odds), size log-uniform [1e3, 1e6], uniform in the unit square / cube.  The server (two
+    engines, continuous batching, admission by device-memory footprint) is driven in real time
+    by a submitting thread; latency = result ready - submission, on the library's clock.  The
+    plain path serves the same stream FIFO one request at a time: its service times are
+    measured back to back and its latencies follow from the arrival times.  A saturation run
+    (every request queued at once) gives both forms' throughput.  Every request's simplex set
+    is compared between the two (order-free hash)."""
+    import threading
+    from voronoids_amd import batch as B
+    from voronoids_amd import datasets
+    lo, hi = 1000, 1_000_000
+    # service-time model of the plain path, per dimension: a + b * n from four sizes
+    fit = {}
+    for dim in (2, 3):
+        ns, ts = [], []
+        for n in (1000, 10_000, 100_000, 1_000_000):
+            Pp = datasets.uniform(n, dim, seed=seed + 7, stream=dim)
+            B.plain_simplices(Pp, 1)                           # warm
+            t0 = time.perf_counter()
+            B.plain_simplices(Pp, 1)
+            ns.append(n)
+            ts.append(time.perf_counter() - t0)
+        b, a = np.polyfit(ns, ts, 1)
+        fit[dim] = (float(a), float(b))
+    mean_n = (hi - lo) / math.log(hi / lo)
+    es = 0.5 * sum(fit[d][0] + fit[d][1] * mean_n for d in (2, 3))
+    lam = load / es
+    n_req = int(min(max(target_s / es, 40), 1000))
+    u = datasets.unit(seed, 30, 0, 3 * n_req).reshape(3, n_req)
+    sizes = np.floor(np.exp(np.log(lo) + u[0] * (np.log(hi) - np.log(lo)))).astype(np.int64)
+    dims = np.where(u[1] < 0.5, 2, 3)
+    arrivals = np.cumsum(-np.log1p(-u[2]) / lam)
+    reqs = [datasets.uniform(int(sizes[k]), int(dims[k]), seed=seed + 31, stream=k) for k in range(n_req)]
+    print(f"# serving: {n_req} requests, E[S_plain] {es * 1e3:.1f} ms, rate {lam:.1f}/s, "
+          f"{int(sizes.sum())} points", file=sys.stderr, flush=True)
+
+    def serve(arrival_times):
+        srv = B.Server(max_points_2d=16_000_000, max_points_3d=16_000_000)
+        try:
+            for d in (2, 3):                                    # warm-up: graphs of both engines
+                srv.submit(datasets.uniform(2000, d, seed=99), seed=0, tag=-d)
+            got, warm = {}, 0
+            while warm < 2:
+                warm += sum(1 for r in srv.results(timeout_ms=500) if r.tag < 0)
+            err = []
+
+            def submitter():
+                t0 = time.perf_counter()
+                try:
+                    for k in range(n_req):
+                        w = t0 + arrival_times[k] - time.perf_counter()
+                        if w > 0:
+                            time.sleep(w)
+                        srv.submit(reqs[k], seed=1000 + k, tag=k)
+                except Exception as ex:                     # reported by the collector
+                    err.append(ex)
+            th = threading.Thread(target=submitter, daemon=True)
+            th.start()
+            hashes, lat, first, last = {}, {}, None, None
+            while len(hashes) < n_req and not err:
+                for r in srv.results(timeout_ms=200):
+                    if r.tag < 0:
+                        continue
+                    hashes[r.tag] = B.simplex_set_hash(r.simplices)
+                    lat[r.tag] = r.latency
+                    first = r.t_submit if first is None else min(first, r.t_submit)
+                    last = r.t_done if last is None else max(last, r.t_done)
+            th.join()
+            if err:
+                raise err[0]
+            stats = {d: srv.stats(d) for d in (2, 3)}
+        finally:
+            srv.close()
+        return hashes, lat, last - first, stats
+
+    h_srv, lat_srv, span, est = serve(arrivals)
+    print(f"# served (open loop): makespan {span:.2f} s, p50 {np.percentile(list(lat_srv.values()), 50) * 1e3:.1f} ms",
+          file=sys.stderr, flush=True)
+    _, _, span_sat, _ = serve(np.zeros(n_req))
+    # plain path: back-to-back service times, then FIFO latencies for the same arrivals
+    svc = np.zeros(n_req)
+    same = 0
+    for k in range(n_req):
+        t0 = time.perf_counter()
+        s, _ = B.plain_simplices(reqs[k], 1000 + k)
+        svc[k] = time.perf_counter() - t0
+        same += int(B.simplex_set_hash(s) == h_srv[k])
+    end = 0.0
+    lat_plain = np.zeros(n_req)
+    for k in range(n_req):
+        start = max(arrivals[k], end)
+        end = start + svc[k]
+        lat_plain[k] = end - arrivals[k]
+    out = {
+        "workload": f"{n_req} requests, 2D/3D equal odds, sizes log-uniform [1e3, 1e6], Poisson arrivals at "
+                    f"{load:.0%} of the plain path's capacity ({lam:.1f}/s)",
+        "served": {**_percentiles(list(lat_srv.values())), "sets_per_s": n_req / span,
+                   "saturated_sets_per_s": n_req / span_sat},
+        "plain_fifo": {**_percentiles(lat_plain), "sets_per_s": n_req / max(end - arrivals[0], 1e-9),
+                       "saturated_sets_per_s": n_req / svc.sum(), "mean_service_ms": float(svc.mean() * 1e3)},
+        "identical_output": {"compared_requests": n_req, "identical": same},
+        "engines": {str(d): {k: est[d][k] for k in ENGINE_STAT_KEYS} for d in (2, 3)},
+    }
+    out["saturated_speedup"] = out["served"]["saturated_sets_per_s"] / out["plain_fifo"]["saturated_sets_per_s"]
+    out["p99_ratio_plain_over_served"] = out["plain_fifo"]["p99_ms"] / out["served"]["p99_ms"]
+    return out
+
+
 def copy_calibration(nbytes: int, reps: int = 3) -> None:
     """Device-to-device copies of a known size (the runtime's copy kernel), so that rocprofv3's
     FETCH_SIZE / WRITE_SIZE can be scaled to bytes."""
@@ -130,6 +301,9 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="3d4m", choices=sorted(WORKLOADS))
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-batched", action="store_true", help="skip the batched-sets measurement")
+    ap.add_argument("--no-served", action="store_true", help="skip the served-requests measurement")
+    ap.add_argument("--only", choices=["batched", "served"], help="run only that measurement")
     ap.add_argument("--check", action="store_true", help="run the exact device check after the last step")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run device-to-device copies of BYTES (for rocprofv3 counter scaling)")
@@ -141,6 +315,11 @@ def main() -> int:
         print(f"bench.py: this build measures one GPU only (asked for {args.gpus}); "
               "multi-GPU sharding is designed but not built or measured", file=sys.stderr)
         return 2
+
+    if args.only:
+        r = run_batched() if args.only == "batched" else run_served()
+        print(json.dumps({"metric": args.only, **r}), flush=True)
+        return 0
 
     import voronoids_amd as vo
     w = WORKLOADS[args.workload]
@@ -187,7 +366,8 @@ def main() -> int:
                     "deferred": st["deferred"][i] / max(tot_calls[i], 1), "calls": tot_calls[i]}
              for i, name in enumerate(("orient", "insphere"))}
     out = {
-        "metric": "3D Delaunay inserted pts/sec @4M" if args.workload == "3d4m" else f"{dim}D Delaunay inserted pts/sec",
+        "metric": "3D Delaunay inserted pts/sec @4M; batched sets/s + p50/p99 latency vs plain GPU"
+        if args.workload == "3d4m" else f"{dim}D Delaunay inserted pts/sec",
         "value": value, "unit": "pts/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": ms, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f64", "data": "synthetic",
@@ -204,6 +384,10 @@ def main() -> int:
     }
     if viol is not None:
         out["violations"] = viol
+    if args.workload == "3d4m" and not args.no_batched:
+        out["batched"] = run_batched()
+    if args.workload == "3d4m" and not args.no_served:
+        out["served"] = run_served()
     if not args.no_cpu_baseline:
         threads = min(16, os.cpu_count() or 1)
         out["cpu_baseline"] = cpu_baseline(dim, threads)

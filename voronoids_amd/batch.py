@@ -65,6 +65,38 @@ def canonical_from_simplices(simplices: np.ndarray, dim: int) -> tuple[np.ndarra
     return Ta.reshape(-1, M), Ea.reshape(-1, 2)
 
 
+def plain_simplices(points, seed: int = 0, device: int = 0) -> tuple[np.ndarray, int]:
+    """the one-set path on one set: vo_tri_create + vo_tri_insert + vo_tri_export (simplices)
+    + vo_tri_destroy; returns the live simplices (reference id space) and the status"""
+    p = np.ascontiguousarray(points, dtype=np.float64)
+    L = _lib.lib()
+    h = ctypes.c_void_p()
+    _lib.check(L.vo_tri_create(device, p.shape[1], p.ctypes.data_as(_dp), p.shape[0], ctypes.byref(h)))
+    try:
+        rc = L.vo_tri_insert(h, p.ctypes.data_as(_dp), p.shape[0], seed)
+        if rc not in (_lib.VO_OK, _lib.VO_ERR_DUPLICATE):
+            _lib.check(rc)
+        nv, ns, mx = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64()
+        _lib.check(L.vo_tri_counts(h, ctypes.byref(nv), ctypes.byref(ns), ctypes.byref(mx)))
+        s = np.zeros((ns.value, p.shape[1] + 1), np.int32)
+        _lib.check(L.vo_tri_export(h, s.ctypes.data_as(_i32p), None))
+        return s, rc
+    finally:
+        L.vo_tri_destroy(h)
+
+
+def simplex_set_hash(simplices) -> int:
+    """order-independent 64-bit hash of a set of simplices (vertex tuples sorted, each mixed,
+    summed mod 2^64): equal simplex sets give equal hashes whatever their order"""
+    s = np.sort(np.asarray(simplices, dtype=np.int64), axis=1).astype(np.uint64)
+    h = np.zeros(s.shape[0], np.uint64)
+    with np.errstate(over="ignore"):
+        for k in range(s.shape[1]):
+            h = (h ^ s[:, k]) * np.uint64(0x9E3779B97F4A7C15)
+            h ^= h >> np.uint64(29)
+        return int(np.sum(h, dtype=np.uint64))
+
+
 def _take(r: _lib.VoSetResult) -> SetResult:
     """copy a C result into numpy and release its buffer"""
     M = r.dim + 1
@@ -151,11 +183,17 @@ class Engine:
         offs = np.zeros(len(arrs) + 1, np.int64)
         offs[1:] = np.cumsum([a.shape[0] for a in arrs])
         pts = np.concatenate(arrs) if arrs else np.zeros((0, self.dim))
-        pts = np.ascontiguousarray(pts)
-        res = (_lib.VoSetResult * max(len(arrs), 1))()
-        rc = _lib.lib().vo_batch_triangulate(self._h, pts.ctypes.data_as(_dp), offs.ctypes.data_as(_i64p), len(arrs),
+        return self.batch_flat(pts, offs, seed)
+
+    def batch_flat(self, pts, offsets, seed: int = 0) -> list[SetResult]:
+        """the same over concatenated points: set k = rows offsets[k] .. offsets[k+1]-1"""
+        pts = np.ascontiguousarray(pts, dtype=np.float64).reshape(-1, self.dim)
+        offs = np.ascontiguousarray(offsets, dtype=np.int64)
+        n_sets = offs.shape[0] - 1
+        res = (_lib.VoSetResult * max(n_sets, 1))()
+        rc = _lib.lib().vo_batch_triangulate(self._h, pts.ctypes.data_as(_dp), offs.ctypes.data_as(_i64p), n_sets,
                                              seed, res)
-        out = [_take(res[i]) for i in range(len(arrs))]
+        out = [_take(res[i]) for i in range(n_sets)]
         _lib.check(rc)
         return out
 
