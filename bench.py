@@ -107,7 +107,7 @@ def measured_traffic(workload: str, kernel: str):
 
 
 ENGINE_STAT_KEYS = ("rounds", "replays", "solo_runs", "range_extensions", "admission_waits", "max_sets_resident",
-                    "ms_replays", "ms_boundary")
+                    "ms_replays", "ms_boundary", "ms_admit", "ms_collect")
 
 
 def run_batched(seed: int = 0, plain_stride: int = 8) -> dict:

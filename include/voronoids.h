@@ -131,6 +131,7 @@ typedef struct {
     int64_t points_inserted, rounds, replays, solo_runs, range_extensions, admission_waits;
     int64_t vertex_capacity, simplex_capacity, simplex_in_use, max_sets_resident;
     double ms_replays, ms_boundary;      /* device time inside graph replays; host time between them */
+    double ms_admit, ms_collect;         /* parts of ms_boundary: admission (upload, scatter) and export */
 } vo_engine_stats;
 
 /* max_points: vertex-arena capacity (points resident at once, super vertices included);

@@ -45,7 +45,8 @@ class VoEngineStats(ctypes.Structure):
         "sets_submitted", "sets_admitted", "sets_done", "sets_resident", "sets_queued",
         "points_inserted", "rounds", "replays", "solo_runs", "range_extensions", "admission_waits",
         "vertex_capacity", "simplex_capacity", "simplex_in_use", "max_sets_resident")] + [
-        ("ms_replays", ctypes.c_double), ("ms_boundary", ctypes.c_double)]
+        ("ms_replays", ctypes.c_double), ("ms_boundary", ctypes.c_double), ("ms_admit", ctypes.c_double),
+        ("ms_collect", ctypes.c_double)]
 
     def as_dict(self) -> dict:
         return {name: getattr(self, name) for name, _ in self._fields_}

@@ -10,6 +10,7 @@ i = 2(d+1) + i).  There is no CPU fallback.
 from __future__ import annotations
 
 import ctypes
+import weakref
 from dataclasses import dataclass
 
 import numpy as np
@@ -98,16 +99,20 @@ def simplex_set_hash(simplices) -> int:
 
 
 def _take(r: _lib.VoSetResult) -> SetResult:
-    """copy a C result into numpy and release its buffer"""
+    """wrap a C result's buffer as a numpy array without copying; vo_free runs when the array
+    is collected"""
     M = r.dim + 1
     n = r.n_simplices
-    if n > 0 and r.simplices:
-        s = np.ctypeslib.as_array(r.simplices, shape=(n * M,)).copy().reshape(n, M)
+    addr = ctypes.cast(r.simplices, ctypes.c_void_p).value if r.simplices else None
+    if n > 0 and addr:
+        buf = (ctypes.c_int32 * (n * M)).from_address(addr)
+        weakref.finalize(buf, _lib.lib().vo_free, ctypes.c_void_p(addr))
+        s = np.frombuffer(buf, dtype=np.int32).reshape(n, M)
     else:
         s = np.zeros((0, M), np.int32)
-    if r.simplices:
-        _lib.lib().vo_free(ctypes.cast(r.simplices, ctypes.c_void_p))
-        r.simplices = _i32p()
+        if addr:
+            _lib.lib().vo_free(ctypes.c_void_p(addr))
+    r.simplices = _i32p()
     return SetResult(r.tag, r.dim, r.status, r.n_points, r.points_duplicate, s, r.t_submit, r.t_admit, r.t_done,
                      r.rounds)
 

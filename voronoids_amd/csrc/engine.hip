@@ -19,6 +19,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <condition_variable>
 #include <cstdint>
@@ -26,6 +27,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <deque>
+#include <functional>
 #include <map>
 #include <memory>
 #include <mutex>
@@ -409,7 +411,15 @@ struct HSet {
     int64_t tag = 0;
     int64_t n = 0;
     uint64_t seed = 0;
-    std::vector<double> pts;               // until admitted
+    std::vector<double> pts;               // as submitted, until prepared
+    const double* ext = nullptr;           // or the caller's buffer (vo_batch_triangulate), until prepared
+    // prepared by the engine's worker threads: frame, super simplex, points in priority order
+    std::atomic<bool> ready{false};
+    double c[3] = {0, 0, 0}, r = 1.0;
+    double sv[4][3] = {};
+    int tet0[4] = {0, 1, 2, 3};
+    std::vector<double> packed;            // n x dim, priority order
+    std::vector<int32_t> inidx;            // input index of each packed point
     int slot = -1;
     int64_t v0 = 0, h0 = 0, hcells = 0;
     int hlevels = 0;
@@ -417,6 +427,26 @@ struct HSet {
     double t_submit = 0, t_admit = 0;
     unsigned round_admit = 0;
 };
+
+// host work of a set that needs no engine state: the reference's bounding sphere (its frame),
+// the super simplex, and the points gathered in priority order
+void prepare_set(HSet* h, int D) {
+    const double* pts = h->ext ? h->ext : h->pts.data();
+    set_frame(D, pts, h->n, h->c, &h->r);
+    super_vertices(D, h->c, h->r * 10.0, h->sv);
+    first_simplex_order(D, h->sv, h->tet0);
+    const std::vector<int32_t> perm = priority_order(h->n, h->seed);
+    h->packed.resize((size_t)h->n * D);
+    h->inidx.resize((size_t)h->n);
+    for (int64_t i = 0; i < h->n; ++i) {
+        const int32_t src = perm[i];
+        for (int j = 0; j < D; ++j) h->packed[(size_t)i * D + j] = pts[(size_t)src * D + j];
+        h->inidx[i] = src;
+    }
+    std::vector<double>().swap(h->pts);
+    h->ext = nullptr;
+    h->ready.store(true, std::memory_order_release);
+}
 
 }  // namespace
 
@@ -469,13 +499,103 @@ struct vo_engine {
     bool launched = false;
     int last_nwin = 0;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    // host worker threads: prepare queued sets ahead of admission, copy exported results
+    std::vector<std::thread> workers;
+    std::mutex prep_mu;
+    std::condition_variable prep_cv, ready_cv;
+    std::deque<std::function<void()>> tasks;   // copies go to the front, preparations to the back
+    bool prep_stop = false;
+    int32_t* out_h = nullptr;                   // pinned bounce buffer of the export
     // stats
     vo_engine_stats stats{};
 };
 
 namespace {
 
+void worker(vo_engine* e) {
+    for (;;) {
+        std::function<void()> task;
+        {
+            std::unique_lock<std::mutex> lk(e->prep_mu);
+            e->prep_cv.wait(lk, [&] { return e->prep_stop || !e->tasks.empty(); });
+            if (e->prep_stop) return;
+            task = std::move(e->tasks.front());
+            e->tasks.pop_front();
+        }
+        task();
+    }
+}
+
+void stop_workers(vo_engine* e) {
+    {
+        std::lock_guard<std::mutex> lk(e->prep_mu);
+        e->prep_stop = true;
+        e->tasks.clear();
+    }
+    e->prep_cv.notify_all();
+    for (auto& t : e->workers)
+        if (t.joinable()) t.join();
+    e->workers.clear();
+}
+
+void enqueue(vo_engine* e, std::unique_ptr<HSet> h) {
+    HSet* raw = h.get();
+    e->queue.push_back(std::move(h));
+    e->stats.sets_submitted++;
+    const int D = e->D;
+    {
+        std::lock_guard<std::mutex> lk(e->prep_mu);
+        e->tasks.push_back([e, raw, D] {
+            prepare_set(raw, D);
+            { std::lock_guard<std::mutex> l2(e->prep_mu); }
+            e->ready_cv.notify_all();
+        });
+    }
+    e->prep_cv.notify_one();
+}
+
+// block until the queue's first set is prepared
+void wait_ready(vo_engine* e, HSet* h) {
+    if (h->ready.load(std::memory_order_acquire)) return;
+    std::unique_lock<std::mutex> lk(e->prep_mu);
+    e->ready_cv.wait(lk, [&] { return h->ready.load(std::memory_order_acquire); });
+}
+
+// run fn(0 .. n-1) on the workers (ahead of queued preparations) and on this thread.  The
+// shared counters live in a shared_ptr that every helper holds, so a helper that starts after
+// the loop is over finds no index left and touches nothing else; fn is called only for indices
+// taken before the last one completes, i.e. while this call is still waiting.
+void parallel_for(vo_engine* e, int n, const std::function<void(int)>& fn) {
+    if (n <= 0) return;
+    struct Shared {
+        std::atomic<int> next{0}, done{0};
+        std::mutex m;
+        std::condition_variable cv;
+    };
+    auto sh = std::make_shared<Shared>();
+    const std::function<void(int)>* f = &fn;
+    auto body = [sh, f, n] {
+        for (int i; (i = sh->next.fetch_add(1)) < n;) {
+            (*f)(i);
+            if (sh->done.fetch_add(1) + 1 == n) {
+                std::lock_guard<std::mutex> lk(sh->m);
+                sh->cv.notify_all();
+            }
+        }
+    };
+    const int helpers = std::min<int>((int)e->workers.size(), n - 1);
+    {
+        std::lock_guard<std::mutex> lk(e->prep_mu);
+        for (int k = 0; k < helpers; ++k) e->tasks.push_front(body);
+    }
+    e->prep_cv.notify_all();
+    body();
+    std::unique_lock<std::mutex> lk(sh->m);
+    sh->cv.wait(lk, [&] { return sh->done.load() == n; });
+}
+
 void engine_free(vo_engine* e) {
+    stop_workers(e);
     void* ptrs[] = {e->a.X64, e->a.X32, e->a.cp, e->a.vmap, e->a.pset, e->a.tv, e->a.ta, e->a.tx, e->a.hint, e->a.sets,
                     e->kres, e->nres, e->win, e->pend[0], e->pend[1], e->cav, e->fac, e->nk, e->nf, e->status, e->big, e->cav1,
                     e->fac1, e->misc1, e->st, e->ctl, e->cnt, e->q, e->cache, e->ws, e->done, e->stage_d, e->out_d};
@@ -484,6 +604,7 @@ void engine_free(vo_engine* e) {
     if (e->st_host) hipHostFree(e->st_host);
     if (e->ctl_host) hipHostFree(e->ctl_host);
     if (e->stage_h) hipHostFree(e->stage_h);
+    if (e->out_h) hipHostFree(e->out_h);
     for (auto& g : e->graph)
         if (g) hipGraphExecDestroy(g);
     if (e->ev0) hipEventDestroy(e->ev0);
@@ -658,6 +779,10 @@ int admit(vo_engine* e) {
     std::vector<HSet*> adm;
     while (!e->queue.empty() && !e->free_slots.empty()) {
         HSet* h = e->queue.front().get();
+        if (!h->ready.load(std::memory_order_acquire)) {
+            if (e->n_resident > 0 || !adm.empty()) break;   // admitted at a later boundary
+            wait_ready(e, h);
+        }
         const int L = hint_levels(e->D, h->n);
         const int64_t hc = hint_cells(e->D, L);
         const int64_t tl = first_range_len(e, h->n);
@@ -709,17 +834,10 @@ int admit(vo_engine* e) {
         HSet* h = adm[k];
         AdmitDesc& d = desc[k];
         std::memset(&d, 0, sizeof(d));
-        double c[3], r;
-        set_frame(D, h->pts.data(), h->n, c, &r);
-        const double R = r * 10.0;
-        double sv[4][3] = {};
-        super_vertices(D, c, R, sv);
-        int tet0[4];
-        first_simplex_order(D, sv, tet0);
-        for (int j = 0; j < 3; ++j) d.c[j] = c[j];
-        std::memcpy(d.sv, sv, sizeof(sv));
-        for (int j = 0; j < 3; ++j) d.glo[j] = (float)(-r);
-        d.ginv = (float)(1.0 / (2.0 * r));
+        for (int j = 0; j < 3; ++j) d.c[j] = h->c[j];
+        std::memcpy(d.sv, h->sv, sizeof(d.sv));
+        for (int j = 0; j < 3; ++j) d.glo[j] = (float)(-h->r);
+        d.ginv = (float)(1.0 / (2.0 * h->r));
         d.pt_off = off;
         d.slot = h->slot;
         d.n = (int)h->n;
@@ -729,16 +847,20 @@ int admit(vo_engine* e) {
         d.hint_off = (int)h->h0;
         d.hlevels = h->hlevels;
         d.hcells = (int)h->hcells;
-        for (int j = 0; j < 4; ++j) d.tet0[j] = tet0[j];
-        const std::vector<int32_t> perm = priority_order(h->n, h->seed);
-        for (int64_t i = 0; i < h->n; ++i) {
-            const int32_t src = perm[i];
-            for (int j = 0; j < D; ++j) P[(off + i) * D + j] = h->pts[(size_t)src * D + j];
-            I[off + i] = src;
-        }
+        for (int j = 0; j < 4; ++j) d.tet0[j] = h->tet0[j];
         off += h->n;
-        std::vector<double>().swap(h->pts);
     }
+    // the prepared points into the pinned staging buffer, one set per worker task
+    parallel_for(e, (int)adm.size(), [&](int k) {
+        HSet* h = adm[k];
+        const long long o = desc[k].pt_off;
+        if (h->n) {
+            std::memcpy(P + o * D, h->packed.data(), sizeof(double) * (size_t)h->n * D);
+            std::memcpy(I + o, h->inidx.data(), sizeof(int32_t) * (size_t)h->n);
+        }
+        std::vector<double>().swap(h->packed);
+        std::vector<int32_t>().swap(h->inidx);
+    });
     VO_HIP(hipMemcpyAsync(e->stage_d, e->stage_h, total, hipMemcpyHostToDevice, e->s));
     const AdmitDesc* dd = reinterpret_cast<const AdmitDesc*>(e->stage_d);
     const double* Pd = reinterpret_cast<const double*>(e->stage_d + pts_off);
@@ -880,12 +1002,16 @@ int collect(vo_engine* e, int n_done) {
             const long long lv = dr[k1].live;
             if (k1 > k0 && (size_t)(rows + lv) * M > e->out_cap) break;
             if ((size_t)lv * M > e->out_cap) {
-                // a set larger than the buffer: grow it
+                // a set larger than the buffer: grow it (device buffer and its pinned bounce)
                 hipFree(e->out_d);
+                hipHostFree(e->out_h);
                 e->out_d = nullptr;
+                e->out_h = nullptr;
                 e->out_cap = 0;
                 int rc = dev_alloc(&e->out_d, (size_t)lv * M);
                 if (rc) return rc;
+                if (hipHostMalloc((void**)&e->out_h, sizeof(int32_t) * (size_t)lv * M) != hipSuccess)
+                    return fail(VO_ERR_OOM, "pinned export buffer");
                 e->out_cap = (size_t)lv * M;
             }
             obase.push_back(rows);
@@ -926,17 +1052,24 @@ int collect(vo_engine* e, int n_done) {
         VO_HIP(hipGetLastError());
         std::vector<int> ocur(ns);
         VO_HIP(hipMemcpyAsync(ocur.data(), e->stage_d + oc_off, sizeof(int) * ns, hipMemcpyDeviceToHost, e->s));
-        std::vector<int32_t*> bufs(ns, nullptr);
-        for (int i = 0; i < ns; ++i) {
-            bufs[i] = (int32_t*)std::malloc(sizeof(int32_t) * std::max<size_t>(1, (size_t)dr[k0 + i].live * M));
-            if (!bufs[i]) return fail(VO_ERR_OOM, "host result buffer");
-        }
-        // one device-to-host copy per set, straight into its result buffer
-        for (int i = 0; i < ns; ++i)
-            if (dr[k0 + i].live > 0)
-                VO_HIP(hipMemcpyAsync(bufs[i], e->out_d + (size_t)obase[i] * M, sizeof(int32_t) * (size_t)dr[k0 + i].live * M,
-                                      hipMemcpyDeviceToHost, e->s));
+        // one device-to-host copy into the pinned bounce buffer, then every set's rows into
+        // its own result buffer on the worker threads
+        if (rows > 0)
+            VO_HIP(hipMemcpyAsync(e->out_h, e->out_d, sizeof(int32_t) * (size_t)rows * M, hipMemcpyDeviceToHost, e->s));
         VO_HIP(hipStreamSynchronize(e->s));
+        std::vector<int32_t*> bufs(ns, nullptr);
+        const int32_t* src = e->out_h;
+        parallel_for(e, ns, [&](int i) {
+            const size_t cnt = (size_t)dr[k0 + i].live * M;
+            int32_t* b = (int32_t*)std::malloc(sizeof(int32_t) * std::max<size_t>(1, cnt));
+            if (b && cnt) std::memcpy(b, src + (size_t)obase[i] * M, sizeof(int32_t) * cnt);
+            bufs[i] = b;
+        });
+        for (int i = 0; i < ns; ++i)
+            if (!bufs[i]) {
+                for (int j = 0; j < ns; ++j) std::free(bufs[j]);
+                return fail(VO_ERR_OOM, "host result buffer");
+            }
         const double t = now_s();
         for (int i = 0; i < ns; ++i) {
             const DoneRec& d = dr[k0 + i];
@@ -1038,6 +1171,8 @@ int vo_engine_create(int hip_device, int dim, int64_t max_points, int64_t max_si
         return bail(rc);
     E->out_cap = (size_t)1 << 26;
     if ((rc = dev_alloc(&E->out_d, E->out_cap))) return bail(rc);
+    if (hipHostMalloc((void**)&E->out_h, sizeof(int32_t) * E->out_cap) != hipSuccess)
+        return bail(fail(VO_ERR_OOM, "pinned export buffer"));
     if (hipHostMalloc((void**)&E->st_host, sizeof(RoundState)) != hipSuccess ||
         hipHostMalloc((void**)&E->ctl_host, sizeof(BatchCtl)) != hipSuccess)
         return bail(fail(VO_ERR_OOM, "pinned"));
@@ -1059,6 +1194,11 @@ int vo_engine_create(int hip_device, int dim, int64_t max_points, int64_t max_si
         hipMemsetAsync(E->cnt, 0, sizeof(Counters), E->s) != hipSuccess || hipStreamSynchronize(E->s) != hipSuccess)
         return bail(fail(VO_ERR_HIP, "engine init"));
     if (hipGetLastError() != hipSuccess) return bail(fail(VO_ERR_HIP, "engine init kernels"));
+    {
+        const unsigned hw = std::thread::hardware_concurrency();
+        const int nw = (int)std::max(1u, std::min(8u, hw ? hw / 2 : 1u));
+        for (int i = 0; i < nw; ++i) E->workers.emplace_back(worker, E);
+    }
     E->valloc.init(E->vcap);
     E->talloc.init(E->tcap);
     E->halloc.init(E->hcap);
@@ -1131,8 +1271,7 @@ int vo_engine_submit(vo_engine* e, const double* pts, int64_t n, uint64_t seed, 
     h->seed = seed;
     h->pts.assign(pts, pts + n * e->D);
     h->t_submit = now_s();
-    e->queue.push_back(std::move(h));
-    e->stats.sets_submitted++;
+    enqueue(e, std::move(h));
     return 0;
 }
 
@@ -1142,7 +1281,9 @@ int vo_engine_launch(vo_engine* e) {
     VO_HIP(hipSetDevice(e->dev));
     const double t0 = now_s();
     int rc = admit(e);
-    e->stats.ms_boundary += (now_s() - t0) * 1e3;
+    const double ms = (now_s() - t0) * 1e3;
+    e->stats.ms_boundary += ms;
+    e->stats.ms_admit += ms;
     if (rc) return rc;
     if (e->n_resident == 0) return 0;
     const int64_t pred = std::max<int64_t>((int64_t)e->last_nwin * 3 / 2, e->n_resident * (int64_t)e->min_window);
@@ -1175,7 +1316,9 @@ int vo_engine_finish(vo_engine* e, int64_t* n_left) {
         if ((rc = check_err(e))) return rc;
         if (e->st_host->n_big > 0 && (rc = run_solo(e))) return rc;
         if (e->st_host->n_nospace > 0 && (rc = extend_ranges(e))) return rc;
+        const double tc = now_s();
         if ((rc = collect(e, e->ctl_host->n_done))) return rc;
+        e->stats.ms_collect += (now_s() - tc) * 1e3;
         hipLaunchKernelGGL(k_boundary_reset, dim3(1), dim3(1), 0, e->s, e->ctl, e->st, e->ctl_host->round);
         VO_HIP(hipGetLastError());
         e->ctl_host->n_done = 0;
@@ -1218,19 +1361,49 @@ int vo_batch_triangulate(vo_engine* e, const double* pts, const int64_t* set_off
     if (!e || !set_offsets || (n_sets > 0 && (!pts || !results))) return fail(VO_ERR_INVALID, "null argument");
     if (n_sets < 0) return fail(VO_ERR_INVALID, "n_sets < 0");
     std::memset(results, 0, sizeof(vo_set_result) * (size_t)n_sets);
+    VO_HIP(hipSetDevice(e->dev));
     for (int64_t k = 0; k < n_sets; ++k) {
         const int64_t a = set_offsets[k], b = set_offsets[k + 1];
-        if (b < a) return fail(VO_ERR_INVALID, "set offsets must be non-decreasing");
-        int rc = vo_engine_submit(e, pts + a * e->D, b - a, seed + (uint64_t)k, k);
-        if (rc) return rc;
+        if (a < 0 || b < a) return fail(VO_ERR_INVALID, "set offsets must be non-decreasing from 0");
+        if (b - a + e->M > e->vcap) return fail(VO_ERR_INVALID, "set larger than the engine's vertex arena");
     }
+    {
+        std::atomic<int> bad{0};
+        const int D = e->D;
+        parallel_for(e, (int)n_sets, [&](int k) {
+            for (int64_t i = set_offsets[k] * D; i < set_offsets[k + 1] * D; ++i)
+                if (!std::isfinite(pts[i])) { bad.store(1); return; }
+        });
+        if (bad.load()) return fail(VO_ERR_INVALID, "non-finite coordinate");
+    }
+    // the sets read the caller's buffer until prepared: no copy; every queued set of this call
+    // is waited for before an early return
+    std::vector<HSet*> mine;
+    mine.reserve((size_t)n_sets);
+    const double t_sub = now_s();
+    for (int64_t k = 0; k < n_sets; ++k) {
+        std::unique_ptr<HSet> h(new HSet());
+        h->tag = k;
+        h->n = set_offsets[k + 1] - set_offsets[k];
+        h->seed = seed + (uint64_t)k;
+        h->ext = pts + set_offsets[k] * e->D;
+        h->t_submit = t_sub;
+        mine.push_back(h.get());
+        enqueue(e, std::move(h));
+    }
+    auto settle = [&](int rc) {
+        for (auto& q : e->queue)
+            for (HSet* m : mine)
+                if (q.get() == m) wait_ready(e, m);
+        return rc;
+    };
     std::vector<char> got((size_t)n_sets, 0);
     int64_t left = 1, have = 0;
     std::vector<vo_set_result> buf(256);
     int64_t last_progress = e->stats.points_inserted, stall = 0;
     while (left > 0 || have < n_sets) {
         int rc = vo_engine_step(e, &left);
-        if (rc) return rc;
+        if (rc) return settle(rc);
         int64_t n = 0;
         do {
             vo_engine_results(e, buf.data(), (int64_t)buf.size(), &n);
@@ -1251,8 +1424,8 @@ int vo_batch_triangulate(vo_engine* e, const double* pts, const int64_t* set_off
         const int64_t prog = sst.points_inserted + sst.sets_done + sst.range_extensions + sst.sets_admitted;
         stall = prog == last_progress ? stall + 1 : 0;
         last_progress = prog;
-        if (left == 0 && have < n_sets) return fail(VO_ERR_INTERNAL, "engine drained with results missing");
-        if (stall > 64) return fail(VO_ERR_OOM, "no progress for 64 replays: simplex arena exhausted");
+        if (left == 0 && have < n_sets) return settle(fail(VO_ERR_INTERNAL, "engine drained with results missing"));
+        if (stall > 64) return settle(fail(VO_ERR_OOM, "no progress for 64 replays: simplex arena exhausted"));
     }
     return 0;
 }
@@ -1319,8 +1492,7 @@ void server_loop(vo_server* s) {
             h->seed = r.seed;
             h->pts.swap(r.pts);
             h->t_submit = r.t_submit;
-            e->queue.push_back(std::move(h));
-            e->stats.sets_submitted++;
+            enqueue(e, std::move(h));
         }
         for (vo_engine* e : s->eng)
             if (!rc && (!e->queue.empty() || e->n_resident > 0)) rc = vo_engine_launch(e);
