@@ -93,7 +93,9 @@ int vo_tri_stats(const vo_tri* t, vo_stats* out);
  * expansion tier.  "group_lanes" = lanes cooperating on one window point in the round
  * kernels (1, 4, 8, 16, 32; 0 = chosen per round: the largest size with window x size <=
  * "lanes_target", default 2^20 in 3D and 2^16 in 2D, from the autotuner).  "window_gamma_milli" = initial window
- * size per inserted point x 1000. */
+ * size per inserted point x 1000.  "local_order" = 0 (default: the seeded random priority order) or k > 0
+ * (the same order with consecutive chunks of position/k points sorted along the Morton curve; set before
+ * vo_tri_stage). */
 int vo_tri_config(vo_tri* t, const char* key, int64_t value);
 
 /* input vertex coordinates in the reference id space (for PyVertex.point): (2(dim+1) + n) x dim */

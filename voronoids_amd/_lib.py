@@ -8,7 +8,8 @@ import ctypes
 import os
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libvoronoids_hip.so")
+# VORONOIDS_LIB names another build of the same library (e.g. a tuning variant)
+LIB_PATH = os.environ.get("VORONOIDS_LIB") or os.path.join(HERE, "libvoronoids_hip.so")
 
 VO_OK = 0
 VO_ERR_INVALID = -1

@@ -60,6 +60,7 @@ struct AdmitDesc {
     long long pt_off;      // first staged point of the set
     int slot, n, v0, root, range_end, hint_off, hlevels, hcells;
     int tet0[4];
+    unsigned long long root_rec[16];   // the first simplex's record (128 B in 3D, 64 B in 2D)
 };
 
 struct ExportChunk {
@@ -221,12 +222,19 @@ struct Arena {
     int* cp;
     int* vmap;
     int* pset;
-    int4* tv;
-    int4* ta;
-    float4* tx;
+    char* rec;             // simplex records
     int* hint;
     SetRec* sets;
 };
+
+// records of a fresh simplex range: all bytes 0xFF (dead, hull adjacency, largest reservation
+// keys), written as 64-bit words
+template <int D>
+__device__ __forceinline__ void clear_records(char* rec, int t0, int t1) {
+    constexpr int W = (D == 3 ? 128 : 64) / 8;
+    unsigned long long* q = reinterpret_cast<unsigned long long*>(rec);
+    for (long long i = (long long)t0 * W + threadIdx.x; i < (long long)t1 * W; i += blockDim.x) q[i] = ~0ull;
+}
 
 // one block per admitted set: hint grid cleared (level 0 = the first simplex), simplex range
 // marked dead, super vertices, first simplex and set record written
@@ -235,7 +243,7 @@ __global__ void __launch_bounds__(256) k_admit_sets(const AdmitDesc* d, Arena a)
     constexpr int M = D + 1, S = D == 3 ? 4 : 2;
     const AdmitDesc& e = d[blockIdx.x];
     for (int i = threadIdx.x; i < e.hcells; i += 256) a.hint[e.hint_off + i] = -1;
-    for (int t = e.root + 1 + threadIdx.x; t < e.range_end; t += 256) a.tv[t] = make_int4(-1, 0, 0, 0);
+    clear_records<D>(a.rec, e.root + 1, e.range_end);
     __syncthreads();
     if (threadIdx.x != 0) return;
     a.hint[e.hint_off] = e.root;
@@ -247,14 +255,8 @@ __global__ void __launch_bounds__(256) k_admit_sets(const AdmitDesc* d, Arena a)
         }
         a.vmap[e.v0 + k] = k;
     }
-    int4 tv0 = make_int4(e.v0 + e.tet0[0], e.v0 + e.tet0[1], e.v0 + e.tet0[2], D == 3 ? e.v0 + e.tet0[3] : -1);
-    a.tv[e.root] = tv0;
-    a.ta[e.root] = make_int4(-1, -1, -1, -1);
-    float rec[16] = {};
-    for (int k = 0; k < M; ++k)
-        for (int j = 0; j < D; ++j) rec[(D == 3 ? 4 : 2) * k + j] = (float)(e.sv[e.tet0[k]][j] - e.c[j]);
-    float4* q = a.tx + (size_t)(D == 3 ? 4 : 2) * e.root;
-    for (int k = 0; k < (D == 3 ? 4 : 2); ++k) q[k] = make_float4(rec[4 * k], rec[4 * k + 1], rec[4 * k + 2], rec[4 * k + 3]);
+    unsigned long long* q = reinterpret_cast<unsigned long long*>(a.rec + (size_t)e.root * (D == 3 ? 128 : 64));
+    for (int k = 0; k < (D == 3 ? 16 : 8); ++k) q[k] = e.root_rec[k];
     SetRec r = {};
     r.cursor = e.root + 1;
     r.end = e.range_end;
@@ -298,9 +300,10 @@ __global__ void __launch_bounds__(256) k_admit_points(const AdmitDesc* d, int nd
 }
 
 // a fresh simplex range for a set whose range filled up
-__global__ void __launch_bounds__(256) k_extend(const ExtendRec* x, int4* tv, SetRec* sets) {
+template <int D>
+__global__ void __launch_bounds__(256) k_extend(const ExtendRec* x, char* rec, SetRec* sets) {
     const ExtendRec e = x[blockIdx.x];
-    for (int t = e.start + threadIdx.x; t < e.start + e.len; t += 256) tv[t] = make_int4(-1, 0, 0, 0);
+    clear_records<D>(rec, e.start, e.start + e.len);
     if (threadIdx.x == 0) {
         sets[e.slot].cursor = e.start;
         sets[e.slot].end = e.start + e.len;
@@ -311,7 +314,7 @@ __global__ void __launch_bounds__(256) k_extend(const ExtendRec* x, int4* tv, Se
 // live simplices of completed sets, vertex ids mapped to the reference id space, compacted
 // per set (block scan + one atomicAdd per 256 simplices on the set's output cursor)
 template <int M>
-__global__ void __launch_bounds__(256) k_export_sets(const ExportChunk* ch, int nch, const int4* tv, const int* vmap,
+__global__ void __launch_bounds__(256) k_export_sets(const ExportChunk* ch, int nch, Ctx c, const int* vmap,
                                                      const long long* obase, const int* olen, int* ocur, int32_t* out) {
     __shared__ int s_w[4 + 1];
     __shared__ int s_base;
@@ -319,7 +322,7 @@ __global__ void __launch_bounds__(256) k_export_sets(const ExportChunk* ch, int 
         const ExportChunk e = ch[b];
         for (int o = 0; o < e.len; o += 256) {
             const bool in = o + (int)threadIdx.x < e.len;
-            const int4 v = in ? tv[e.start + o + threadIdx.x] : make_int4(-1, 0, 0, 0);
+            const int4 v = in ? ld_v<M - 1>(c, e.start + o + threadIdx.x) : make_int4(-1, 0, 0, 0);
             const int live = in && v.x >= 0;
             const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
             int x = live;
@@ -435,7 +438,7 @@ void prepare_set(HSet* h, int D) {
     set_frame(D, pts, h->n, h->c, &h->r);
     super_vertices(D, h->c, h->r * 10.0, h->sv);
     first_simplex_order(D, h->sv, h->tet0);
-    const std::vector<int32_t> perm = priority_order(h->n, h->seed);
+    const std::vector<int32_t> perm = priority_order_local(D, pts, h->n, h->seed, h->c, h->r);
     h->packed.resize((size_t)h->n * D);
     h->inidx.resize((size_t)h->n);
     for (int64_t i = 0; i < h->n; ++i) {
@@ -460,7 +463,6 @@ struct vo_engine {
     int ws_threads = 1024, ws_cap = 1 << 14;
     // device arrays
     Arena a{};
-    unsigned long long *kres = nullptr, *nres = nullptr;
     int *win = nullptr, *pend[2] = {nullptr, nullptr}, *cav = nullptr, *fac = nullptr, *nk = nullptr, *nf = nullptr,
         *status = nullptr, *big = nullptr, *cav1 = nullptr, *fac1 = nullptr, *misc1 = nullptr;
     RoundState* st = nullptr;
@@ -596,8 +598,8 @@ void parallel_for(vo_engine* e, int n, const std::function<void(int)>& fn) {
 
 void engine_free(vo_engine* e) {
     stop_workers(e);
-    void* ptrs[] = {e->a.X64, e->a.X32, e->a.cp, e->a.vmap, e->a.pset, e->a.tv, e->a.ta, e->a.tx, e->a.hint, e->a.sets,
-                    e->kres, e->nres, e->win, e->pend[0], e->pend[1], e->cav, e->fac, e->nk, e->nf, e->status, e->big, e->cav1,
+    void* ptrs[] = {e->a.X64, e->a.X32, e->a.cp, e->a.vmap, e->a.pset, e->a.rec, e->a.hint, e->a.sets,
+                    e->win, e->pend[0], e->pend[1], e->cav, e->fac, e->nk, e->nf, e->status, e->big, e->cav1,
                     e->fac1, e->misc1, e->st, e->ctl, e->cnt, e->q, e->cache, e->ws, e->done, e->stage_d, e->out_d};
     for (void* p : ptrs)
         if (p) hipFree(p);
@@ -615,11 +617,7 @@ void engine_free(vo_engine* e) {
 
 Ctx engine_ctx(vo_engine* e) {
     Ctx c{};
-    c.tv = e->a.tv;
-    c.ta = e->a.ta;
-    c.tx = e->a.tx;
-    c.kres = e->kres;
-    c.nres = e->nres;
+    c.rec = e->a.rec;
     c.X64 = e->a.X64;
     c.X32 = e->a.X32;
     c.cp = e->a.cp;
@@ -848,6 +846,7 @@ int admit(vo_engine* e) {
         d.hlevels = h->hlevels;
         d.hcells = (int)h->hcells;
         for (int j = 0; j < 4; ++j) d.tet0[j] = h->tet0[j];
+        root_record(D, h->tet0, (int)h->v0, h->sv, h->c, reinterpret_cast<unsigned char*>(d.root_rec));
         off += h->n;
     }
     // the prepared points into the pinned staging buffer, one set per worker task
@@ -976,8 +975,12 @@ int extend_ranges(vo_engine* e) {
     if (rc) return rc;
     std::memcpy(e->stage_h, ext.data(), sizeof(ExtendRec) * ext.size());
     VO_HIP(hipMemcpyAsync(e->stage_d, e->stage_h, sizeof(ExtendRec) * ext.size(), hipMemcpyHostToDevice, e->s));
-    hipLaunchKernelGGL(k_extend, dim3((unsigned)ext.size()), dim3(256), 0, e->s,
-                       reinterpret_cast<const ExtendRec*>(e->stage_d), e->a.tv, e->a.sets);
+    if (e->D == 3)
+        hipLaunchKernelGGL(k_extend<3>, dim3((unsigned)ext.size()), dim3(256), 0, e->s,
+                           reinterpret_cast<const ExtendRec*>(e->stage_d), e->a.rec, e->a.sets);
+    else
+        hipLaunchKernelGGL(k_extend<2>, dim3((unsigned)ext.size()), dim3(256), 0, e->s,
+                           reinterpret_cast<const ExtendRec*>(e->stage_d), e->a.rec, e->a.sets);
     VO_HIP(hipGetLastError());
     VO_HIP(hipStreamSynchronize(e->s));
     e->stats.range_extensions += (int64_t)ext.size();
@@ -1039,13 +1042,13 @@ int collect(vo_engine* e, int n_done) {
         const int blocks = (int)std::min<size_t>(ch.size(), 4096);
         if (M == 4)
             hipLaunchKernelGGL(k_export_sets<4>, dim3(blocks), dim3(256), 0, e->s,
-                               reinterpret_cast<const ExportChunk*>(e->stage_d + ch_off), (int)ch.size(), e->a.tv,
+                               reinterpret_cast<const ExportChunk*>(e->stage_d + ch_off), (int)ch.size(), engine_ctx(e),
                                e->a.vmap, reinterpret_cast<const long long*>(e->stage_d + ob_off),
                                reinterpret_cast<const int*>(e->stage_d + ol_off),
                                reinterpret_cast<int*>(e->stage_d + oc_off), e->out_d);
         else
             hipLaunchKernelGGL(k_export_sets<3>, dim3(blocks), dim3(256), 0, e->s,
-                               reinterpret_cast<const ExportChunk*>(e->stage_d + ch_off), (int)ch.size(), e->a.tv,
+                               reinterpret_cast<const ExportChunk*>(e->stage_d + ch_off), (int)ch.size(), engine_ctx(e),
                                e->a.vmap, reinterpret_cast<const long long*>(e->stage_d + ob_off),
                                reinterpret_cast<const int*>(e->stage_d + ol_off),
                                reinterpret_cast<int*>(e->stage_d + oc_off), e->out_d);
@@ -1150,13 +1153,10 @@ int vo_engine_create(int hip_device, int dim, int64_t max_points, int64_t max_si
     if (hipEventCreate(&E->ev0) != hipSuccess || hipEventCreate(&E->ev1) != hipSuccess)
         return bail(fail(VO_ERR_HIP, "event"));
     const int S = E->S;
-    const int txs = dim == 3 ? 4 : 2;
     int rc;
     if ((rc = dev_alloc(&E->a.X64, (size_t)E->vcap * S)) || (rc = dev_alloc(&E->a.X32, (size_t)E->vcap * S)) ||
         (rc = dev_alloc(&E->a.cp, E->vcap)) || (rc = dev_alloc(&E->a.vmap, E->vcap)) ||
-        (rc = dev_alloc(&E->a.pset, E->vcap)) || (rc = dev_alloc(&E->a.tv, E->tcap)) ||
-        (rc = dev_alloc(&E->a.ta, E->tcap)) || (rc = dev_alloc(&E->a.tx, (size_t)E->tcap * txs)) ||
-        (rc = dev_alloc(&E->kres, E->tcap)) || (rc = dev_alloc(&E->nres, E->tcap)) ||
+        (rc = dev_alloc(&E->a.pset, E->vcap)) || (rc = dev_alloc(&E->a.rec, (size_t)E->tcap * record_bytes(dim))) ||
         (rc = dev_alloc(&E->a.hint, E->hcap)) || (rc = dev_alloc(&E->a.sets, E->nslots)) ||
         (rc = dev_alloc(&E->win, E->win_cap)) || (rc = dev_alloc(&E->pend[0], E->win_cap + E->big_cap)) ||
         (rc = dev_alloc(&E->pend[1], E->win_cap + E->big_cap)) ||
@@ -1177,8 +1177,8 @@ int vo_engine_create(int hip_device, int dim, int64_t max_points, int64_t max_si
         hipHostMalloc((void**)&E->ctl_host, sizeof(BatchCtl)) != hipSuccess)
         return bail(fail(VO_ERR_OOM, "pinned"));
     // reservations start at the largest key; no set record is active
-    hipLaunchKernelGGL(k_fill_u64, dim3(4096), dim3(256), 0, E->s, E->kres, (long long)E->tcap, ~0ull);
-    hipLaunchKernelGGL(k_fill_u64, dim3(4096), dim3(256), 0, E->s, E->nres, (long long)E->tcap, ~0ull);
+    hipLaunchKernelGGL(k_fill_u64, dim3(4096), dim3(256), 0, E->s, reinterpret_cast<unsigned long long*>(E->a.rec),
+                       (long long)E->tcap * record_bytes(dim) / 8, ~0ull);
     hipLaunchKernelGGL(k_fill_u64, dim3(1024), dim3(256), 0, E->s, reinterpret_cast<unsigned long long*>(E->cache),
                        (long long)(sizeof(CEntry) / 8) * ((long long)E->cmask + 1), ~0ull);
     hipLaunchKernelGGL(k_fill_i32, dim3(64), dim3(256), 0, E->s, reinterpret_cast<int*>(E->a.sets),

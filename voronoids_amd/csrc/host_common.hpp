@@ -4,6 +4,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdint>
 #include <cstring>
@@ -103,6 +104,26 @@ inline void first_simplex_order(int D, const double sv[4][3], int tet0[4]) {
     }
 }
 
+// simplex record bytes (the device layout of kernels.hpp: Rec<3> / Rec<2>)
+inline int record_bytes(int D) { return D == 3 ? 128 : 64; }
+
+// the first simplex's record: vertex ids v0 + tet0[k], hull adjacency, fp32 coordinates of the
+// super vertices translated by c, reservations at the largest key
+inline void root_record(int D, const int tet0[4], int v0, const double sv[4][3], const double* c, unsigned char* out) {
+    const int B = record_bytes(D), M = D + 1;
+    const int V = 0, A = D == 3 ? 16 : 12, X = D == 3 ? 32 : 24, K = D == 3 ? 96 : 48;
+    std::memset(out, 0, (size_t)B);
+    int* v = reinterpret_cast<int*>(out + V);
+    int* a = reinterpret_cast<int*>(out + A);
+    float* x = reinterpret_cast<float*>(out + X);
+    for (int k = 0; k < M; ++k) {
+        v[k] = v0 + tet0[k];
+        a[k] = -1;
+        for (int j = 0; j < D; ++j) x[(D == 3 ? 4 : 2) * k + j] = (float)(sv[tet0[k]][j] - c[j]);
+    }
+    std::memset(out + K, 0xFF, 16);
+}
+
 inline uint64_t splitmix_next(uint64_t& s) {
     uint64_t z = (s += 0x9E3779B97F4A7C15ull);
     z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
@@ -119,6 +140,46 @@ inline std::vector<int32_t> priority_order(int64_t n, uint64_t seed) {
     for (int64_t i = n - 1; i > 0; --i) {
         const uint64_t j = (uint64_t)(((unsigned __int128)splitmix_next(sm) * (uint64_t)(i + 1)) >> 64);
         std::swap(perm[i], perm[j]);
+    }
+    return perm;
+}
+
+// 3D / 2D Morton code of a point in the box [c - r, c + r]^D (21 / 32 bits per axis)
+inline uint64_t morton_code(int D, const double* x, const double* c, double r) {
+    const int bits = D == 3 ? 21 : 32;
+    const double scale = (double)((1ull << bits) - 1) / (2.0 * r);
+    uint64_t code = 0;
+    uint64_t q[3] = {0, 0, 0};
+    for (int k = 0; k < D; ++k) {
+        double t = (x[k] - (c[k] - r)) * scale;
+        t = t < 0.0 ? 0.0 : (t > (double)((1ull << bits) - 1) ? (double)((1ull << bits) - 1) : t);
+        q[k] = (uint64_t)t;
+    }
+    for (int b = bits - 1; b >= 0; --b)
+        for (int k = D - 1; k >= 0; --k) code = (code << 1) | ((q[k] >> b) & 1ull);
+    return code;
+}
+
+// Insertion priority with spatial coherence at fine scale: the seeded random order, cut into
+// consecutive chunks of max(1, position / divisor) points, each chunk sorted along the Morton
+// curve.  A round's window (about 1/20 of the points inserted so far) spans many whole chunks,
+// so it is still a uniform random sample of the remaining points (same conflicts, same
+// parallelism), while neighboring window slots -- the points of one wavefront -- lie close
+// together, and so do the simplices they create and later revisit.
+inline std::vector<int32_t> priority_order_local(int D, const double* pts, int64_t n, uint64_t seed, const double* c,
+                                                 double r, int divisor = 256) {
+    std::vector<int32_t> perm = priority_order(n, seed);
+    std::vector<std::pair<uint64_t, int32_t>> key;
+    for (int64_t s0 = 0; s0 < n;) {
+        const int64_t len = std::min<int64_t>(n - s0, std::max<int64_t>(1, s0 / divisor));
+        key.resize((size_t)len);
+        for (int64_t i = 0; i < len; ++i) {
+            const int32_t v = perm[s0 + i];
+            key[i] = {morton_code(D, pts + (size_t)v * D, c, r), v};
+        }
+        std::sort(key.begin(), key.end());
+        for (int64_t i = 0; i < len; ++i) perm[s0 + i] = key[i].second;
+        s0 += len;
     }
     return perm;
 }

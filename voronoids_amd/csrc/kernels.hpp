@@ -5,12 +5,14 @@
 // insert_point / insert_points_parallel :125-334 become k_commit; the static level schedule
 // of make_queue/find_placement (src/scheduler.rs:6-55) becomes per-round reservation.
 //
-// Device state (D = 2 or 3, M = D + 1 vertices per simplex):
-//   tv[t]   int4: vertex ids, positively oriented; dead: x = -1, y = forward simplex
-//   ta[t]   int4: (neighbor << 2) | face-in-neighbor across the face opposite vertex k, -1 on
-//           the super-simplex hull
-//   kres[t], nres[t]: 64-bit reservation keys ((2^32-1-round) << 32 | priority): the least
-//           key of the round's points that kill t / that have t as an outside neighbor
+// Device state (D = 2 or 3, M = D + 1 vertices per simplex).  Each simplex is one record of
+// 128 B (3D) / 64 B (2D), so that a visit touches one cache line and one page:
+//   v       vertex ids, positively oriented; dead: v[0] = -1, v[1] = forward simplex
+//   a       (neighbor << 2) | face-in-neighbor across the face opposite vertex k, -1 on the
+//           super-simplex hull
+//   x       fp32 vertex coordinates (translated by the bounding-sphere center), vertex order
+//   kres, nres: 64-bit reservation keys ((2^32-1-round) << 32 | priority): the least key of
+//           the round's points that kill the simplex / that have it as an outside neighbor
 //   X64     f64 coordinates, stride 4 (3D) or 2 (2D); vertex v = super (v < M) or point v-M
 //   X32     f32 coordinates translated by the bounding-sphere center, same strides
 //   cp[p]   conflict pointer of point p: a simplex containing it (or stale: dead -> forward)
@@ -82,11 +84,7 @@ struct BatchCtl {
 };
 
 struct Ctx {
-    int4* tv;
-    int4* ta;
-    float4* tx;            // per-simplex fp32 vertex coordinates (4 x float4 in 3D, 2 in 2D)
-    unsigned long long* kres;
-    unsigned long long* nres;
+    char* rec;             // simplex records (layout in Rec<D>)
     const double* X64;
     const float* X32;
     int* cp;
@@ -172,7 +170,56 @@ __device__ __forceinline__ int comp(int4 a, int k) {
 __device__ __forceinline__ void set_comp(int4& a, int k, int v) {
     if (k == 0) a.x = v; else if (k == 1) a.y = v; else if (k == 2) a.z = v; else a.w = v;
 }
-__device__ __forceinline__ int* comp_ptr(int4* a, int k) { return reinterpret_cast<int*>(a) + k; }
+
+// simplex record layout: byte offsets of v (int x M), a (int x M), x (fp32 coordinates),
+// kres, nres; 3D: v int4 | a int4 | x 4 x float4 | kres | nres | 16 B pad;
+// 2D: v int3 | a int3 | x 3 x float2 | kres | nres
+template <int D> struct Rec;
+template <> struct Rec<3> { static constexpr int B = 128, V = 0, A = 16, X = 32, K = 96, N = 104; };
+template <> struct Rec<2> { static constexpr int B = 64, V = 0, A = 12, X = 24, K = 48, N = 56; };
+
+template <int D> __device__ __forceinline__ char* rp(const Ctx& c, int t) { return c.rec + (size_t)t * Rec<D>::B; }
+template <int D> __device__ __forceinline__ int* v_ptr(const Ctx& c, int t, int k) {
+    return reinterpret_cast<int*>(rp<D>(c, t) + Rec<D>::V) + k;
+}
+template <int D> __device__ __forceinline__ int* a_ptr(const Ctx& c, int t, int k) {
+    return reinterpret_cast<int*>(rp<D>(c, t) + Rec<D>::A) + k;
+}
+template <int D> __device__ __forceinline__ unsigned long long* kres_p(const Ctx& c, int t) {
+    return reinterpret_cast<unsigned long long*>(rp<D>(c, t) + Rec<D>::K);
+}
+template <int D> __device__ __forceinline__ unsigned long long* nres_p(const Ctx& c, int t) {
+    return reinterpret_cast<unsigned long long*>(rp<D>(c, t) + Rec<D>::N);
+}
+// vertex ids (2D: w = -1)
+template <int D> __device__ __forceinline__ int4 ld_v(const Ctx& c, int t) {
+    if constexpr (D == 3) return *reinterpret_cast<const int4*>(rp<D>(c, t) + Rec<D>::V);
+    else {
+        const int* q = v_ptr<D>(c, t, 0);
+        return make_int4(q[0], q[1], q[2], -1);
+    }
+}
+template <int D> __device__ __forceinline__ void st_v(const Ctx& c, int t, int4 v) {
+    if constexpr (D == 3) *reinterpret_cast<int4*>(rp<D>(c, t) + Rec<D>::V) = v;
+    else {
+        int* q = v_ptr<D>(c, t, 0);
+        q[0] = v.x; q[1] = v.y; q[2] = v.z;
+    }
+}
+// adjacency (2D: w = -1)
+template <int D> __device__ __forceinline__ int4 ld_a(const Ctx& c, int t) {
+    if constexpr (D == 3) return *reinterpret_cast<const int4*>(rp<D>(c, t) + Rec<D>::A);
+    else {
+        const int* q = a_ptr<D>(c, t, 0);
+        return make_int4(q[0], q[1], q[2], -1);
+    }
+}
+// a dead simplex keeps a forward pointer to a live one of its killer's star
+template <int D> __device__ __forceinline__ void kill(const Ctx& c, int t, int fwd) {
+    int* q = v_ptr<D>(c, t, 0);
+    q[0] = -1;
+    q[1] = fwd;
+}
 
 template <int D> __device__ __forceinline__ int index_of(int4 a, int v) {
     if (a.x == v) return 0;
@@ -262,27 +309,24 @@ __device__ __forceinline__ void enqueue(const Ctx& c, uint32_t type, const uint3
 // ---------------------------------------------------------------- predicates
 // Tier 1 reads the simplex's own coordinate record tx[t] (the fp32 coordinates of its
 // vertices, in vertex order); tiers 2 and 3 need vertex ids and load tv[t] only then.
-template <int D> constexpr int kTxStride = D == 3 ? 4 : 2;   // float4 per simplex record
 
 template <int D> __device__ __forceinline__ void load_tx(const Ctx& c, int t, typename VT<D>::F* a) {
-    const float4* q = c.tx + (size_t)kTxStride<D> * t;
     if constexpr (D == 3) {
+        const float4* q = reinterpret_cast<const float4*>(rp<D>(c, t) + Rec<D>::X);
         a[0] = q[0]; a[1] = q[1]; a[2] = q[2]; a[3] = q[3];
     } else {
-        const float4 q0 = q[0], q1 = q[1];
-        a[0] = make_float2(q0.x, q0.y);
-        a[1] = make_float2(q0.z, q0.w);
-        a[2] = make_float2(q1.x, q1.y);
+        const float2* q = reinterpret_cast<const float2*>(rp<D>(c, t) + Rec<D>::X);
+        a[0] = q[0]; a[1] = q[1]; a[2] = q[2];
     }
 }
 
 template <int D> __device__ __forceinline__ void store_tx(const Ctx& c, int t, const typename VT<D>::F* a) {
-    float4* q = c.tx + (size_t)kTxStride<D> * t;
     if constexpr (D == 3) {
+        float4* q = reinterpret_cast<float4*>(rp<D>(c, t) + Rec<D>::X);
         q[0] = a[0]; q[1] = a[1]; q[2] = a[2]; q[3] = a[3];
     } else {
-        q[0] = make_float4(a[0].x, a[0].y, a[1].x, a[1].y);
-        q[1] = make_float4(a[2].x, a[2].y, 0.0f, 0.0f);
+        float2* q = reinterpret_cast<float2*>(rp<D>(c, t) + Rec<D>::X);
+        q[0] = a[0]; q[1] = a[1]; q[2] = a[2];
     }
 }
 
@@ -338,7 +382,7 @@ template <int D> __device__ __forceinline__ int conflict_at(const Ctx& c, const 
         else s = t1_incircle(a[0], a[1], a[2], pf);
         if (s != SIGN_UNDECIDED) { lc.t1[1]++; return s; }
     }
-    return conflict_slow<D>(c, c.tv[t], pv, lc);
+    return conflict_slow<D>(c, ld_v<D>(c, t), pv, lc);
 }
 
 // orientation of simplex t (coordinates a[]) with vertex k replaced by p: < 0 means p lies
@@ -353,10 +397,27 @@ template <int D> __device__ __forceinline__ int orient_at(const Ctx& c, const ty
         else s = t1_orient2d(b[0], b[1], b[2]);
         if (s != SIGN_UNDECIDED) { lc.t1[0]++; return s; }
     }
-    const int4 v = c.tv[t];
+    const int4 v = ld_v<D>(c, t);
     int ids[4] = {v.x, v.y, v.z, v.w};
     ids[k] = pv;
     return orient_slow<D>(c, ids, lc);
+}
+
+// coordinate record of replace_first(v, i, p) from the old record ox: p first, then the old
+// vertices in the order replace_first puts them (selects only: a dynamically indexed private
+// array would live in scratch memory)
+template <int D>
+__device__ __forceinline__ void star_record(const typename VT<D>::F* ox, int i, typename VT<D>::F pf,
+                                            typename VT<D>::F* nx) {
+    nx[0] = pf;
+    if constexpr (D == 3) {
+        nx[1] = i == 0 ? ox[1] : ox[0];
+        nx[2] = i == 0 ? ox[2] : (i == 1 ? ox[3] : (i == 2 ? ox[1] : ox[2]));
+        nx[3] = i == 0 ? ox[3] : (i == 1 ? ox[2] : (i == 2 ? ox[3] : ox[1]));
+    } else {
+        nx[1] = i == 0 ? ox[1] : (i == 1 ? ox[2] : ox[0]);
+        nx[2] = i == 0 ? ox[2] : (i == 1 ? ox[0] : ox[1]);
+    }
 }
 
 // position in the old simplex of vertex k (k >= 1) of replace_first(v, i, p)
@@ -426,11 +487,11 @@ __global__ void __launch_bounds__(256) k_cavity(Ctx c, int nwin, unsigned long l
         int st = ST_OK;
         int t = c.cp[p];
         if (t < 0) t = hint_lookup<D>(hv, pf);
-        int4 tt = c.tv[t];
+        int4 tt = ld_v<D>(c, t);
         for (int hops = 0; tt.x < 0; ++hops) {
             if (hops > kChainLimit) { atomicOr(&c.st->err, ERR_CHAIN); st = ST_BROKEN; break; }
             t = tt.y;
-            tt = c.tv[t];
+            tt = ld_v<D>(c, t);
             lc.chain++;
         }
         // remembering stochastic visibility walk
@@ -440,7 +501,7 @@ __global__ void __launch_bounds__(256) k_cavity(Ctx c, int nwin, unsigned long l
         load_tx<D>(c, t, a);
         for (int step = 0; st == ST_OK; ++step) {
             if (step > kWalkLimit) { atomicOr(&c.st->err, ERR_WALK); st = ST_BROKEN; break; }
-            const int4 aa = c.ta[t];
+            const int4 aa = ld_a<D>(c, t);
             rng = rng * 1664525u + 1013904223u;
             const int r = (int)((rng >> 16) % M);
             int moved = 0;
@@ -475,18 +536,18 @@ __global__ void __launch_bounds__(256) k_cavity(Ctx c, int nwin, unsigned long l
         if (st == ST_OK) {
             c.cav[slot] = t;
             s_cav[0][lt] = t;
-            if (c.solo) c.kres[t] = skey;
+            if (c.solo) *kres_p<D>(c, t) = skey;
             nk = 1;
             for (int qi = 0; qi < nk && st == ST_OK; ++qi) {
                 const int ct = qi < KL ? s_cav[qi][lt] : c.cav[qi * W + slot];
-                const int4 aa = c.ta[ct];
+                const int4 aa = ld_a<D>(c, ct);
                 for (int f = 0; f < M; ++f) {
                     const int nb = comp(aa, f);
                     if (nb >= 0) {
                         const int n = nb >> 2;
                         bool seen = false;
                         if (c.solo) {
-                            seen = c.kres[n] == skey;
+                            seen = *kres_p<D>(c, n) == skey;
                         } else {
                             const int nl = nk < KL ? nk : KL;
                             for (int j = 0; j < nl && !seen; ++j) seen = s_cav[j][lt] == n;
@@ -501,7 +562,7 @@ __global__ void __launch_bounds__(256) k_cavity(Ctx c, int nwin, unsigned long l
                             if (nk == c.kcap) { st = ST_OVERFLOW; break; }
                             c.cav[nk * W + slot] = n;
                             if (nk < KL) s_cav[nk][lt] = n;
-                            if (c.solo) c.kres[n] = skey;
+                            if (c.solo) *kres_p<D>(c, n) = skey;
                             nk++;
                             continue;
                         }
@@ -514,11 +575,11 @@ __global__ void __launch_bounds__(256) k_cavity(Ctx c, int nwin, unsigned long l
         }
         if (st == ST_OK) {
             const unsigned long long key = key_hi | (unsigned)p;
-            for (int j = 0; j < nk; ++j) atomicMin(&c.kres[c.cav[j * W + slot]], key);
+            for (int j = 0; j < nk; ++j) atomicMin(kres_p<D>(c, c.cav[j * W + slot]), key);
             for (int j = 0; j < nf; ++j) {
                 const int fe = c.fac[j * W + slot];
-                const int nb = comp(c.ta[fe >> 2], fe & 3);
-                if (nb >= 0) atomicMin(&c.nres[nb >> 2], key);
+                const int nb = *a_ptr<D>(c, fe >> 2, fe & 3);
+                if (nb >= 0) atomicMin(nres_p<D>(c, nb >> 2), key);
             }
         }
         // a window point whose cavity outgrows the lists is re-run alone; alone, it is an error
@@ -562,12 +623,12 @@ __global__ void __launch_bounds__(256) k_commit(Ctx c, int nwin, unsigned long l
             win = true;
             for (int j = 0; j < nk && win; ++j) {
                 const int t = c.cav[j * W + slot];
-                win = c.kres[t] == key && c.nres[t] > key;
+                win = *kres_p<D>(c, t) == key && *nres_p<D>(c, t) > key;
             }
             for (int j = 0; j < nf && win; ++j) {
                 const int fe = c.fac[j * W + slot];
-                const int nb = comp(c.ta[fe >> 2], fe & 3);
-                if (nb >= 0) win = c.kres[nb >> 2] > key;
+                const int nb = *a_ptr<D>(c, fe >> 2, fe & 3);
+                if (nb >= 0) win = *kres_p<D>(c, nb >> 2) > key;
             }
         }
     }
@@ -604,11 +665,11 @@ __global__ void __launch_bounds__(256) k_commit(Ctx c, int nwin, unsigned long l
         for (int j = 0; j < nf; ++j) {
             const int fe = c.fac[j * W + slot];
             const int t = fe >> 2, i = fe & 3;
-            int* slot_ptr = comp_ptr(&c.ta[t], i);
+            int* slot_ptr = a_ptr<D>(c, t, i);
             const int outside = *slot_ptr;
             const int N = base + j;
-            *comp_ptr(&c.ta[N], 0) = outside;
-            if (outside >= 0) *comp_ptr(&c.ta[outside >> 2], outside & 3) = N << 2;
+            *a_ptr<D>(c, N, 0) = outside;
+            if (outside >= 0) *a_ptr<D>(c, outside >> 2, outside & 3) = N << 2;
             *slot_ptr = -2 - N;
         }
         // phase 2: internal faces.  Face k (k >= 1) of the new simplex contains p and the ridge
@@ -617,8 +678,8 @@ __global__ void __launch_bounds__(256) k_commit(Ctx c, int nwin, unsigned long l
         for (int j = 0; j < nf; ++j) {
             const int fe = c.fac[j * W + slot];
             const int t = fe >> 2, i = fe & 3;
-            const int4 tvt = c.tv[t];
-            const int4 tat = c.ta[t];
+            const int4 tvt = ld_v<D>(c, t);
+            const int4 tat = ld_a<D>(c, t);
             const int4 nv = replace_first<D>(tvt, i, pv);
             const int N = base + j;
             for (int k = 1; k < M; ++k) {
@@ -630,12 +691,12 @@ __global__ void __launch_bounds__(256) k_commit(Ctx c, int nwin, unsigned long l
                 for (int guard = 0; nbe >= 0; ++guard) {
                     if (guard > 4096) { atomicOr(&c.st->err, ERR_WALK); break; }
                     const int T2 = nbe >> 2;
-                    const int4 TV2 = c.tv[T2];
+                    const int4 TV2 = ld_v<D>(c, T2);
                     const int w = comp(TV2, nbe & 3);
                     e = index_of<D>(TV2, y);
                     y = w;
                     TV = TV2;
-                    TA = c.ta[T2];
+                    TA = ld_a<D>(c, T2);
                     nbe = comp(TA, e);
                 }
                 if (nbe > -2) { atomicOr(&c.st->err, ERR_WALK); continue; }
@@ -649,19 +710,17 @@ __global__ void __launch_bounds__(256) k_commit(Ctx c, int nwin, unsigned long l
                         if (r != k && comp(nv, r) == x) in_ridge = true;
                     if (!in_ridge) { kk = m; break; }
                 }
-                *comp_ptr(&c.ta[N], k) = (N2 << 2) | kk;
+                *a_ptr<D>(c, N, k) = (N2 << 2) | kk;
             }
-            if (D == 2) *comp_ptr(&c.ta[N], 3) = -1;
-            c.tv[N] = nv;
+            st_v<D>(c, N, nv);
             typename VT<D>::F ox[4], nx[4];
             load_tx<D>(c, t, ox);
-            nx[0] = pf;
-            for (int k = 1; k < M; ++k) nx[k] = ox[first_src<D>(i, k)];
+            star_record<D>(ox, i, pf, nx);
             store_tx<D>(c, N, nx);
         }
         // phase 3: killed simplices die with a forward pointer into the star
-        for (int j = 0; j < nk; ++j) c.tv[c.cav[j * W + slot]] = make_int4(-1, base, 0, 0);
-        for (int j = 0; j < nf; ++j) c.tv[c.fac[j * W + slot] >> 2].y = base + j;
+        for (int j = 0; j < nk; ++j) kill<D>(c, c.cav[j * W + slot], base);
+        for (int j = 0; j < nf; ++j) *v_ptr<D>(c, c.fac[j * W + slot] >> 2, 1) = base + j;
         lc.killed += nk;
         lc.created += nf;
         lc.wins += 1;
@@ -742,15 +801,14 @@ __global__ void __launch_bounds__(64) k_exact(const QEntry* q, int nq, const dou
 // ---------------------------------------------------------------- export / check
 // compacted export: vertex ids mapped to the reference id space, neighbors to export indices
 template <int D>
-__global__ void k_export(const int4* tv, const int4* ta, const int* pos, int n, const int* vmap, int32_t* simp,
-                         int32_t* nbr) {
+__global__ void k_export(Ctx c, const int* pos, int n, const int* vmap, int32_t* simp, int32_t* nbr) {
     constexpr int M = D + 1;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    const int4 v = tv[i];
+    const int4 v = ld_v<D>(c, i);
     if (v.x < 0) return;
     const int o = pos[i];
-    const int4 a = ta[i];
+    const int4 a = ld_a<D>(c, i);
     for (int k = 0; k < M; ++k) {
         if (simp) simp[(size_t)o * M + k] = vmap[comp(v, k)];
         if (nbr) {
@@ -769,9 +827,9 @@ __global__ void __launch_bounds__(256) k_check(Ctx c, int n, unsigned long long*
     LocalCnt lc = {};
     unsigned nbad = 0, nun = 0;
     if (t < n) {
-        const int4 v = c.tv[t];
+        const int4 v = ld_v<D>(c, t);
         if (v.x >= 0) {
-            const int4 a = c.ta[t];
+            const int4 a = ld_a<D>(c, t);
             typename V::F x[4], x2[4];
             load_tx<D>(c, t, x);
             bool coords_ok = true;                   // the record matches the vertex table
@@ -787,8 +845,8 @@ __global__ void __launch_bounds__(256) k_check(Ctx c, int n, unsigned long long*
                 const int nb = comp(a, f);
                 if (nb < 0) continue;
                 const int n2 = nb >> 2, f2 = nb & 3;
-                const int4 v2 = c.tv[n2];
-                if (v2.x < 0 || comp(c.ta[n2], f2) != ((t << 2) | f)) { nbad++; continue; }
+                const int4 v2 = ld_v<D>(c, n2);
+                if (v2.x < 0 || *a_ptr<D>(c, n2, f2) != ((t << 2) | f)) { nbad++; continue; }
                 const int w = comp(v2, f2);
                 load_tx<D>(c, n2, x2);
                 const int cs = conflict_at<D>(c, x, t, w, x2[f2], lc);

@@ -34,15 +34,24 @@ template <int G> struct Group {
     __device__ __forceinline__ int bcast(int v, int k) const { return __shfl(v, gid * G + k, 64); }
 };
 
-template <int D> constexpr int kGroupKC = D == 3 ? 192 : 64;    // cavity entries per point
-template <int D> constexpr int kGroupFC = D == 3 ? 256 : 96;    // boundary facets per point
+// LDS list capacities per point (a point whose cavity outgrows them is re-run alone) and the
+// least waves per SIMD the cavity kernel is compiled for; uniform 3D data at n = 4M peaks at
+// 116 killed / 106 facets
+#ifndef VO_GROUP_KC3
+#define VO_GROUP_KC3 192
+#endif
+#ifndef VO_GROUP_FC3
+#define VO_GROUP_FC3 256
+#endif
+#ifndef VO_CAVITY_WAVES
+#define VO_CAVITY_WAVES 1
+#endif
+template <int D> constexpr int kGroupKC = D == 3 ? VO_GROUP_KC3 : 64;    // cavity entries per point
+template <int D> constexpr int kGroupFC = D == 3 ? VO_GROUP_FC3 : 96;    // boundary facets per point
 
-__device__ __forceinline__ int ta_get(const int4* ta, int t, int f) {
-    return reinterpret_cast<const int*>(ta)[(size_t)t * 4 + f];
-}
 
 template <int D, int G>
-__global__ void __launch_bounds__(64) k_cavity_g(Ctx c, int nwin, unsigned long long key_hi) {
+__global__ void __launch_bounds__(64, VO_CAVITY_WAVES) k_cavity_g(Ctx c, int nwin, unsigned long long key_hi) {
     using V = VT<D>;
     using F = typename V::F;
     constexpr int M = D + 1;
@@ -50,7 +59,6 @@ __global__ void __launch_bounds__(64) k_cavity_g(Ctx c, int nwin, unsigned long 
     constexpr int KC = kGroupKC<D>;
     constexpr int FC = kGroupFC<D>;
     __shared__ int s_cav[NG][KC];
-    __shared__ int s_fac[NG][FC];
     __shared__ int s_tmp[64];
     const Group<G> g;
     const int W = c.win_cap;
@@ -71,11 +79,11 @@ __global__ void __launch_bounds__(64) k_cavity_g(Ctx c, int nwin, unsigned long 
         pf = V::ld(c.X32, pv);
         t = c.cp[p];
         if (t < 0) t = hint_lookup<D>(hint_view(c, p), pf);
-        tt = c.tv[t];
+        tt = ld_v<D>(c, t);
         for (int hops = 0; tt.x < 0; ++hops) {
             if (hops > kChainLimit) { if (lead) atomicOr(&c.st->err, ERR_CHAIN); st = ST_BROKEN; break; }
             t = tt.y;
-            tt = c.tv[t];
+            tt = ld_v<D>(c, t);
             if (lead) lc.chain++;
         }
     }
@@ -89,7 +97,7 @@ __global__ void __launch_bounds__(64) k_cavity_g(Ctx c, int nwin, unsigned long 
         int s = 1, nb = -1;
         bool test = false;
         if (walking) {
-            const int4 aa = c.ta[t];
+            const int4 aa = ld_a<D>(c, t);
             rng = rng * 1664525u + 1013904223u;
             const int r = (int)((rng >> 16) % M);
             int k = r + g.gl;
@@ -154,7 +162,7 @@ __global__ void __launch_bounds__(64) k_cavity_g(Ctx c, int nwin, unsigned long 
         if (valid) {
             tet = s_cav[g.gid][lo + i / M];
             face = i % M;
-            const int nb = ta_get(c.ta, tet, face);
+            const int nb = *a_ptr<D>(c, tet, face);
             hull = nb < 0;
             if (!hull) {
                 n = nb >> 2;
@@ -188,8 +196,10 @@ __global__ void __launch_bounds__(64) k_cavity_g(Ctx c, int nwin, unsigned long 
             } else {
                 if (isnew) s_cav[g.gid][nk + g.below(newm)] = n;
                 if (isfac) {
-                    s_fac[g.gid][nf + g.below(facm)] = (tet << 2) | face;
-                    if (!hull) atomicMin(&c.nres[n], key);
+                    // facets go straight to the point's global list: only the killed list,
+                    // which every candidate is checked against, occupies LDS
+                    c.fac[(nf + g.below(facm)) * W + slot] = (tet << 2) | face;
+                    if (!hull) atomicMin(nres_p<D>(c, n), key);
                 }
                 nk += cnew;
                 nf += cfac;
@@ -207,10 +217,9 @@ __global__ void __launch_bounds__(64) k_cavity_g(Ctx c, int nwin, unsigned long 
     if (st == ST_OK) {
         for (int j = g.gl; j < nk; j += G) {
             const int x = s_cav[g.gid][j];
-            atomicMin(&c.kres[x], key);
+            atomicMin(kres_p<D>(c, x), key);
             c.cav[j * W + slot] = x;
         }
-        for (int j = g.gl; j < nf; j += G) c.fac[j * W + slot] = s_fac[g.gid][j];
     }
     if (slot < nwin && lead) {
         c.nk[slot] = nk;
@@ -247,13 +256,13 @@ __global__ void __launch_bounds__(64) k_commit_g(Ctx c, int nwin, unsigned long 
             nf = c.nf[slot];
             for (int j = g.gl; j < nk && ok; j += G) {
                 const int x = c.cav[j * W + slot];
-                ok = c.kres[x] == key && c.nres[x] > key;
+                ok = *kres_p<D>(c, x) == key && *nres_p<D>(c, x) > key;
             }
             for (int j = g.gl; j < nf && ok; j += G) {
                 const int fe = c.fac[j * W + slot];
                 s_fac[g.gid][j] = fe;
-                const int nb = ta_get(c.ta, fe >> 2, fe & 3);
-                if (nb >= 0) ok = c.kres[nb >> 2] > key;
+                const int nb = *a_ptr<D>(c, fe >> 2, fe & 3);
+                if (nb >= 0) ok = *kres_p<D>(c, nb >> 2) > key;
             }
         }
     }
@@ -297,25 +306,23 @@ __global__ void __launch_bounds__(64) k_commit_g(Ctx c, int nwin, unsigned long 
         for (int j = g.gl; j < nf; j += G) {
             const int fe = s_fac[g.gid][j];
             const int t = fe >> 2, i = fe & 3;
-            const int4 nv = replace_first<D>(c.tv[t], i, pv);
-            const int outside = ta_get(c.ta, t, i);
+            const int4 nv = replace_first<D>(ld_v<D>(c, t), i, pv);
+            const int outside = *a_ptr<D>(c, t, i);
             const int N = base + j;
-            c.tv[N] = nv;
+            st_v<D>(c, N, nv);
             typename VT<D>::F ox[4], nx[4];
             load_tx<D>(c, t, ox);
-            nx[0] = pf;
-            for (int k = 1; k < M; ++k) nx[k] = ox[first_src<D>(i, k)];
+            star_record<D>(ox, i, pf, nx);
             store_tx<D>(c, N, nx);
-            *comp_ptr(&c.ta[N], 0) = outside;
-            if (D == 2) *comp_ptr(&c.ta[N], 3) = -1;
-            if (outside >= 0) *comp_ptr(&c.ta[outside >> 2], outside & 3) = N << 2;
+            *a_ptr<D>(c, N, 0) = outside;
+            if (outside >= 0) *a_ptr<D>(c, outside >> 2, outside & 3) = N << 2;
         }
         // phase 2: internal faces, one lane per (facet, ridge)
         for (int q = g.gl; q < nf * D; q += G) {
             const int j = q / D, k = 1 + q % D;
             const int fe = s_fac[g.gid][j];
             const int t = fe >> 2, i = fe & 3;
-            const int4 tvt = c.tv[t];
+            const int4 tvt = ld_v<D>(c, t);
             const int4 nv = replace_first<D>(tvt, i, pv);
             const int cv = comp(nv, k);
             int T = t;
@@ -323,11 +330,11 @@ __global__ void __launch_bounds__(64) k_commit_g(Ctx c, int nwin, unsigned long 
             int e = index_of<D>(TV, cv);
             int y = comp(tvt, i);
             for (int guard = 0;; ++guard) {
-                const int nbe = ta_get(c.ta, T, e);
-                if (nbe < 0 || c.kres[nbe >> 2] != key) break;
+                const int nbe = *a_ptr<D>(c, T, e);
+                if (nbe < 0 || *kres_p<D>(c, nbe >> 2) != key) break;
                 if (guard > 4096) { atomicOr(&c.st->err, ERR_WALK); break; }
                 const int T2 = nbe >> 2;
-                const int4 TV2 = c.tv[T2];
+                const int4 TV2 = ld_v<D>(c, T2);
                 const int w = comp(TV2, nbe & 3);
                 e = index_of<D>(TV2, y);
                 y = w;
@@ -348,7 +355,7 @@ __global__ void __launch_bounds__(64) k_commit_g(Ctx c, int nwin, unsigned long 
                 if (!in_ridge) { kk = m; break; }
             }
             if (jj < 0) atomicOr(&c.st->err, ERR_WALK);
-            *comp_ptr(&c.ta[base + j], k) = jj < 0 ? -1 : (((base + jj) << 2) | kk);
+            *a_ptr<D>(c, base + j, k) = jj < 0 ? -1 : (((base + jj) << 2) | kk);
         }
         // phase 3: killed simplices die, forwarding to a new simplex on one of their facets
         for (int j = g.gl; j < nk; j += G) {
@@ -356,7 +363,7 @@ __global__ void __launch_bounds__(64) k_commit_g(Ctx c, int nwin, unsigned long 
             int fwd = base;
             for (int r = 0; r < nf; ++r)
                 if ((s_fac[g.gid][r] >> 2) == t) { fwd = base + r; break; }
-            c.tv[t] = make_int4(-1, fwd, 0, 0);
+            kill<D>(c, t, fwd);
         }
         if (lead) {
             lc.killed += nk;

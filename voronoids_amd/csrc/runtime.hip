@@ -83,11 +83,7 @@ struct vo_tri {
     int* cp = nullptr;
     int* vmap = nullptr;               // vertex id -> reference id
     // simplices
-    int4* tv = nullptr;
-    int4* ta = nullptr;
-    float4* tx = nullptr;              // per-simplex fp32 vertex coordinates
-    unsigned long long* kres = nullptr;
-    unsigned long long* nres = nullptr;
+    char* rec = nullptr;               // simplex records (128 B in 3D, 64 B in 2D)
     int64_t tcap = 0;
     int n_tets = 1;
     int64_t n_live = 1;
@@ -125,6 +121,7 @@ struct vo_tri {
     hipEvent_t ev[6] = {};
     double gamma = 0.25;
     int filters = 3;
+    int local_order = 0;               // priority order: random, Morton-sorted in chunks of position/this (0: off)
     int group = 0;                     // lanes per window point (0 = chosen per round, 1 = one thread)
     int64_t lanes_target = 0;          // auto group size: largest G with window * G <= this (0 = default)
     int64_t inserted = 0, dups = 0, rounds = 0, windows_total = 0;
@@ -137,11 +134,7 @@ namespace {
 
 Ctx make_ctx(vo_tri* t) {
     Ctx c;
-    c.tv = t->tv;
-    c.ta = t->ta;
-    c.tx = t->tx;
-    c.kres = t->kres;
-    c.nres = t->nres;
+    c.rec = t->rec;
     c.X64 = t->X64;
     c.X32 = t->X32;
     c.cp = t->cp;
@@ -185,9 +178,10 @@ __global__ void k_window(int* win, const int* pend, int npend, int head, int nfr
     else if (i < npend + nfresh) win[i] = head + (i - npend);
 }
 
-__global__ void k_live_flags(const int4* tv, int n, int* flag) {
+template <int D>
+__global__ void k_live_flags(Ctx c, int n, int* flag) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) flag[i] = tv[i].x >= 0;
+    if (i < n) flag[i] = *v_ptr<D>(c, i, 0) >= 0;
 }
 
 // exclusive scan of flags within each 256-block (block sums out), then add block offsets
@@ -211,10 +205,6 @@ __global__ void k_add_offsets(int* out, const int* boff, int n) {
     if (i < n) out[i] += boff[blockIdx.x];
 }
 
-__global__ void k_init_first(int4* tv, int4* ta, int a, int b, int c, int d) {
-    tv[0] = make_int4(a, b, c, d);
-    ta[0] = make_int4(-1, -1, -1, -1);
-}
 
 __global__ void k_reset_round(RoundState* st) {
     st->n_next_pending = 0;
@@ -226,30 +216,23 @@ __global__ void k_reset_round(RoundState* st) {
 
 
 
-// grow the simplex arrays to cap (copying the first n), new reservations = max key
+// grow the simplex records to cap (copying the first n_tets); new records are all 0xFF bytes:
+// dead, hull adjacency, reservations at the largest key
 int grow_tets(vo_tri* t, int64_t cap) {
     if (cap <= t->tcap) return 0;
     if (cap > (int64_t)1 << 29) return fail(VO_ERR_OOM, "simplex capacity above 2^29 (adjacency packs index << 2)");
-    int4 *tv, *ta;
-    float4* tx;
-    unsigned long long *kr, *nr;
+    const size_t B = (size_t)record_bytes(t->D);
+    char* rec;
     int rc;
-    const int txs = t->D == 3 ? 4 : 2;
-    if ((rc = dev_alloc(&tv, cap)) || (rc = dev_alloc(&ta, cap)) || (rc = dev_alloc(&kr, cap)) ||
-        (rc = dev_alloc(&nr, cap)) || (rc = dev_alloc(&tx, (size_t)cap * txs)))
-        return rc;
-    VO_HIP(hipMemsetAsync(kr, 0xFF, sizeof(unsigned long long) * cap, t->s));
-    VO_HIP(hipMemsetAsync(nr, 0xFF, sizeof(unsigned long long) * cap, t->s));
-    if (t->tv) {
-        VO_HIP(hipMemcpyAsync(tv, t->tv, sizeof(int4) * t->n_tets, hipMemcpyDeviceToDevice, t->s));
-        VO_HIP(hipMemcpyAsync(ta, t->ta, sizeof(int4) * t->n_tets, hipMemcpyDeviceToDevice, t->s));
-        VO_HIP(hipMemcpyAsync(kr, t->kres, sizeof(unsigned long long) * t->n_tets, hipMemcpyDeviceToDevice, t->s));
-        VO_HIP(hipMemcpyAsync(nr, t->nres, sizeof(unsigned long long) * t->n_tets, hipMemcpyDeviceToDevice, t->s));
-        VO_HIP(hipMemcpyAsync(tx, t->tx, sizeof(float4) * txs * t->n_tets, hipMemcpyDeviceToDevice, t->s));
+    if ((rc = dev_alloc(&rec, (size_t)cap * B))) return rc;
+    const size_t keep = t->rec ? (size_t)t->n_tets * B : 0;
+    VO_HIP(hipMemsetAsync(rec + keep, 0xFF, (size_t)cap * B - keep, t->s));
+    if (t->rec) {
+        VO_HIP(hipMemcpyAsync(rec, t->rec, keep, hipMemcpyDeviceToDevice, t->s));
         VO_HIP(hipStreamSynchronize(t->s));
-        hipFree(t->tv); hipFree(t->ta); hipFree(t->kres); hipFree(t->nres); hipFree(t->tx);
+        hipFree(t->rec);
     }
-    t->tv = tv; t->ta = ta; t->kres = kr; t->nres = nr; t->tx = tx;
+    t->rec = rec;
     t->tcap = cap;
     return 0;
 }
@@ -298,7 +281,7 @@ int upload_super(vo_tri* t) {
 }
 
 void free_all(vo_tri* t) {
-    void* ptrs[] = {t->X64, t->X32, t->cp, t->vmap, t->tv, t->ta, t->tx, t->kres, t->nres, t->win, t->pend[0], t->pend[1],
+    void* ptrs[] = {t->X64, t->X32, t->cp, t->vmap, t->rec, t->win, t->pend[0], t->pend[1],
                     t->cav, t->fac, t->nk, t->nf, t->status, t->hint, t->st, t->cnt, t->q, t->cache, t->ws,
                     t->big, t->cav1, t->fac1, t->misc1};
     for (void* p : ptrs)
@@ -511,19 +494,10 @@ int vo_tri_create(int hip_device, int dim, const double* pts, int64_t n, vo_tri*
     if ((rc = grow_points(t, std::max<int64_t>(n, 16)))) return bail(rc);
     if ((rc = upload_super(t))) return bail(rc);
     if ((rc = grow_tets(t, 1024))) return bail(rc);
-    hipLaunchKernelGGL(k_init_first, dim3(1), dim3(1), 0, t->s, t->tv, t->ta, t->tet0[0], t->tet0[1], t->tet0[2],
-                       t->tet0[3]);
     {
-        float rec[16] = {};
-        for (int k = 0; k < t->M; ++k) {
-            const int v = t->tet0[k];
-            for (int j = 0; j < dim; ++j) {
-                const float x = (float)(t->sv[v][j] - t->c[j]);
-                if (dim == 3) rec[4 * k + j] = x;
-                else rec[2 * k + j] = x;
-            }
-        }
-        if (hipMemcpyAsync(t->tx, rec, sizeof(float) * (dim == 3 ? 16 : 8), hipMemcpyHostToDevice, t->s) != hipSuccess)
+        unsigned char r0[128];
+        root_record(dim, t->tet0, 0, t->sv, t->c, r0);
+        if (hipMemcpyAsync(t->rec, r0, (size_t)record_bytes(dim), hipMemcpyHostToDevice, t->s) != hipSuccess)
             return bail(fail(VO_ERR_HIP, "init record"));
     }
     RoundState init = {};
@@ -552,7 +526,8 @@ int vo_tri_stage(vo_tri* t, const double* pts, int64_t n, uint64_t seed) {
     VO_HIP(hipSetDevice(t->dev));
     int rc;
     if ((rc = grow_points(t, t->n_pts + n))) return rc;
-    const std::vector<int32_t> perm = priority_order(n, seed);
+    const std::vector<int32_t> perm =
+        t->local_order ? priority_order_local(D, pts, n, seed, t->c, t->r, t->local_order) : priority_order(n, seed);
     std::vector<double> x64((size_t)n * S, 0.0);
     std::vector<float> x32((size_t)n * S, 0.0f);
     std::vector<int> vm(n);
@@ -653,9 +628,13 @@ int vo_tri_run(vo_tri* t) {
         t->n_live += r.n_live_delta;
         t->inserted += r.n_winners;
         t->windows_total += nwin;
-        if (trace)
-            std::fprintf(trace, "%lld %d %d %d %d %.4f %.4f %d %lld %d %d\n", (long long)t->rounds, nwin, t->npend,
-                         nfresh, r.n_winners, a, b, r.n_q, (long long)t->inserted, n_big, r.n_next_pending);
+        if (trace) {
+            Counters h;
+            hipMemcpy(&h, t->cnt, sizeof(h), hipMemcpyDeviceToHost);
+            std::fprintf(trace, "%lld %d %d %d %d %.4f %.4f %d %lld %d %d %llu %llu %llu %llu\n", (long long)t->rounds,
+                         nwin, t->npend, nfresh, r.n_winners, a, b, r.n_q, (long long)t->inserted, n_big,
+                         r.n_next_pending, h.walk_steps, h.chain_hops, h.killed, h.created);
+        }
         const int64_t consumed = (int64_t)nwin - r.n_next_pending - r.n_winners;   // duplicates
         t->dups += consumed;
         const double frac = nwin ? (double)r.n_winners / nwin : 1.0;
@@ -699,7 +678,9 @@ static int export_device(const vo_tri* tc, int32_t** d_simp, int32_t** d_nbr, in
     int *flag, *pos, *bsum;
     int rc;
     if ((rc = dev_alloc(&flag, n)) || (rc = dev_alloc(&pos, n)) || (rc = dev_alloc(&bsum, nb))) return rc;
-    hipLaunchKernelGGL(k_live_flags, dim3(nb), dim3(256), 0, t->s, t->tv, n, flag);
+    const Ctx c = make_ctx(t);
+    if (t->D == 3) hipLaunchKernelGGL(k_live_flags<3>, dim3(nb), dim3(256), 0, t->s, c, n, flag);
+    else hipLaunchKernelGGL(k_live_flags<2>, dim3(nb), dim3(256), 0, t->s, c, n, flag);
     hipLaunchKernelGGL(k_block_scan, dim3(nb), dim3(256), 0, t->s, flag, pos, bsum, n);
     std::vector<int> hb(nb);
     VO_HIP(hipMemcpyAsync(hb.data(), bsum, sizeof(int) * nb, hipMemcpyDeviceToHost, t->s));
@@ -712,9 +693,9 @@ static int export_device(const vo_tri* tc, int32_t** d_simp, int32_t** d_nbr, in
     const int M = t->M;
     if ((rc = dev_alloc(&simp, acc * M)) || (rc = dev_alloc(&nbr, acc * M))) return rc;
     if (t->D == 3)
-        hipLaunchKernelGGL(k_export<3>, dim3(nb), dim3(256), 0, t->s, t->tv, t->ta, pos, n, t->vmap, simp, nbr);
+        hipLaunchKernelGGL(k_export<3>, dim3(nb), dim3(256), 0, t->s, c, pos, n, t->vmap, simp, nbr);
     else
-        hipLaunchKernelGGL(k_export<2>, dim3(nb), dim3(256), 0, t->s, t->tv, t->ta, pos, n, t->vmap, simp, nbr);
+        hipLaunchKernelGGL(k_export<2>, dim3(nb), dim3(256), 0, t->s, c, pos, n, t->vmap, simp, nbr);
     VO_HIP(hipGetLastError());
     VO_HIP(hipStreamSynchronize(t->s));
     hipFree(flag); hipFree(pos); hipFree(bsum);
@@ -840,6 +821,9 @@ int vo_tri_config(vo_tri* t, const char* key, int64_t value) {
     } else if (k == "window_gamma_milli") {
         if (value <= 0) return fail(VO_ERR_INVALID, "window_gamma_milli must be positive");
         t->gamma = (double)value / 1000.0;
+    } else if (k == "local_order") {
+        if (value < 0 || value > (1 << 20)) return fail(VO_ERR_INVALID, "local_order must be in 0..2^20");
+        t->local_order = (int)value;
     } else {
         return fail(VO_ERR_INVALID, "unknown option " + k);
     }
