@@ -50,6 +50,30 @@ template <int D> constexpr int kGroupKC = D == 3 ? VO_GROUP_KC3 : 64;    // cavi
 template <int D> constexpr int kGroupFC = D == 3 ? VO_GROUP_FC3 : 96;    // boundary facets per point
 
 
+// LDS list scans, four entries per ds_read_b128 (entries at or past n are ignored: they may
+// hold another point's values)
+__device__ __forceinline__ bool lds_contains(const int* list, int n, int x) {
+    const int4* q = reinterpret_cast<const int4*>(list);
+    for (int j = 0; j < n; j += 4) {
+        const int4 w = q[j >> 2];
+        if ((w.x == x) | ((j + 1 < n) & (w.y == x)) | ((j + 2 < n) & (w.z == x)) | ((j + 3 < n) & (w.w == x))) return true;
+    }
+    return false;
+}
+// index of the first entry equal to x (shift = 0) or whose value >> 2 equals x (shift = 2), -1
+template <int SHIFT>
+__device__ __forceinline__ int lds_find(const int* list, int n, int x) {
+    const int4* q = reinterpret_cast<const int4*>(list);
+    for (int j = 0; j < n; j += 4) {
+        const int4 w = q[j >> 2];
+        if ((w.x >> SHIFT) == x) return j;
+        if (j + 1 < n && (w.y >> SHIFT) == x) return j + 1;
+        if (j + 2 < n && (w.z >> SHIFT) == x) return j + 2;
+        if (j + 3 < n && (w.w >> SHIFT) == x) return j + 3;
+    }
+    return -1;
+}
+
 template <int D, int G>
 __global__ void __launch_bounds__(64, VO_CAVITY_WAVES) k_cavity_g(Ctx c, int nwin, unsigned long long key_hi) {
     using V = VT<D>;
@@ -58,8 +82,9 @@ __global__ void __launch_bounds__(64, VO_CAVITY_WAVES) k_cavity_g(Ctx c, int nwi
     constexpr int NG = Group<G>::NG;
     constexpr int KC = kGroupKC<D>;
     constexpr int FC = kGroupFC<D>;
-    __shared__ int s_cav[NG][KC];
-    __shared__ int s_tmp[64];
+    static_assert(KC % 4 == 0, "int4 scans");
+    __shared__ __align__(16) int s_cav[NG][KC];
+    __shared__ __align__(16) int s_tmp[64];
     const Group<G> g;
     const int W = c.win_cap;
     const bool lead = g.gl == 0;
@@ -166,9 +191,7 @@ __global__ void __launch_bounds__(64, VO_CAVITY_WAVES) k_cavity_g(Ctx c, int nwi
             hull = nb < 0;
             if (!hull) {
                 n = nb >> 2;
-                bool inC = false;
-                for (int j = 0; j < nk && !inC; ++j) inC = s_cav[g.gid][j] == n;
-                test = !inC;
+                test = !lds_contains(s_cav[g.gid], nk, n);
                 if (test) {
                     F an[4];
                     load_tx<D>(c, n, an);
@@ -180,9 +203,7 @@ __global__ void __launch_bounds__(64, VO_CAVITY_WAVES) k_cavity_g(Ctx c, int nwi
         bool isnew = test && s != SIGN_UNDECIDED && s > 0;
         s_tmp[g.lane] = isnew ? n : -1;
         __builtin_amdgcn_wave_barrier();
-        if (isnew)
-            for (int j = 0; j < g.gl; ++j)
-                if (s_tmp[g.gid * G + j] == n) { isnew = false; break; }
+        if (isnew && lds_contains(s_tmp + g.gid * G, g.gl, n)) isnew = false;
         const bool isfac = valid && (hull || (test && s != SIGN_UNDECIDED && s <= 0));
         const unsigned long long newm = g.ballot(isnew), facm = g.ballot(isfac);
         const int cnew = __popcll(newm), cfac = __popcll(facm);
@@ -236,7 +257,8 @@ __global__ void __launch_bounds__(64) k_commit_g(Ctx c, int nwin, unsigned long 
     constexpr int M = D + 1;
     constexpr int NG = Group<G>::NG;
     constexpr int FC = kGroupFC<D>;
-    __shared__ int s_fac[NG][FC];
+    static_assert(FC % 4 == 0, "int4 scans");
+    __shared__ __align__(16) int s_fac[NG][FC];
     const Group<G> g;
     const int W = c.win_cap;
     const bool lead = g.gl == 0;
@@ -342,9 +364,7 @@ __global__ void __launch_bounds__(64) k_commit_g(Ctx c, int nwin, unsigned long 
                 TV = TV2;
             }
             const int target = (T << 2) | e;
-            int jj = -1;
-            for (int r = 0; r < nf; ++r)
-                if (s_fac[g.gid][r] == target) { jj = r; break; }
+            const int jj = lds_find<0>(s_fac[g.gid], nf, target);
             const int4 nv2 = replace_first<D>(TV, e, pv);
             int kk = 1;
             for (int m = 1; m < M; ++m) {
@@ -360,9 +380,8 @@ __global__ void __launch_bounds__(64) k_commit_g(Ctx c, int nwin, unsigned long 
         // phase 3: killed simplices die, forwarding to a new simplex on one of their facets
         for (int j = g.gl; j < nk; j += G) {
             const int t = c.cav[j * W + slot];
-            int fwd = base;
-            for (int r = 0; r < nf; ++r)
-                if ((s_fac[g.gid][r] >> 2) == t) { fwd = base + r; break; }
+            const int r = lds_find<2>(s_fac[g.gid], nf, t);
+            const int fwd = base + (r < 0 ? 0 : r);
             kill<D>(c, t, fwd);
         }
         if (lead) {
