@@ -128,6 +128,7 @@ typedef struct {
     int64_t rounds;         /* engine rounds the set was resident for */
 } vo_set_result;
 
+/* engine statistics; points_inserted counts every set's inserted points as of the last replay */
 typedef struct {
     int64_t sets_submitted, sets_admitted, sets_done, sets_resident, sets_queued;
     int64_t points_inserted, rounds, replays, solo_runs, range_extensions, admission_waits;

@@ -469,6 +469,7 @@ struct vo_engine {
     RoundState* st_host = nullptr;
     BatchCtl* ctl = nullptr;
     BatchCtl* ctl_host = nullptr;
+    Counters* cnt_host = nullptr;          // device counters as of the last replay (pinned)
     Counters* cnt = nullptr;
     QEntry* q = nullptr;
     CEntry* cache = nullptr;
@@ -605,6 +606,7 @@ void engine_free(vo_engine* e) {
         if (p) hipFree(p);
     if (e->st_host) hipHostFree(e->st_host);
     if (e->ctl_host) hipHostFree(e->ctl_host);
+    if (e->cnt_host) hipHostFree(e->cnt_host);
     if (e->stage_h) hipHostFree(e->stage_h);
     if (e->out_h) hipHostFree(e->out_h);
     for (auto& g : e->graph)
@@ -1100,7 +1102,6 @@ int collect(vo_engine* e, int n_done) {
             e->free_slots.push_back(d.slot);
             e->n_resident--;
             e->stats.sets_done++;
-            e->stats.points_inserted += d.inserted;
         }
         e->cache_stale = true;
         k0 = k1;
@@ -1174,7 +1175,8 @@ int vo_engine_create(int hip_device, int dim, int64_t max_points, int64_t max_si
     if (hipHostMalloc((void**)&E->out_h, sizeof(int32_t) * E->out_cap) != hipSuccess)
         return bail(fail(VO_ERR_OOM, "pinned export buffer"));
     if (hipHostMalloc((void**)&E->st_host, sizeof(RoundState)) != hipSuccess ||
-        hipHostMalloc((void**)&E->ctl_host, sizeof(BatchCtl)) != hipSuccess)
+        hipHostMalloc((void**)&E->ctl_host, sizeof(BatchCtl)) != hipSuccess ||
+        hipHostMalloc((void**)&E->cnt_host, sizeof(Counters)) != hipSuccess)
         return bail(fail(VO_ERR_OOM, "pinned"));
     // reservations start at the largest key; no set record is active
     hipLaunchKernelGGL(k_fill_u64, dim3(4096), dim3(256), 0, E->s, reinterpret_cast<unsigned long long*>(E->a.rec),
@@ -1295,6 +1297,7 @@ int vo_engine_launch(vo_engine* e) {
     VO_HIP(hipEventRecord(e->ev1, e->s));
     VO_HIP(hipMemcpyAsync(e->st_host, e->st, sizeof(RoundState), hipMemcpyDeviceToHost, e->s));
     VO_HIP(hipMemcpyAsync(e->ctl_host, e->ctl, sizeof(BatchCtl), hipMemcpyDeviceToHost, e->s));
+    VO_HIP(hipMemcpyAsync(e->cnt_host, e->cnt, sizeof(Counters), hipMemcpyDeviceToHost, e->s));
     e->launched = true;
     return 0;
 }
@@ -1313,6 +1316,8 @@ int vo_engine_finish(vo_engine* e, int64_t* n_left) {
         e->stats.replays++;
         e->stats.rounds += e->rounds_per_replay;
         e->last_nwin = e->ctl_host->nwin;
+        // points inserted so far, all sets (device count): the progress measure of a replay
+        e->stats.points_inserted = (int64_t)e->cnt_host->winners;
         if ((rc = check_err(e))) return rc;
         if (e->st_host->n_big > 0 && (rc = run_solo(e))) return rc;
         if (e->st_host->n_nospace > 0 && (rc = extend_ranges(e))) return rc;

@@ -298,6 +298,85 @@ __device__ __forceinline__ int wave_excl_scan(int v, int& total) {
     return x - v;
 }
 
+// Simplex slots for this lane's request of n (req false: none): the base slot, or -1 when
+// there is no request or no room (one set: ERR_TETCAP; several sets: the set is flagged for
+// another range and the point retries).  One atomicAdd per wavefront when all requests come
+// from one set (always, with one set), else one per request.  Every lane of the wavefront
+// must call this.
+__device__ inline int alloc_slots(const Ctx& c, bool req, int n, int p) {
+    if (!c.sets) {
+        int total = 0;
+        const int off = wave_excl_scan(req ? n : 0, total);
+        int base0 = 0;
+        if (__lane_id() == 0 && total > 0) base0 = atomicAdd(&c.st->n_tets, total);
+        base0 = __shfl(base0, 0, 64);
+        if (!req) return -1;
+        if (base0 + off + n > c.tet_cap) {
+            atomicOr(&c.st->err, ERR_TETCAP);
+            return -1;
+        }
+        return base0 + off;
+    }
+    const int s = req ? c.pset[p] : -1;
+    const unsigned long long rm = __ballot(req);
+    if (!rm) return -1;
+    const int lead = __ffsll((long long)rm) - 1;
+    const int s0 = __shfl(s, lead, 64);
+    int base = -1, set = s;
+    if (__ballot(req && s != s0) == 0) {
+        int total = 0;
+        const int off = wave_excl_scan(req ? n : 0, total);
+        int base0 = 0;
+        if ((int)__lane_id() == lead) base0 = atomicAdd(&c.sets[s0].cursor, total);
+        base0 = __shfl(base0, lead, 64);
+        base = base0 + off;
+        set = s0;
+    } else if (req) {
+        base = atomicAdd(&c.sets[s].cursor, n);
+    }
+    if (!req) return -1;
+    if (base + n > c.sets[set].end) {
+        atomicOr(&c.sets[set].flags, SET_NEED_SPACE);
+        atomicAdd(&c.st->n_nospace, 1);
+        return -1;
+    }
+    return base;
+}
+
+// a point's outcome in its set's counters (several sets only): inserted (with its live-count
+// change), back to the pending list, or dropped as a duplicate.  One atomic per counter and
+// wavefront when all reporting lanes share a set, else one per lane.  Every lane of the
+// wavefront must call this.
+__device__ inline void set_outcome(const Ctx& c, bool report, int p, bool ins, int dlive, bool pend, bool dup) {
+    if (!c.sets) return;
+    const int s = report ? c.pset[p] : -1;
+    const unsigned long long rm = __ballot(report);
+    if (!rm) return;
+    const int lead = __ffsll((long long)rm) - 1;
+    const int s0 = __shfl(s, lead, 64);
+    if (__ballot(report && s != s0) == 0) {
+        const long long ni = wave_sum_i64(report && ins ? 1 : 0), nl = wave_sum_i64(report && ins ? dlive : 0);
+        const long long np = wave_sum_i64(report && pend ? 1 : 0), nd = wave_sum_i64(report && dup ? 1 : 0);
+        if ((int)__lane_id() == lead) {
+            SetRec& r = c.sets[s0];
+            if (ni) atomicAdd(&r.inserted, (int)ni);
+            if (nl) atomicAdd(&r.live, (int)nl);
+            if (np) atomicAdd(&r.pend, (int)np);
+            if (nd) atomicAdd(&r.dups, (int)nd);
+        }
+    } else if (report) {
+        SetRec& r = c.sets[s];
+        if (ins) {
+            atomicAdd(&r.inserted, 1);
+            atomicAdd(&r.live, dlive);
+        } else if (pend) {
+            atomicAdd(&r.pend, 1);
+        } else if (dup) {
+            atomicAdd(&r.dups, 1);
+        }
+    }
+}
+
 __device__ __forceinline__ void enqueue(const Ctx& c, uint32_t type, const uint32_t* ids) {
     int slot = atomicAdd(&c.st->n_q, 1);
     if (slot < c.qcap) {
@@ -632,30 +711,8 @@ __global__ void __launch_bounds__(256) k_commit(Ctx c, int nwin, unsigned long l
             }
         }
     }
-    int base = 0;
-    if (c.sets) {
-        // several sets: slots from the set's own range; a full range sends the point back
-        if (win) {
-            SetRec& r = c.sets[c.pset[p]];
-            base = atomicAdd(&r.cursor, nf);
-            if (base + nf > r.end) {
-                atomicOr(&r.flags, SET_NEED_SPACE);
-                atomicAdd(&c.st->n_nospace, 1);
-                win = false;
-            }
-        }
-    } else {
-        int total = 0;
-        const int off = wave_excl_scan(win ? nf : 0, total);
-        int base0 = 0;
-        if (__lane_id() == 0 && total > 0) base0 = atomicAdd(&c.st->n_tets, total);
-        base0 = __shfl(base0, 0, 64);
-        base = base0 + off;
-        if (win && base + nf > c.tet_cap) {
-            atomicOr(&c.st->err, ERR_TETCAP);
-            win = false;
-        }
-    }
+    const int base = alloc_slots(c, win, nf, p);
+    if (base < 0) win = false;
     if (win) {
         const int pv = c.M + p;
         const typename VT<D>::F pf = VT<D>::ld(c.X32, pv);
@@ -740,17 +797,7 @@ __global__ void __launch_bounds__(256) k_commit(Ctx c, int nwin, unsigned long l
             else again = true;             // list full: retry in the next window
         }
     }
-    if (c.sets && st != ST_IDLE) {
-        SetRec& r = c.sets[c.pset[p]];
-        if (win) {
-            atomicAdd(&r.inserted, 1);
-            atomicAdd(&r.live, nf - nk);
-        } else if (again) {
-            atomicAdd(&r.pend, 1);
-        } else if (st == ST_DUP) {
-            atomicAdd(&r.dups, 1);
-        }
-    }
+    set_outcome(c, st != ST_IDLE, p, win, nf - nk, !win && again, !win && !again && st == ST_DUP);
     // next window: losers (valid cavity, lost a reservation) and deferred points
     const unsigned long long mask = __ballot(again);
     const int cnt = __popcll(mask);

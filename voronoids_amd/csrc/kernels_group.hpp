@@ -289,36 +289,9 @@ __global__ void __launch_bounds__(64) k_commit_g(Ctx c, int nwin, unsigned long 
         }
     }
     const bool win = st == ST_OK && g.ballot(!ok) == 0;
-    int base = 0;
-    bool build = win;
-    if (c.sets) {
-        // several sets: one range request per winning point on its set's cursor; a full
-        // range sends the point back to the window
-        int b = 0, room = 1;
-        if (win && lead) {
-            SetRec& r = c.sets[c.pset[p]];
-            b = atomicAdd(&r.cursor, nf);
-            room = b + nf <= r.end;
-            if (!room) {
-                atomicOr(&r.flags, SET_NEED_SPACE);
-                atomicAdd(&c.st->n_nospace, 1);
-            }
-        }
-        base = g.bcast(b, 0);
-        build = win && g.bcast(room, 0);
-    } else {
-        // one simplex slot range per winning group, one atomicAdd per wavefront
-        int total = 0;
-        const int off = wave_excl_scan(win && lead ? nf : 0, total);
-        int base0 = 0;
-        if (g.lane == 0 && total > 0) base0 = atomicAdd(&c.st->n_tets, total);
-        base0 = __shfl(base0, 0, 64);
-        base = base0 + g.bcast(off, 0);
-        if (build && base + nf > c.tet_cap) {
-            if (lead) atomicOr(&c.st->err, ERR_TETCAP);
-            build = false;
-        }
-    }
+    // one slot range per winning group (the group's lead lane asks)
+    const int base = g.bcast(alloc_slots(c, win && lead, nf, p), 0);
+    const bool build = win && base >= 0;
     // a winner's check loops ran to completion, so s_fac holds its whole facet list
     __builtin_amdgcn_wave_barrier();
     const int pv = c.M + p;
@@ -405,17 +378,7 @@ __global__ void __launch_bounds__(64) k_commit_g(Ctx c, int nwin, unsigned long 
             else again = true;             // list full: retry in the next window
         }
     }
-    if (c.sets && lead && st != ST_IDLE) {
-        SetRec& r = c.sets[c.pset[p]];
-        if (build) {
-            atomicAdd(&r.inserted, 1);
-            atomicAdd(&r.live, nf - nk);
-        } else if (again) {
-            atomicAdd(&r.pend, 1);
-        } else if (st == ST_DUP) {
-            atomicAdd(&r.dups, 1);
-        }
-    }
+    set_outcome(c, lead && st != ST_IDLE, p, build, nf - nk, !build && again, !build && !again && st == ST_DUP);
     const unsigned long long mask = __ballot(again);
     const int cnt = __popcll(mask);
     int pos0 = 0;
