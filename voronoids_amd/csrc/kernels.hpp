@@ -523,13 +523,16 @@ template <int D> __device__ __forceinline__ int hint_cell(const HintView& v, typ
 }
 
 // walk start for a fresh point: the finest level whose cell has been visited (level 0 holds
-// the set's first simplex)
+// the set's first simplex); the levels' loads are issued together
 template <int D> __device__ __forceinline__ int hint_lookup(const HintView& v, typename VT<D>::F pf) {
-    int best = v.h[0];
-    for (int l = 1; l <= v.levels; ++l) {
-        const int h = v.h[hint_cell<D>(v, pf, l)];
-        if (h >= 0) best = h;
-    }
+    constexpr int LMAX = D == 3 ? 8 : 12;
+    int h[LMAX + 1];
+#pragma unroll
+    for (int l = 0; l <= LMAX; ++l) h[l] = l <= v.levels ? v.h[l == 0 ? 0 : hint_cell<D>(v, pf, l)] : -1;
+    int best = h[0];
+#pragma unroll
+    for (int l = 1; l <= LMAX; ++l)
+        if (h[l] >= 0) best = h[l];
     return best;
 }
 

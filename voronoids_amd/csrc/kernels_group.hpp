@@ -258,7 +258,9 @@ __global__ void __launch_bounds__(64) k_commit_g(Ctx c, int nwin, unsigned long 
     constexpr int NG = Group<G>::NG;
     constexpr int FC = kGroupFC<D>;
     static_assert(FC % 4 == 0, "int4 scans");
+    constexpr int DW = (FC * D + 31) / 32;     // one bit per (facet, ridge): internal face linked
     __shared__ __align__(16) int s_fac[NG][FC];
+    __shared__ unsigned s_done[NG][DW];
     const Group<G> g;
     const int W = c.win_cap;
     const bool lead = g.gl == 0;
@@ -312,8 +314,12 @@ __global__ void __launch_bounds__(64) k_commit_g(Ctx c, int nwin, unsigned long 
             *a_ptr<D>(c, N, 0) = outside;
             if (outside >= 0) *a_ptr<D>(c, outside >> 2, outside & 3) = N << 2;
         }
-        // phase 2: internal faces, one lane per (facet, ridge)
+        // phase 2: internal faces, one lane per (facet, ridge); each rotation links both sides
+        // of the face it finds, and the other side's lane skips it when it comes later
+        for (int w = g.gl; w < DW; w += G) s_done[g.gid][w] = 0u;
+        __builtin_amdgcn_wave_barrier();
         for (int q = g.gl; q < nf * D; q += G) {
+            if (s_done[g.gid][q >> 5] & (1u << (q & 31))) continue;
             const int j = q / D, k = 1 + q % D;
             const int fe = s_fac[g.gid][j];
             const int t = fe >> 2, i = fe & 3;
@@ -347,8 +353,15 @@ __global__ void __launch_bounds__(64) k_commit_g(Ctx c, int nwin, unsigned long 
                     if (r != k && comp(nv, r) == x) in_ridge = true;
                 if (!in_ridge) { kk = m; break; }
             }
-            if (jj < 0) atomicOr(&c.st->err, ERR_WALK);
-            *a_ptr<D>(c, base + j, k) = jj < 0 ? -1 : (((base + jj) << 2) | kk);
+            if (jj < 0) {
+                atomicOr(&c.st->err, ERR_WALK);
+                *a_ptr<D>(c, base + j, k) = -1;
+                continue;
+            }
+            *a_ptr<D>(c, base + j, k) = ((base + jj) << 2) | kk;
+            *a_ptr<D>(c, base + jj, kk) = ((base + j) << 2) | k;
+            const int qq = jj * D + kk - 1;
+            atomicOr(&s_done[g.gid][qq >> 5], 1u << (qq & 31));
         }
         // phase 3: killed simplices die, forwarding to a new simplex on one of their facets
         for (int j = g.gl; j < nk; j += G) {
