@@ -95,7 +95,9 @@ int vo_tri_stats(const vo_tri* t, vo_stats* out);
  * "lanes_target", default 2^20 in 3D and 2^16 in 2D, from the autotuner).  "window_gamma_milli" = initial window
  * size per inserted point x 1000.  "local_order" = 0 (default: the seeded random priority order) or k > 0
  * (the same order with consecutive chunks of position/k points sorted along the Morton curve; set before
- * vo_tri_stage). */
+ * vo_tri_stage).  "big_window" (default 16384 in 3D, 65536 in 2D; 0 = off), "big_frac_lo_milli" /
+ * "big_frac_hi_milli" (default 550 / 800): rounds whose window is at least big_window steer the window
+ * size so that this fraction of it wins (smaller windows aim at 0.35 .. 0.6). */
 int vo_tri_config(vo_tri* t, const char* key, int64_t value);
 
 /* input vertex coordinates in the reference id space (for PyVertex.point): (2(dim+1) + n) x dim */

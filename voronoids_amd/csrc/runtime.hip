@@ -121,6 +121,8 @@ struct vo_tri {
     hipEvent_t ev[6] = {};
     double gamma = 0.25;
     int filters = 3;
+    int64_t big_window = 0;            // windows at least this large target big_frac_lo..hi winners (0: off)
+    double big_frac_lo = 0.55, big_frac_hi = 0.8;
     int local_order = 0;               // priority order: random, Morton-sorted in chunks of position/this (0: off)
     int group = 0;                     // lanes per window point (0 = chosen per round, 1 = one thread)
     int64_t lanes_target = 0;          // auto group size: largest G with window * G <= this (0 = default)
@@ -478,6 +480,7 @@ int vo_tri_create(int hip_device, int dim, const double* pts, int64_t n, vo_tri*
     t->ws_threads = 1024;
     t->ws_cap = 1 << 14;
     t->gamma = dim == 3 ? 0.2 : 0.6;
+    t->big_window = dim == 3 ? 16384 : 65536;
     if ((rc = dev_alloc(&t->win, t->win_cap)) || (rc = dev_alloc(&t->pend[0], t->win_cap)) ||
         (rc = dev_alloc(&t->pend[1], t->win_cap)) || (rc = dev_alloc(&t->cav, (size_t)t->win_cap * t->kcap)) ||
         (rc = dev_alloc(&t->fac, (size_t)t->win_cap * t->fcap)) || (rc = dev_alloc(&t->nk, t->win_cap)) ||
@@ -638,8 +641,13 @@ int vo_tri_run(vo_tri* t) {
         const int64_t consumed = (int64_t)nwin - r.n_next_pending - r.n_winners;   // duplicates
         t->dups += consumed;
         const double frac = nwin ? (double)r.n_winners / nwin : 1.0;
-        if (frac > 0.6) t->gamma = std::min(t->gamma * 1.5, 4.0);
-        else if (frac < 0.35) t->gamma = std::max(t->gamma * 0.7, 0.02);
+        // keep about half of the window winning; windows past big_window are throughput-bound
+        // (a loser's cavity is wasted work, a round costs little beyond its points), so there
+        // the target winning fraction is higher
+        const bool big = t->big_window > 0 && nwin >= t->big_window;
+        const double hi = big ? t->big_frac_hi : 0.6, lo = big ? t->big_frac_lo : 0.35;
+        if (frac > hi) t->gamma = std::min(t->gamma * 1.5, 4.0);
+        else if (frac < lo) t->gamma = std::max(t->gamma * 0.7, 0.002);
         // every round either inserts, drops a duplicate, or resolves queued predicates
         stalled = (r.n_winners == 0 && consumed == 0 && r.n_q == 0) ? stalled + 1 : 0;
         if (stalled > 64) return fail(VO_ERR_INTERNAL, "no progress for 64 rounds");
@@ -821,6 +829,12 @@ int vo_tri_config(vo_tri* t, const char* key, int64_t value) {
     } else if (k == "window_gamma_milli") {
         if (value <= 0) return fail(VO_ERR_INVALID, "window_gamma_milli must be positive");
         t->gamma = (double)value / 1000.0;
+    } else if (k == "big_window") {
+        if (value < 0) return fail(VO_ERR_INVALID, "big_window must be >= 0");
+        t->big_window = value;
+    } else if (k == "big_frac_lo_milli" || k == "big_frac_hi_milli") {
+        if (value <= 0 || value >= 1000) return fail(VO_ERR_INVALID, k + " must be in 1..999");
+        (k == "big_frac_lo_milli" ? t->big_frac_lo : t->big_frac_hi) = (double)value / 1000.0;
     } else if (k == "local_order") {
         if (value < 0 || value > (1 << 20)) return fail(VO_ERR_INVALID, "local_order must be in 0..2^20");
         t->local_order = (int)value;
