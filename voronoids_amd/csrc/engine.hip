@@ -115,9 +115,12 @@ __global__ void __launch_bounds__(kPlanThreads) k_plan(BatchCtl* ctl, SetRec* se
         const int prev = ctl->nwin;
         float g = ctl->gamma;
         if (prev > 0) {
+            // as the one-set driver: about half of a window wins, more of a large one
             const float frac = (float)st->n_winners / (float)prev;
-            if (frac > 0.6f) g = fminf(g * 1.5f, 4.0f);
-            else if (frac < 0.35f) g = fmaxf(g * 0.7f, 0.02f);
+            const bool big = ctl->big_window > 0 && prev >= ctl->big_window;
+            const float hi = big ? 0.8f : 0.6f, lo = big ? 0.55f : 0.35f;
+            if (frac > hi) g = fminf(g * 1.5f, 4.0f);
+            else if (frac < lo) g = fmaxf(g * 0.7f, 0.002f);
         }
         ctl->gamma = g;
         ctl->round += 1u;
@@ -438,7 +441,8 @@ void prepare_set(HSet* h, int D) {
     set_frame(D, pts, h->n, h->c, &h->r);
     super_vertices(D, h->c, h->r * 10.0, h->sv);
     first_simplex_order(D, h->sv, h->tet0);
-    const std::vector<int32_t> perm = priority_order_local(D, pts, h->n, h->seed, h->c, h->r);
+    // the one-set path's priority order, so that every set (duplicates included) matches it
+    const std::vector<int32_t> perm = priority_order(h->n, h->seed);
     h->packed.resize((size_t)h->n * D);
     h->inidx.resize((size_t)h->n);
     for (int64_t i = 0; i < h->n; ++i) {
@@ -1189,6 +1193,7 @@ int vo_engine_create(int hip_device, int dim, int64_t max_points, int64_t max_si
     BatchCtl c0 = {};
     c0.gamma = E->gamma0;
     c0.min_window = E->min_window;
+    c0.big_window = dim == 3 ? 16384 : 65536;
     *E->st_host = st0;
     *E->ctl_host = c0;
     if (hipMemcpyAsync(E->st, E->st_host, sizeof(RoundState), hipMemcpyHostToDevice, E->s) != hipSuccess ||
@@ -1239,6 +1244,11 @@ int vo_engine_config(vo_engine* e, const char* key, int64_t value) {
         e->ctl_host->min_window = (int)value;
         VO_HIP(hipSetDevice(e->dev));
         VO_HIP(hipMemcpy(&e->ctl->min_window, &e->ctl_host->min_window, sizeof(int), hipMemcpyHostToDevice));
+    } else if (k == "big_window") {
+        if (value < 0 || value > (1 << 30)) return fail(VO_ERR_INVALID, "big_window must be in 0..2^30");
+        e->ctl_host->big_window = (int)value;
+        VO_HIP(hipSetDevice(e->dev));
+        VO_HIP(hipMemcpy(&e->ctl->big_window, &e->ctl_host->big_window, sizeof(int), hipMemcpyHostToDevice));
     } else if (k == "window_cap") {
         if (value < 64 || value > e->win_cap) return fail(VO_ERR_INVALID, "window_cap must be in 64..allocated window");
         e->win_limit = (int)value;

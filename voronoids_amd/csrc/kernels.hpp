@@ -80,7 +80,8 @@ struct BatchCtl {
     int n_done;            // sets completed since the host last collected them
     int n_active;
     int min_window;        // least fresh window of a set per round
-    int pad[2];
+    int big_window;        // windows at least this large aim at a higher winning fraction
+    int pad;
 };
 
 struct Ctx {
