@@ -48,6 +48,10 @@ typedef struct {
     int64_t solo_runs;
     /* points found to coincide with an existing vertex, counted on the device */
     int64_t duplicates_detected;
+    /* compactions of the simplex records (live ones moved to a dense array in spatial order)
+     * and their device time, ms (included in ms_run) */
+    int64_t compactions;
+    double ms_compact;
 } vo_stats;
 
 /* DelaunayTree::<3,4>::new / ::<2,3>::new (src/delaunay_tree.rs:389-510, :544-640):
@@ -97,7 +101,10 @@ int vo_tri_stats(const vo_tri* t, vo_stats* out);
  * (the same order with consecutive chunks of position/k points sorted along the Morton curve; set before
  * vo_tri_stage).  "big_window" (default 16384 in 3D, 65536 in 2D; 0 = off), "big_frac_lo_milli" /
  * "big_frac_hi_milli" (default 550 / 800): rounds whose window is at least big_window steer the window
- * size so that this fraction of it wins (smaller windows aim at 0.35 .. 0.6). */
+ * size so that this fraction of it wins (smaller windows aim at 0.35 .. 0.6).  "compact_ratio_milli" (default
+ * 0 = never; measured no faster at n = 4M: every visit misses the caches either way): between rounds, when the record array holds more than this many records (x 1000) per live
+ * simplex, the live ones move to a dense array in spatial order; "compact_min" (default 2^20): not before this
+ * many records exist. */
 int vo_tri_config(vo_tri* t, const char* key, int64_t value);
 
 /* input vertex coordinates in the reference id space (for PyVertex.point): (2(dim+1) + n) x dim */
@@ -160,8 +167,8 @@ int vo_engine_stats_get(const vo_engine* e, vo_engine_stats* out);
  * "group_lanes" (0 = per replay, or 1/4/8/16/32), "min_window" (least fresh points per set
  * and round, default 8), "window_cap" (points per round, up to the allocated 2^20/2^21),
  * "tets_per_point_milli" (first simplex range per point x 1000, default 30000 (3D) /
- * 7000 (2D)), "window_gamma_milli" (initial window per inserted point x 1000), "filters" (as
- * vo_tri_config) */
+ * 7000 (2D)), "window_gamma_milli" (initial window per inserted point x 1000), "filters" and
+ * "big_window" (as vo_tri_config) */
 int vo_engine_config(vo_engine* e, const char* key, int64_t value);
 /* batched mode: n_sets sets, set k = points set_offsets[k] .. set_offsets[k+1]-1 of pts
  * (row-major, dim columns), seed of set k = seed + k; runs to completion and writes results[k]

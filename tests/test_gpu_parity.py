@@ -177,3 +177,24 @@ def test_every_lane_group_size_matches_oracle(dim, lanes):
     ref = O.Tree.delaunay(P, O.MODE_EXACT).canonical()
     assert np.array_equal(T, ref.T) and np.array_equal(E, ref.E)
     assert t.violations() == 0
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_compaction_keeps_the_triangulation(dim):
+    # compact after every few rounds: pending points' conflict pointers, walk hints and
+    # adjacency are renumbered many times; output, check, counts and later inserts stay exact
+    P = datasets.uniform(30000, dim, seed=110 + dim)
+    Q = datasets.uniform(5000, dim, seed=120 + dim) * 0.5 + 0.25
+    t = vo.DelaunayTree(np.vstack([P, Q]))
+    t.config("compact_min", 256)
+    t.config("compact_ratio_milli", 1500)
+    t.add_points_to_tree(P, seed=5)
+    t.add_points_to_tree(Q, seed=6)
+    st = t.stats()
+    assert st["compactions"] >= 3
+    T, E = t.canonical()
+    ref = O.Tree.delaunay(np.vstack([P, Q]), O.MODE_EXACT).canonical()
+    assert np.array_equal(T, ref.T) and np.array_equal(E, ref.E)
+    assert t.violations() == 0
+    nv, ns, mx = t.counts()
+    assert ns == (t.export()[0].shape[0]) and mx == st["created"] + dim + 1 - 1 + 1

@@ -92,8 +92,8 @@ struct Ctx {
     const int* win;
     int* pend_next;
     int* big;              // points to re-run alone with large cavity buffers
-    int* cav;              // killed lists, interleaved: cav[k * win_cap + slot]
-    int* fac;              // boundary facets (t << 2 | face), interleaved likewise
+    int* cav;              // killed lists, one run per window slot: cav[slot * kcap + k]
+    int* fac;              // boundary facets (t << 2 | face), likewise: fac[slot * fcap + k]
     int* nk;
     int* nf;
     int* status;
@@ -566,7 +566,6 @@ __global__ void __launch_bounds__(256) k_cavity(Ctx c, int nwin, unsigned long l
         const int pv = c.M + p;
         const typename V::F pf = V::ld(c.X32, pv);
         const HintView hv = hint_view(c, p);
-        const int W = c.win_cap;
         int st = ST_OK;
         int t = c.cp[p];
         if (t < 0) t = hint_lookup<D>(hv, pf);
@@ -617,12 +616,12 @@ __global__ void __launch_bounds__(256) k_cavity(Ctx c, int nwin, unsigned long l
         }
         const unsigned long long skey = key_hi | (unsigned)p;
         if (st == ST_OK) {
-            c.cav[slot] = t;
+            c.cav[(size_t)slot * c.kcap] = t;
             s_cav[0][lt] = t;
             if (c.solo) *kres_p<D>(c, t) = skey;
             nk = 1;
             for (int qi = 0; qi < nk && st == ST_OK; ++qi) {
-                const int ct = qi < KL ? s_cav[qi][lt] : c.cav[qi * W + slot];
+                const int ct = qi < KL ? s_cav[qi][lt] : c.cav[(size_t)slot * c.kcap + (qi)];
                 const int4 aa = ld_a<D>(c, ct);
                 for (int f = 0; f < M; ++f) {
                     const int nb = comp(aa, f);
@@ -634,7 +633,7 @@ __global__ void __launch_bounds__(256) k_cavity(Ctx c, int nwin, unsigned long l
                         } else {
                             const int nl = nk < KL ? nk : KL;
                             for (int j = 0; j < nl && !seen; ++j) seen = s_cav[j][lt] == n;
-                            for (int j = KL; j < nk && !seen; ++j) seen = c.cav[j * W + slot] == n;
+                            for (int j = KL; j < nk && !seen; ++j) seen = c.cav[(size_t)slot * c.kcap + (j)] == n;
                         }
                         if (seen) continue;
                         typename V::F an[4];
@@ -643,7 +642,7 @@ __global__ void __launch_bounds__(256) k_cavity(Ctx c, int nwin, unsigned long l
                         if (s == SIGN_UNDECIDED) { st = ST_DEFER; break; }
                         if (s > 0) {
                             if (nk == c.kcap) { st = ST_OVERFLOW; break; }
-                            c.cav[nk * W + slot] = n;
+                            c.cav[(size_t)slot * c.kcap + (nk)] = n;
                             if (nk < KL) s_cav[nk][lt] = n;
                             if (c.solo) *kres_p<D>(c, n) = skey;
                             nk++;
@@ -651,16 +650,16 @@ __global__ void __launch_bounds__(256) k_cavity(Ctx c, int nwin, unsigned long l
                         }
                     }
                     if (nf == c.fcap) { st = ST_OVERFLOW; break; }
-                    c.fac[nf * W + slot] = (ct << 2) | f;
+                    c.fac[(size_t)slot * c.fcap + (nf)] = (ct << 2) | f;
                     nf++;
                 }
             }
         }
         if (st == ST_OK) {
             const unsigned long long key = key_hi | (unsigned)p;
-            for (int j = 0; j < nk; ++j) atomicMin(kres_p<D>(c, c.cav[j * W + slot]), key);
+            for (int j = 0; j < nk; ++j) atomicMin(kres_p<D>(c, c.cav[(size_t)slot * c.kcap + (j)]), key);
             for (int j = 0; j < nf; ++j) {
-                const int fe = c.fac[j * W + slot];
+                const int fe = c.fac[(size_t)slot * c.fcap + (j)];
                 const int nb = *a_ptr<D>(c, fe >> 2, fe & 3);
                 if (nb >= 0) atomicMin(nres_p<D>(c, nb >> 2), key);
             }
@@ -688,7 +687,6 @@ __global__ void __launch_bounds__(256) k_cavity(Ctx c, int nwin, unsigned long l
 template <int D>
 __global__ void __launch_bounds__(256) k_commit(Ctx c, int nwin, unsigned long long key_hi) {
     constexpr int M = D + 1;
-    const int W = c.win_cap;
     LocalCnt lc = {};
     round_params(c, nwin, key_hi);
     for (int wbase = blockIdx.x * blockDim.x; wbase < nwin; wbase += gridDim.x * blockDim.x) {
@@ -705,11 +703,11 @@ __global__ void __launch_bounds__(256) k_commit(Ctx c, int nwin, unsigned long l
             nf = c.nf[slot];
             win = true;
             for (int j = 0; j < nk && win; ++j) {
-                const int t = c.cav[j * W + slot];
+                const int t = c.cav[(size_t)slot * c.kcap + (j)];
                 win = *kres_p<D>(c, t) == key && *nres_p<D>(c, t) > key;
             }
             for (int j = 0; j < nf && win; ++j) {
-                const int fe = c.fac[j * W + slot];
+                const int fe = c.fac[(size_t)slot * c.fcap + (j)];
                 const int nb = *a_ptr<D>(c, fe >> 2, fe & 3);
                 if (nb >= 0) win = *kres_p<D>(c, nb >> 2) > key;
             }
@@ -724,7 +722,7 @@ __global__ void __launch_bounds__(256) k_commit(Ctx c, int nwin, unsigned long l
         // neighbor, which is patched to point back; the cavity side of the facet is overwritten
         // with the marker -2 - (base + j) so that ridge rotations find it without a search
         for (int j = 0; j < nf; ++j) {
-            const int fe = c.fac[j * W + slot];
+            const int fe = c.fac[(size_t)slot * c.fcap + (j)];
             const int t = fe >> 2, i = fe & 3;
             int* slot_ptr = a_ptr<D>(c, t, i);
             const int outside = *slot_ptr;
@@ -737,7 +735,7 @@ __global__ void __launch_bounds__(256) k_commit(Ctx c, int nwin, unsigned long l
         // nv \ {p, nv[k]}; rotating around that ridge through the cavity reaches the other
         // boundary facet containing it, whose marker names the neighbor.
         for (int j = 0; j < nf; ++j) {
-            const int fe = c.fac[j * W + slot];
+            const int fe = c.fac[(size_t)slot * c.fcap + (j)];
             const int t = fe >> 2, i = fe & 3;
             const int4 tvt = ld_v<D>(c, t);
             const int4 tat = ld_a<D>(c, t);
@@ -780,8 +778,8 @@ __global__ void __launch_bounds__(256) k_commit(Ctx c, int nwin, unsigned long l
             store_tx<D>(c, N, nx);
         }
         // phase 3: killed simplices die with a forward pointer into the star
-        for (int j = 0; j < nk; ++j) kill<D>(c, c.cav[j * W + slot], base);
-        for (int j = 0; j < nf; ++j) *v_ptr<D>(c, c.fac[j * W + slot] >> 2, 1) = base + j;
+        for (int j = 0; j < nk; ++j) kill<D>(c, c.cav[(size_t)slot * c.kcap + (j)], base);
+        for (int j = 0; j < nf; ++j) *v_ptr<D>(c, c.fac[(size_t)slot * c.fcap + (j)] >> 2, 1) = base + j;
         lc.killed += nk;
         lc.created += nf;
         lc.wins += 1;

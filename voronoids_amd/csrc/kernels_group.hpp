@@ -86,7 +86,6 @@ __global__ void __launch_bounds__(64, VO_CAVITY_WAVES) k_cavity_g(Ctx c, int nwi
     __shared__ __align__(16) int s_cav[NG][KC];
     __shared__ __align__(16) int s_tmp[64];
     const Group<G> g;
-    const int W = c.win_cap;
     const bool lead = g.gl == 0;
     LocalCnt lc = {};
     round_params(c, nwin, key_hi);
@@ -219,7 +218,7 @@ __global__ void __launch_bounds__(64, VO_CAVITY_WAVES) k_cavity_g(Ctx c, int nwi
                 if (isfac) {
                     // facets go straight to the point's global list: only the killed list,
                     // which every candidate is checked against, occupies LDS
-                    c.fac[(nf + g.below(facm)) * W + slot] = (tet << 2) | face;
+                    c.fac[(size_t)slot * c.fcap + nf + g.below(facm)] = (tet << 2) | face;
                     if (!hull) atomicMin(nres_p<D>(c, n), key);
                 }
                 nk += cnew;
@@ -239,7 +238,7 @@ __global__ void __launch_bounds__(64, VO_CAVITY_WAVES) k_cavity_g(Ctx c, int nwi
         for (int j = g.gl; j < nk; j += G) {
             const int x = s_cav[g.gid][j];
             atomicMin(kres_p<D>(c, x), key);
-            c.cav[j * W + slot] = x;
+            c.cav[(size_t)slot * c.kcap + (j)] = x;
         }
     }
     if (slot < nwin && lead) {
@@ -262,7 +261,6 @@ __global__ void __launch_bounds__(64) k_commit_g(Ctx c, int nwin, unsigned long 
     __shared__ __align__(16) int s_fac[NG][FC];
     __shared__ unsigned s_done[NG][DW];
     const Group<G> g;
-    const int W = c.win_cap;
     const bool lead = g.gl == 0;
     LocalCnt lc = {};
     round_params(c, nwin, key_hi);
@@ -279,11 +277,11 @@ __global__ void __launch_bounds__(64) k_commit_g(Ctx c, int nwin, unsigned long 
             nk = c.nk[slot];
             nf = c.nf[slot];
             for (int j = g.gl; j < nk && ok; j += G) {
-                const int x = c.cav[j * W + slot];
+                const int x = c.cav[(size_t)slot * c.kcap + (j)];
                 ok = *kres_p<D>(c, x) == key && *nres_p<D>(c, x) > key;
             }
             for (int j = g.gl; j < nf && ok; j += G) {
-                const int fe = c.fac[j * W + slot];
+                const int fe = c.fac[(size_t)slot * c.fcap + (j)];
                 s_fac[g.gid][j] = fe;
                 const int nb = *a_ptr<D>(c, fe >> 2, fe & 3);
                 if (nb >= 0) ok = *kres_p<D>(c, nb >> 2) > key;
@@ -365,7 +363,7 @@ __global__ void __launch_bounds__(64) k_commit_g(Ctx c, int nwin, unsigned long 
         }
         // phase 3: killed simplices die, forwarding to a new simplex on one of their facets
         for (int j = g.gl; j < nk; j += G) {
-            const int t = c.cav[j * W + slot];
+            const int t = c.cav[(size_t)slot * c.kcap + (j)];
             const int r = lds_find<2>(s_fac[g.gid], nf, t);
             const int fwd = base + (r < 0 ? 0 : r);
             kill<D>(c, t, fwd);

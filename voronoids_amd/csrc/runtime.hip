@@ -22,6 +22,7 @@
 #include "host_common.hpp"
 #include "kernels.hpp"
 #include "kernels_group.hpp"
+#include "compact.hpp"
 
 #pragma clang fp contract(off)
 
@@ -84,6 +85,14 @@ struct vo_tri {
     int* vmap = nullptr;               // vertex id -> reference id
     // simplices
     char* rec = nullptr;               // simplex records (128 B in 3D, 64 B in 2D)
+    char* rec_spare = nullptr;         // compaction target, same capacity
+    int* cmap = nullptr;               // compaction: old record -> new record
+    int* chist = nullptr;              // compaction: bin counts / cursors
+    int64_t retired = 0;               // records dropped by compactions (max_simplex_id counts them)
+    double compact_ratio = 0.0;        // compact when records exceed this many per live simplex (0: never)
+    int64_t compact_min = 1 << 20;     // ... and at least this many records are allocated
+    int64_t n_compact = 0;
+    double ms_compact = 0;
     int64_t tcap = 0;
     int n_tets = 1;
     int64_t n_live = 1;
@@ -236,6 +245,14 @@ int grow_tets(vo_tri* t, int64_t cap) {
     }
     t->rec = rec;
     t->tcap = cap;
+    if (t->rec_spare) {                // reallocated at the next compaction
+        hipFree(t->rec_spare);
+        t->rec_spare = nullptr;
+    }
+    if (t->cmap) {
+        hipFree(t->cmap);
+        t->cmap = nullptr;
+    }
     return 0;
 }
 
@@ -283,7 +300,7 @@ int upload_super(vo_tri* t) {
 }
 
 void free_all(vo_tri* t) {
-    void* ptrs[] = {t->X64, t->X32, t->cp, t->vmap, t->rec, t->win, t->pend[0], t->pend[1],
+    void* ptrs[] = {t->X64, t->X32, t->cp, t->vmap, t->rec, t->rec_spare, t->cmap, t->chist, t->win, t->pend[0], t->pend[1],
                     t->cav, t->fac, t->nk, t->nf, t->status, t->hint, t->st, t->cnt, t->q, t->cache, t->ws,
                     t->big, t->cav1, t->fac1, t->misc1};
     for (void* p : ptrs)
@@ -373,6 +390,79 @@ int run_solo(vo_tri* t, int n_big) {
         t->n_tets = t->st_host->n_tets;
     }
     t->n_solo += n_big;
+    return 0;
+}
+
+// Live records to a fresh array in spatial order (compact.hpp), between rounds: the pending
+// points' conflict pointers and the walk hints are first moved to live simplices, then
+// renumbered with the records.
+int compact(vo_tri* t) {
+    const int D = t->D;
+    const int n = t->n_tets;
+    const size_t B = (size_t)record_bytes(D);
+    int rc;
+    if (!t->rec_spare) {
+        // a fresh spare starts as all 0xFF records (largest reservation keys); afterwards its
+        // records keep keys of earlier rounds, which are larger than any later round's
+        if ((rc = dev_alloc(&t->rec_spare, (size_t)t->tcap * B))) return rc;
+        VO_HIP(hipMemsetAsync(t->rec_spare, 0xFF, (size_t)t->tcap * B, t->s));
+    }
+    if (!t->cmap && (rc = dev_alloc(&t->cmap, (size_t)t->tcap))) return rc;
+    if (!t->chist && (rc = dev_alloc(&t->chist, kCompactBins))) return rc;
+    VO_HIP(hipEventRecord(t->ev[4], t->s));
+    const Ctx c = make_ctx(t);
+    const int cells = (int)hint_cells(D, t->hlevels);
+    const int blocks = std::max(1, std::min(4096, (n + 255) / 256));
+    int* pend = t->pend[t->cur];
+    if (D == 3) {
+        if (t->npend) hipLaunchKernelGGL(k_chase_points<3>, dim3(256), dim3(256), 0, t->s, c, pend, t->npend);
+        hipLaunchKernelGGL(k_chase_hints<3>, dim3(1024), dim3(256), 0, t->s, c, t->hint, cells);
+    } else {
+        if (t->npend) hipLaunchKernelGGL(k_chase_points<2>, dim3(256), dim3(256), 0, t->s, c, pend, t->npend);
+        hipLaunchKernelGGL(k_chase_hints<2>, dim3(1024), dim3(256), 0, t->s, c, t->hint, cells);
+    }
+    VO_HIP(hipMemsetAsync(t->chist, 0, sizeof(int) * kCompactBins, t->s));
+    if (D == 3)
+        hipLaunchKernelGGL(k_bin_count<3>, dim3(blocks), dim3(256), 0, t->s, c, n, t->chist, t->glo[0], t->glo[1],
+                           t->glo[2], t->ginv);
+    else
+        hipLaunchKernelGGL(k_bin_count<2>, dim3(blocks), dim3(256), 0, t->s, c, n, t->chist, t->glo[0], t->glo[1],
+                           t->glo[2], t->ginv);
+    std::vector<int> h(kCompactBins);
+    VO_HIP(hipMemcpyAsync(h.data(), t->chist, sizeof(int) * kCompactBins, hipMemcpyDeviceToHost, t->s));
+    VO_HIP(hipStreamSynchronize(t->s));
+    int live = 0;
+    for (int i = 0; i < kCompactBins; ++i) {
+        const int k = h[i];
+        h[i] = live;
+        live += k;
+    }
+    VO_HIP(hipMemcpyAsync(t->chist, h.data(), sizeof(int) * kCompactBins, hipMemcpyHostToDevice, t->s));
+    if (D == 3) {
+        hipLaunchKernelGGL(k_bin_place<3>, dim3(blocks), dim3(256), 0, t->s, c, n, t->chist, t->cmap, t->glo[0],
+                           t->glo[1], t->glo[2], t->ginv);
+        hipLaunchKernelGGL(k_move<3>, dim3(blocks), dim3(256), 0, t->s, c, t->rec_spare, t->cmap, n);
+    } else {
+        hipLaunchKernelGGL(k_bin_place<2>, dim3(blocks), dim3(256), 0, t->s, c, n, t->chist, t->cmap, t->glo[0],
+                           t->glo[1], t->glo[2], t->ginv);
+        hipLaunchKernelGGL(k_move<2>, dim3(blocks), dim3(256), 0, t->s, c, t->rec_spare, t->cmap, n);
+    }
+    if (t->npend) hipLaunchKernelGGL(k_remap_points, dim3(256), dim3(256), 0, t->s, t->cp, pend, t->npend, t->cmap);
+    hipLaunchKernelGGL(k_remap_hints, dim3(1024), dim3(256), 0, t->s, t->hint, cells, t->cmap);
+    VO_HIP(hipGetLastError());
+    // records past the live ones keep stale contents: the commit writes every field of a new
+    // record except its reservations, and stale keys are larger than any later round's
+    VO_HIP(hipMemcpyAsync(&t->st->n_tets, &live, sizeof(int), hipMemcpyHostToDevice, t->s));
+    VO_HIP(hipEventRecord(t->ev[5], t->s));
+    if ((rc = sync_state(t)) || (rc = check_err(t))) return rc;
+    float ms = 0;
+    hipEventElapsedTime(&ms, t->ev[4], t->ev[5]);
+    t->ms_compact += ms;
+    std::swap(t->rec, t->rec_spare);
+    t->retired += n - live;
+    t->n_tets = live;
+    t->n_live = live;
+    t->n_compact++;
     return 0;
 }
 
@@ -655,6 +745,9 @@ int vo_tri_run(vo_tri* t) {
         t->cur ^= 1;
         t->head += nfresh;
         t->rounds++;
+        if (t->compact_ratio > 0 && t->n_tets >= t->compact_min && (double)t->n_tets > t->compact_ratio * (double)t->n_live &&
+            (rc = compact(t)))
+            return rc;
     }
     VO_HIP(hipEventRecord(t->ev[1], t->s));
     VO_HIP(hipEventSynchronize(t->ev[1]));
@@ -675,7 +768,7 @@ int vo_tri_counts(const vo_tri* t, int64_t* n_vertices, int64_t* n_simplices, in
     if (!t) return fail(VO_ERR_INVALID, "null handle");
     if (n_vertices) *n_vertices = t->M + t->inserted;
     if (n_simplices) *n_simplices = t->n_live;
-    if (max_simplex_id) *max_simplex_id = (int64_t)t->n_tets - 1 + t->M;
+    if (max_simplex_id) *max_simplex_id = (int64_t)t->n_tets + t->retired - 1 + t->M;
     return 0;
 }
 
@@ -809,6 +902,8 @@ int vo_tri_stats(const vo_tri* t, vo_stats* out) {
     out->launches_locate_cavity = t->n_cav;
     out->launches_commit = t->n_com;
     out->solo_runs = t->n_solo;
+    out->compactions = t->n_compact;
+    out->ms_compact = t->ms_compact;
     out->duplicates_detected = (int64_t)h.dups;
     return 0;
 }
@@ -835,6 +930,12 @@ int vo_tri_config(vo_tri* t, const char* key, int64_t value) {
     } else if (k == "big_frac_lo_milli" || k == "big_frac_hi_milli") {
         if (value <= 0 || value >= 1000) return fail(VO_ERR_INVALID, k + " must be in 1..999");
         (k == "big_frac_lo_milli" ? t->big_frac_lo : t->big_frac_hi) = (double)value / 1000.0;
+    } else if (k == "compact_min") {
+        if (value < 16) return fail(VO_ERR_INVALID, "compact_min must be >= 16");
+        t->compact_min = value;
+    } else if (k == "compact_ratio_milli") {
+        if (value != 0 && value < 1500) return fail(VO_ERR_INVALID, "compact_ratio_milli must be 0 or >= 1500");
+        t->compact_ratio = (double)value / 1000.0;
     } else if (k == "local_order") {
         if (value < 0 || value > (1 << 20)) return fail(VO_ERR_INVALID, "local_order must be in 0..2^20");
         t->local_order = (int)value;
