@@ -277,6 +277,51 @@ def run_served(seed: int = 0, load: float = 0.8, target_s: float = 8.0) -> dict:
     return out
 
 
+def _tier_fractions(st: dict) -> dict:
+    out = {}
+    for i, name in enumerate(("orient", "insphere")):
+        tot = st["tier1"][i] + st["tier2"][i] + st["tier3"][i] + st["deferred"][i]
+        out[name] = {"calls": tot, "fp32": st["tier1"][i] / max(tot, 1), "fp64_filter": st["tier2"][i] / max(tot, 1),
+                     "fp64_expansion": (st["tier3"][i] + st["deferred"][i]) / max(tot, 1)}
+    return out
+
+
+def run_precision(sample: int = 20000) -> dict:
+    """Precision tiers on the two hard 3D inputs: (a) n = 4,000,000 clustered (80% from 64
+    Gaussians, sigma 0.02, 20% uniform), (b) the 128^3 lattice with +-1e-9 jitter.  Each is
+    triangulated once (per-tier resolution fractions of every predicate call, exact device
+    check of the result), and `sample` predicate instances of the same kinds of geometry go
+    through the tier cascade one by one (vo_predicates) and through the oracle's exact
+    big-integer predicates: the sign disagreements are counted (expected: none)."""
+    import voronoids_amd as vo
+    from oracle import oracle as O
+    from voronoids_amd import batch as B
+    from voronoids_amd import datasets
+    out = {}
+    for key, P in (("clustered_4m", datasets.clustered(4_000_000, seed=0)), ("lattice_128", datasets.lattice_jitter(128, seed=0))):
+        t = vo.DelaunayTree(P)
+        t.stage(P, seed=0)
+        t.run()
+        st = t.stats()
+        viol = t.violations()
+        t.close()
+        out[key] = {"n": P.shape[0], "ms": st["ms_run"], "pts_per_s": P.shape[0] / (st["ms_run"] / 1e3),
+                    "rounds": st["rounds"], "tiers": _tier_fractions(st), "exact_evaluated": st["exact_evaluated"],
+                    "solo_runs": st["solo_runs"], "violations": viol}
+        print(f"# precision {key}: {st['ms_run']:.1f} ms, violations {viol}", file=sys.stderr, flush=True)
+    fn = {"orient3d": O.orient3d, "insphere": O.insphere}
+    check = {}
+    for fam in ("clustered", "lattice_jitter", "rounded_sphere", "exact_degenerate"):
+        for name in ("orient3d", "insphere"):
+            T = datasets.PREDICATE_FAMILIES[fam](name, sample // 8, seed=7)
+            signs, tiers = B.predicates(name, T)
+            ref = np.array([fn[name](*T[i]) for i in range(T.shape[0])])
+            check[f"{fam}/{name}"] = {"n": int(T.shape[0]), "disagreements": int((signs != ref).sum()),
+                                      "tier_counts": [int((tiers == k).sum()) for k in (1, 2, 3)]}
+    out["sign_check_vs_exact_oracle"] = check
+    return out
+
+
 def copy_calibration(nbytes: int, reps: int = 3) -> None:
     """Device-to-device copies of a known size (the runtime's copy kernel), so that rocprofv3's
     FETCH_SIZE / WRITE_SIZE can be scaled to bytes."""
@@ -303,7 +348,8 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched-sets measurement")
     ap.add_argument("--no-served", action="store_true", help="skip the served-requests measurement")
-    ap.add_argument("--only", choices=["batched", "served"], help="run only that measurement")
+    ap.add_argument("--no-precision", action="store_true", help="skip the precision-tier measurement")
+    ap.add_argument("--only", choices=["batched", "served", "precision"], help="run only that measurement")
     ap.add_argument("--check", action="store_true", help="run the exact device check after the last step")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run device-to-device copies of BYTES (for rocprofv3 counter scaling)")
@@ -317,7 +363,7 @@ def main() -> int:
         return 2
 
     if args.only:
-        r = run_batched() if args.only == "batched" else run_served()
+        r = {"batched": run_batched, "served": run_served, "precision": run_precision}[args.only]()
         print(json.dumps({"metric": args.only, **r}), flush=True)
         return 0
 
@@ -388,6 +434,8 @@ def main() -> int:
         out["batched"] = run_batched()
     if args.workload == "3d4m" and not args.no_served:
         out["served"] = run_served()
+    if args.workload == "3d4m" and not args.no_precision:
+        out["precision"] = run_precision()
     if not args.no_cpu_baseline:
         threads = min(16, os.cpu_count() or 1)
         out["cpu_baseline"] = cpu_baseline(dim, threads)

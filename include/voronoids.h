@@ -107,6 +107,16 @@ int vo_tri_stats(const vo_tri* t, vo_stats* out);
  * many records exist. */
 int vo_tri_config(vo_tri* t, const char* key, int64_t value);
 
+/* The precision tiers one predicate at a time (no counterpart in the reference, whose in_sphere
+ * (src/geometry.rs:91-97) is a plain f64 distance test): signs of n instances of one predicate through
+ * the construction's cascade -- FP32 on fl32(x - center), FP64 filter, FP64 expansions -- and the tier
+ * that decided each (1, 2, 3).  type: 1 orient2d(a,b,c), 2 incircle(a,b,c,d), 3 orient3d(a,b,c,d),
+ * 4 insphere(a,b,c,d,e); pts = n x arity x dim f64.  Signs: orient = sign det[b-a; c-a(; d-a)];
+ * incircle / insphere > 0 iff the last point is inside the circle / sphere of the others when they are
+ * positively oriented.  A sign of 2 means the expansions left the f64 range. */
+int vo_predicates(int hip_device, int type, const double* pts, int64_t n, const double* center, int32_t* signs,
+                  int32_t* tiers);
+
 /* input vertex coordinates in the reference id space (for PyVertex.point): (2(dim+1) + n) x dim */
 int vo_tri_vertices(const vo_tri* t, double* coords);
 

@@ -28,6 +28,7 @@ EXPORTED = [
     "vo_engine_finish", "vo_engine_results", "vo_engine_stats_get", "vo_engine_config", "vo_batch_triangulate",
     "vo_result_canonical", "vo_clock",
     "vo_server_create", "vo_server_submit", "vo_server_results", "vo_server_stats_get", "vo_server_destroy",
+    "vo_predicates",
 ]
 
 
@@ -130,6 +131,7 @@ def lib() -> ctypes.CDLL:
         L.vo_server_stats_get.argtypes = [P, ctypes.c_int, ctypes.POINTER(VoEngineStats)]
         L.vo_server_destroy.argtypes = [P]
         L.vo_server_destroy.restype = None
+        L.vo_predicates.argtypes = [ctypes.c_int, ctypes.c_int, dp, ctypes.c_int64, dp, i32p, i32p]
         L.vo_last_error.restype = ctypes.c_char_p
         L.vo_version.restype = ctypes.c_int
         _lib = L

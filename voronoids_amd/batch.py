@@ -253,3 +253,24 @@ class Server:
         s = _lib.VoEngineStats()
         _lib.check(_lib.lib().vo_server_stats_get(self._h, dim, ctypes.byref(s)))
         return s.as_dict()
+
+
+PRED_TYPES = {"orient2d": (1, 2, 3), "incircle": (2, 2, 4), "orient3d": (3, 3, 4), "insphere": (4, 3, 5)}
+
+
+def predicates(name: str, pts, center=None, device: int = 0) -> tuple[np.ndarray, np.ndarray]:
+    """signs and deciding tiers (1 FP32, 2 FP64 filter, 3 FP64 expansions) of predicate
+    instances through the construction's cascade (vo_predicates); pts: (n, arity, dim)"""
+    t, dim, arity = PRED_TYPES[name]
+    p = np.ascontiguousarray(pts, dtype=np.float64).reshape(-1, arity, dim)
+    n = p.shape[0]
+    if center is None:
+        flat = p.reshape(-1, dim)
+        center = (flat.min(axis=0) + flat.max(axis=0)) / 2 if n else np.zeros(dim)
+    c = np.zeros(3)
+    c[:dim] = center
+    signs = np.zeros(n, np.int32)
+    tiers = np.zeros(n, np.int32)
+    _lib.check(_lib.lib().vo_predicates(device, t, p.ctypes.data_as(_dp), n, c.ctypes.data_as(_dp),
+                                        signs.ctypes.data_as(_i32p), tiers.ctypes.data_as(_i32p)))
+    return signs, tiers

@@ -520,8 +520,102 @@ int launch_round(vo_tri* t, const Ctx& c, int nwin, unsigned long long key_hi, i
 
 }  // namespace
 
+// ------------------------------------------------------------------ predicate tiers, one by one
+// Sign of each predicate instance through the tier cascade of the construction: FP32 on
+// coordinates stored as fl32(x - center), then the FP64 filter, then FP64 expansions; the tier
+// that decided is reported.  Grid-stride over the instances, one expansion workspace slice per
+// thread.
+template <int D>
+__global__ void __launch_bounds__(64) k_eval_predicates(int type, const double* pts, int n, double c0, double c1,
+                                                        double c2, double* ws, int ws_cap, int32_t* sign,
+                                                        int32_t* tier) {
+    using F = typename VT<D>::F;
+    const int A = pred_arity((uint32_t)type);
+    const int tid = blockIdx.x * blockDim.x + threadIdx.x;
+    const double cc[3] = {c0, c1, c2};
+    for (int i = tid; i < n; i += gridDim.x * blockDim.x) {
+        const double* P[5];
+        F f[5];
+        for (int k = 0; k < A; ++k) {
+            P[k] = pts + ((size_t)i * A + k) * D;
+            float q[3] = {0.f, 0.f, 0.f};
+            for (int j = 0; j < D; ++j) q[j] = (float)(P[k][j] - cc[j]);
+            if constexpr (D == 3) f[k] = make_float4(q[0], q[1], q[2], 0.f);
+            else f[k] = make_float2(q[0], q[1]);
+        }
+        int s, t = 1;
+        if constexpr (D == 3) {
+            s = type == PRED_ORIENT3 ? t1_orient3d(f[0], f[1], f[2], f[3]) : t1_insphere(f[0], f[1], f[2], f[3], f[4]);
+            if (s == SIGN_UNDECIDED) {
+                t = 2;
+                s = type == PRED_ORIENT3 ? t2_orient3d(P[0], P[1], P[2], P[3]) : t2_insphere(P[0], P[1], P[2], P[3], P[4]);
+            }
+        } else {
+            s = type == PRED_ORIENT2 ? t1_orient2d(f[0], f[1], f[2]) : t1_incircle(f[0], f[1], f[2], f[3]);
+            if (s == SIGN_UNDECIDED) {
+                t = 2;
+                s = type == PRED_ORIENT2 ? t2_orient2d(P[0], P[1], P[2]) : t2_incircle(P[0], P[1], P[2], P[3]);
+            }
+        }
+        if (s == SIGN_UNDECIDED) {
+            t = 3;
+            Ws w{ws + (size_t)tid * ws_cap, ws_cap, 0, 0};
+            switch (type) {
+                case PRED_ORIENT2: s = ex_orient2d(w, P[0], P[1], P[2]); break;
+                case PRED_INCIRCLE: s = ex_incircle(w, P[0], P[1], P[2], P[3]); break;
+                case PRED_ORIENT3: s = ex_orient3d(w, P[0], P[1], P[2], P[3]); break;
+                default: s = ex_insphere(w, P[0], P[1], P[2], P[3], P[4]); break;
+            }
+            if (w.ovf) s = SIGN_UNDECIDED;
+        }
+        sign[i] = s;
+        tier[i] = t;
+    }
+}
+
 // =================================================================== C ABI
 extern "C" {
+
+int vo_predicates(int hip_device, int type, const double* pts, int64_t n, const double* center, int32_t* signs,
+                  int32_t* tiers) {
+    if (type < 1 || type > 4) return fail(VO_ERR_INVALID, "type must be 1 (orient2d), 2 (incircle), 3 (orient3d) or 4 (insphere)");
+    if (n < 0 || (n > 0 && (!pts || !center || !signs || !tiers))) return fail(VO_ERR_INVALID, "null argument");
+    if (n > (int64_t)1 << 28) return fail(VO_ERR_INVALID, "too many predicates for one call");
+    if (n == 0) return 0;
+    const int D = type <= 2 ? 2 : 3;
+    const int A = pred_arity((uint32_t)type);
+    for (int64_t i = 0; i < n * A * D; ++i)
+        if (!std::isfinite(pts[i])) return fail(VO_ERR_INVALID, "non-finite coordinate");
+    int ndev = 0;
+    VO_HIP(hipGetDeviceCount(&ndev));
+    if (hip_device < 0 || hip_device >= ndev) return fail(VO_ERR_INVALID, "no such HIP device");
+    VO_HIP(hipSetDevice(hip_device));
+    const int threads = 64, blocks = 256, ws_cap = 1 << 12;
+    double *dp = nullptr, *ws = nullptr;
+    int32_t *ds = nullptr, *dt = nullptr;
+    int rc;
+    auto release = [&] { hipFree(dp); hipFree(ws); hipFree(ds); hipFree(dt); };
+    if ((rc = dev_alloc(&dp, (size_t)n * A * D)) || (rc = dev_alloc(&ws, (size_t)threads * blocks * ws_cap)) ||
+        (rc = dev_alloc(&ds, (size_t)n)) || (rc = dev_alloc(&dt, (size_t)n))) {
+        release();
+        return rc;
+    }
+    hipError_t e = hipMemcpy(dp, pts, sizeof(double) * (size_t)n * A * D, hipMemcpyHostToDevice);
+    if (e == hipSuccess) {
+        if (D == 3)
+            hipLaunchKernelGGL(k_eval_predicates<3>, dim3(blocks), dim3(threads), 0, 0, type, dp, (int)n, center[0],
+                               center[1], center[2], ws, ws_cap, ds, dt);
+        else
+            hipLaunchKernelGGL(k_eval_predicates<2>, dim3(blocks), dim3(threads), 0, 0, type, dp, (int)n, center[0],
+                               center[1], 0.0, ws, ws_cap, ds, dt);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipMemcpy(signs, ds, sizeof(int32_t) * (size_t)n, hipMemcpyDeviceToHost);
+    if (e == hipSuccess) e = hipMemcpy(tiers, dt, sizeof(int32_t) * (size_t)n, hipMemcpyDeviceToHost);
+    release();
+    if (e != hipSuccess) return fail(VO_ERR_HIP, std::string("vo_predicates: ") + hipGetErrorString(e));
+    return 0;
+}
 
 const char* vo_last_error(void) { return vo_host::g_err.c_str(); }
 int vo_version(void) { return 1; }

@@ -79,3 +79,52 @@ def batch_sets(sizes: np.ndarray, dim: int = 3, seed: int = 0) -> tuple[np.ndarr
     off = np.zeros(len(sizes) + 1, np.int64)
     off[1:] = np.cumsum(sizes)
     return uniform(int(off[-1]), dim, seed, stream=18), off
+
+
+# ------------------------------------------------------------------ predicate instances
+# Tuples (n, arity, dim) of f64 points for one predicate, in families that make every
+# precision tier decide some of them: generic, tight clusters, a jittered lattice, points
+# rounded onto one circle / sphere, and exactly degenerate lattice tuples (sign 0).
+
+PREDICATE_ARITY = {"orient2d": (3, 2), "incircle": (4, 2), "orient3d": (4, 3), "insphere": (5, 3)}
+
+
+def tuples_uniform(name, n, seed):
+    a, d = PREDICATE_ARITY[name]
+    return unit(seed, 40, 0, n * a * d).reshape(n, a, d)
+
+
+def tuples_clustered(name, n, seed, sigma=0.02):
+    # tuples from one tight Gaussian cluster each (the clustered configuration's local geometry)
+    a, d = PREDICATE_ARITY[name]
+    rng = np.random.default_rng(seed)
+    centers = rng.random((n, 1, d))
+    return centers + sigma * rng.normal(size=(n, a, d))
+
+
+def tuples_lattice_jitter(name, n, seed, side=128, jitter=1e-9):
+    # neighboring points of a side^d lattice scaled to [0,1)^d, jittered by +-jitter
+    a, d = PREDICATE_ARITY[name]
+    rng = np.random.default_rng(seed)
+    base = rng.integers(0, side - 2, size=(n, 1, d))
+    offs = rng.integers(0, 2, size=(n, a, d))
+    return (base + offs) / side + (2 * rng.random((n, a, d)) - 1) * jitter
+
+
+def tuples_exact_degenerate(name, n, seed):
+    # lattice points without jitter: many exactly coplanar / cospherical tuples (sign 0)
+    a, d = PREDICATE_ARITY[name]
+    rng = np.random.default_rng(seed)
+    return rng.integers(0, 3, size=(n, a, d)) / 8.0
+
+
+def tuples_rounded_sphere(name, n, seed):
+    # points rounded onto one circle / sphere: in-sphere determinants at the f64 rounding level
+    a, d = PREDICATE_ARITY[name]
+    rng = np.random.default_rng(seed)
+    u = rng.normal(size=(n, a, d))
+    return 0.5 + 0.3 * u / np.linalg.norm(u, axis=2, keepdims=True)
+
+
+PREDICATE_FAMILIES = {"uniform": tuples_uniform, "clustered": tuples_clustered, "lattice_jitter": tuples_lattice_jitter,
+                      "exact_degenerate": tuples_exact_degenerate, "rounded_sphere": tuples_rounded_sphere}
