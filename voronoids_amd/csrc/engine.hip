@@ -50,6 +50,7 @@ namespace {
 // ------------------------------------------------------------------ device side
 struct DoneRec {
     int slot, live, inserted, dups;
+    int cursor, fail_lo, end;   // the last range's written part ends at min of the three
 };
 
 struct AdmitDesc {
@@ -204,7 +205,7 @@ __global__ void k_collect(BatchCtl* ctl, SetRec* sets, int nslots, DoneRec* done
         if ((r.flags & SET_ACTIVE) && r.inserted + r.dups == r.n) {
             r.flags = SET_DONE;
             const int k = atomicAdd(&ctl->n_done, 1);
-            done[k] = DoneRec{s, r.live, r.inserted, r.dups};
+            done[k] = DoneRec{s, r.live, r.inserted, r.dups, r.cursor, r.fail_lo, r.end};
         }
     }
 }
@@ -230,23 +231,17 @@ struct Arena {
     SetRec* sets;
 };
 
-// records of a fresh simplex range: all bytes 0xFF (dead, hull adjacency, largest reservation
-// keys), written as 64-bit words
-template <int D>
-__device__ __forceinline__ void clear_records(char* rec, int t0, int t1) {
-    constexpr int W = (D == 3 ? 128 : 64) / 8;
-    unsigned long long* q = reinterpret_cast<unsigned long long*>(rec);
-    for (long long i = (long long)t0 * W + threadIdx.x; i < (long long)t1 * W; i += blockDim.x) q[i] = ~0ull;
-}
 
-// one block per admitted set: hint grid cleared (level 0 = the first simplex), simplex range
-// marked dead, super vertices, first simplex and set record written
+// one block per admitted set: hint grid cleared (level 0 = the first simplex), super
+// vertices, first simplex and set record written.  The rest of the simplex range is not
+// cleared: a record is read only after the commit wrote it, the export scans only the written
+// part of each range, and reservation keys left by earlier sets are from earlier rounds, so
+// larger than any current key (a never-used arena starts as all 0xFF bytes)
 template <int D>
 __global__ void __launch_bounds__(256) k_admit_sets(const AdmitDesc* d, Arena a) {
     constexpr int M = D + 1, S = D == 3 ? 4 : 2;
     const AdmitDesc& e = d[blockIdx.x];
     for (int i = threadIdx.x; i < e.hcells; i += 256) a.hint[e.hint_off + i] = -1;
-    clear_records<D>(a.rec, e.root + 1, e.range_end);
     __syncthreads();
     if (threadIdx.x != 0) return;
     a.hint[e.hint_off] = e.root;
@@ -272,6 +267,7 @@ __global__ void __launch_bounds__(256) k_admit_sets(const AdmitDesc* d, Arena a)
     r.ginv = e.ginv;
     r.live = 1;
     r.n = e.n;
+    r.fail_lo = 0x7FFFFFFF;
     a.sets[e.slot] = r;
 }
 
@@ -306,8 +302,9 @@ __global__ void __launch_bounds__(256) k_admit_points(const AdmitDesc* d, int nd
 template <int D>
 __global__ void __launch_bounds__(256) k_extend(const ExtendRec* x, char* rec, SetRec* sets) {
     const ExtendRec e = x[blockIdx.x];
-    clear_records<D>(rec, e.start, e.start + e.len);
+    (void)rec;
     if (threadIdx.x == 0) {
+        sets[e.slot].fail_lo = 0x7FFFFFFF;
         sets[e.slot].cursor = e.start;
         sets[e.slot].end = e.start + e.len;
         sets[e.slot].flags &= ~SET_NEED_SPACE;
@@ -429,7 +426,9 @@ struct HSet {
     int slot = -1;
     int64_t v0 = 0, h0 = 0, hcells = 0;
     int hlevels = 0;
-    std::vector<std::pair<int64_t, int64_t>> ranges;   // simplex ranges [start, start + len)
+    // simplex ranges [start, start + len) and, once closed, how many records were written
+    struct Rng { int64_t start, len, used; };
+    std::vector<Rng> ranges;
     double t_submit = 0, t_admit = 0;
     unsigned round_admit = 0;
 };
@@ -800,7 +799,7 @@ int admit(vo_engine* e) {
         h->h0 = h0;
         h->hcells = hc;
         h->hlevels = L;
-        h->ranges.push_back({t0, tl});
+        h->ranges.push_back({t0, tl, -1});
         h->slot = e->free_slots.back();
         e->free_slots.pop_back();
         h->t_admit = now_s();
@@ -846,8 +845,8 @@ int admit(vo_engine* e) {
         d.slot = h->slot;
         d.n = (int)h->n;
         d.v0 = (int)h->v0;
-        d.root = (int)h->ranges[0].first;
-        d.range_end = (int)(h->ranges[0].first + h->ranges[0].second);
+        d.root = (int)h->ranges[0].start;
+        d.range_end = (int)(h->ranges[0].start + h->ranges[0].len);
         d.hint_off = (int)h->h0;
         d.hlevels = h->hlevels;
         d.hcells = (int)h->hcells;
@@ -973,7 +972,11 @@ int extend_ranges(vo_engine* e) {
             st = e->talloc.alloc(len);
         }
         if (st < 0) continue;                       // retried after other sets free their ranges
-        h->ranges.push_back({st, len});
+        // the full range is closed: its written part ends at the first failed request
+        HSet::Rng& last = h->ranges.back();
+        const int64_t used_end = std::min<int64_t>({(int64_t)recs[s].cursor, (int64_t)recs[s].fail_lo, last.start + last.len});
+        last.used = std::max<int64_t>(0, used_end - last.start);
+        h->ranges.push_back({st, len, -1});
         ext.push_back(ExtendRec{s, (int)st, (int)len, 0});
     }
     if (ext.empty()) return 0;
@@ -1026,9 +1029,15 @@ int collect(vo_engine* e, int n_done) {
             obase.push_back(rows);
             rows += lv;
             HSet* h = e->resident[dr[k1].slot].get();
+            {
+                const DoneRec& d = dr[k1];
+                HSet::Rng& last = h->ranges.back();
+                const int64_t used_end = std::min<int64_t>({(int64_t)d.cursor, (int64_t)d.fail_lo, last.start + last.len});
+                last.used = std::max<int64_t>(0, used_end - last.start);
+            }
             for (auto& rg : h->ranges)
-                for (int64_t a = 0; a < rg.second; a += 4096)
-                    ch.push_back(ExportChunk{(int)(rg.first + a), (int)std::min<int64_t>(4096, rg.second - a),
+                for (int64_t a = 0; a < rg.used; a += 4096)
+                    ch.push_back(ExportChunk{(int)(rg.start + a), (int)std::min<int64_t>(4096, rg.used - a),
                                              (int)(k1 - k0)});
             ++k1;
         }
@@ -1102,7 +1111,7 @@ int collect(vo_engine* e, int n_done) {
             e->ready.push_back(r);
             e->valloc.release(h->v0, e->M + h->n);
             e->halloc.release(h->h0, h->hcells);
-            for (auto& rg : h->ranges) e->talloc.release(rg.first, rg.second);
+            for (auto& rg : h->ranges) e->talloc.release(rg.start, rg.len);
             e->free_slots.push_back(d.slot);
             e->n_resident--;
             e->stats.sets_done++;

@@ -67,6 +67,7 @@ struct SetRec {
     int live;              // live simplices of the set
     int n;                 // points of the set
     int fresh, woff;       // planner: fresh points windowed this round and their offset
+    int fail_lo;           // least slot of a failed request in the current range (slots below it are all written)
 };
 
 // round control planned on the device when several sets share the arrays: the kernels read
@@ -337,6 +338,7 @@ __device__ inline int alloc_slots(const Ctx& c, bool req, int n, int p) {
     }
     if (!req) return -1;
     if (base + n > c.sets[set].end) {
+        atomicMin(&c.sets[set].fail_lo, base);
         atomicOr(&c.sets[set].flags, SET_NEED_SPACE);
         atomicAdd(&c.st->n_nospace, 1);
         return -1;
