@@ -512,6 +512,10 @@ struct vo_engine {
     std::deque<std::function<void()>> tasks;   // copies go to the front, preparations to the back
     bool prep_stop = false;
     int32_t* out_h = nullptr;                   // pinned bounce buffer of the export
+    // exported results whose rows still sit in out_h: copied into their own buffers later, so
+    // that the copies overlap the next replay
+    std::vector<vo_set_result> copy_res;
+    std::vector<int64_t> copy_off;              // first row in out_h of each
     // stats
     vo_engine_stats stats{};
 };
@@ -996,9 +1000,40 @@ int extend_ranges(vo_engine* e) {
     return 0;
 }
 
-// export and free completed sets
+// rows of exported sets from the pinned bounce buffer into each result's own buffer (worker
+// threads); the results join the ready queue
+int flush_copies(vo_engine* e) {
+    const int ns = (int)e->copy_res.size();
+    if (ns == 0) return 0;
+    const int M = e->M;
+    const int32_t* src = e->out_h;
+    std::atomic<int> oom{0};
+    parallel_for(e, ns, [&](int i) {
+        vo_set_result& r = e->copy_res[i];
+        const size_t cnt = (size_t)r.n_simplices * M;
+        int32_t* b = (int32_t*)std::malloc(sizeof(int32_t) * std::max<size_t>(1, cnt));
+        if (!b) { oom.store(1); return; }
+        if (cnt) std::memcpy(b, src + (size_t)e->copy_off[i] * M, sizeof(int32_t) * cnt);
+        r.simplices = b;
+    });
+    const double t = now_s();
+    for (auto& r : e->copy_res) {
+        r.t_done = t;
+        e->ready.push_back(r);
+    }
+    e->copy_res.clear();
+    e->copy_off.clear();
+    if (oom.load()) return fail(VO_ERR_OOM, "host result buffer");
+    return 0;
+}
+
+// export and free completed sets: their live simplices go to the pinned bounce buffer (one
+// device-to-host copy), their arena ranges and slot are released at once, and their result
+// buffers are filled by flush_copies
 int collect(vo_engine* e, int n_done) {
     if (n_done <= 0) return 0;
+    int rc0 = flush_copies(e);
+    if (rc0) return rc0;
     std::vector<DoneRec> dr(n_done);
     VO_HIP(hipMemcpyAsync(dr.data(), e->done, sizeof(DoneRec) * n_done, hipMemcpyDeviceToHost, e->s));
     VO_HIP(hipStreamSynchronize(e->s));
@@ -1047,7 +1082,9 @@ int collect(vo_engine* e, int n_done) {
         const size_t oc_off = ol_off + ((sizeof(int) * ns + 255) & ~(size_t)255);
         const size_t ch_off = oc_off + ((sizeof(int) * ns + 255) & ~(size_t)255);
         const size_t total = ch_off + sizeof(ExportChunk) * ch.size();
-        int rc = ensure_stage(e, total);
+        int rc = flush_copies(e);              // out_h is about to be overwritten
+        if (rc) return rc;
+        rc = ensure_stage(e, total);
         if (rc) return rc;
         std::memcpy(e->stage_h + ob_off, obase.data(), sizeof(long long) * ns);
         for (int i = 0; i < ns; ++i) reinterpret_cast<int*>(e->stage_h + ol_off)[i] = dr[k0 + i].live;
@@ -1075,27 +1112,10 @@ int collect(vo_engine* e, int n_done) {
         if (rows > 0)
             VO_HIP(hipMemcpyAsync(e->out_h, e->out_d, sizeof(int32_t) * (size_t)rows * M, hipMemcpyDeviceToHost, e->s));
         VO_HIP(hipStreamSynchronize(e->s));
-        std::vector<int32_t*> bufs(ns, nullptr);
-        const int32_t* src = e->out_h;
-        parallel_for(e, ns, [&](int i) {
-            const size_t cnt = (size_t)dr[k0 + i].live * M;
-            int32_t* b = (int32_t*)std::malloc(sizeof(int32_t) * std::max<size_t>(1, cnt));
-            if (b && cnt) std::memcpy(b, src + (size_t)obase[i] * M, sizeof(int32_t) * cnt);
-            bufs[i] = b;
-        });
-        for (int i = 0; i < ns; ++i)
-            if (!bufs[i]) {
-                for (int j = 0; j < ns; ++j) std::free(bufs[j]);
-                return fail(VO_ERR_OOM, "host result buffer");
-            }
-        const double t = now_s();
         for (int i = 0; i < ns; ++i) {
             const DoneRec& d = dr[k0 + i];
             std::unique_ptr<HSet> h = std::move(e->resident[d.slot]);
-            if (ocur[i] != d.live) {
-                for (int j = i; j < ns; ++j) std::free(bufs[j]);
-                return fail(VO_ERR_INTERNAL, "export count differs from the set's live count");
-            }
+            if (ocur[i] != d.live) return fail(VO_ERR_INTERNAL, "export count differs from the set's live count");
             vo_set_result r{};
             r.tag = h->tag;
             r.dim = e->D;
@@ -1103,12 +1123,12 @@ int collect(vo_engine* e, int n_done) {
             r.n_points = h->n;
             r.points_duplicate = d.dups;
             r.n_simplices = d.live;
-            r.simplices = bufs[i];
+            r.simplices = nullptr;
             r.t_submit = h->t_submit;
             r.t_admit = h->t_admit;
-            r.t_done = t;
             r.rounds = (int64_t)(e->ctl_host->round - h->round_admit);
-            e->ready.push_back(r);
+            e->copy_res.push_back(r);
+            e->copy_off.push_back(obase[i]);
             e->valloc.release(h->v0, e->M + h->n);
             e->halloc.release(h->h0, h->hcells);
             for (auto& rg : h->ranges) e->talloc.release(rg.start, rg.len);
@@ -1306,7 +1326,7 @@ int vo_engine_launch(vo_engine* e) {
     e->stats.ms_boundary += ms;
     e->stats.ms_admit += ms;
     if (rc) return rc;
-    if (e->n_resident == 0) return 0;
+    if (e->n_resident == 0) return flush_copies(e);
     const int64_t pred = std::max<int64_t>((int64_t)e->last_nwin * 3 / 2, e->n_resident * (int64_t)e->min_window);
     const int G = lanes_group(e, std::min<int64_t>(pred, e->win_limit));
     hipGraphExec_t g;
@@ -1318,7 +1338,11 @@ int vo_engine_launch(vo_engine* e) {
     VO_HIP(hipMemcpyAsync(e->ctl_host, e->ctl, sizeof(BatchCtl), hipMemcpyDeviceToHost, e->s));
     VO_HIP(hipMemcpyAsync(e->cnt_host, e->cnt, sizeof(Counters), hipMemcpyDeviceToHost, e->s));
     e->launched = true;
-    return 0;
+    // the previous boundary's result copies overlap this replay
+    const double tc = now_s();
+    rc = flush_copies(e);
+    e->stats.ms_collect += (now_s() - tc) * 1e3;
+    return rc;
 }
 
 int vo_engine_finish(vo_engine* e, int64_t* n_left) {
@@ -1362,6 +1386,10 @@ int vo_engine_step(vo_engine* e, int64_t* n_left) {
 
 int vo_engine_results(vo_engine* e, vo_set_result* out, int64_t max, int64_t* n_out) {
     if (!e || !n_out || (max > 0 && !out)) return fail(VO_ERR_INVALID, "null argument");
+    if (!e->launched) {               // with a replay in flight, its launch already flushed
+        int rc = flush_copies(e);
+        if (rc) return rc;
+    }
     int64_t k = 0;
     while (k < max && !e->ready.empty()) {
         out[k++] = e->ready.front();
@@ -1426,22 +1454,30 @@ int vo_batch_triangulate(vo_engine* e, const double* pts, const int64_t* set_off
     std::vector<vo_set_result> buf(256);
     int64_t last_progress = e->stats.points_inserted, stall = 0;
     while (left > 0 || have < n_sets) {
-        int rc = vo_engine_step(e, &left);
+        // launch (which starts the previous boundary's result copies beside the replay),
+        // collect what is ready, then finish the replay
+        int rc = vo_engine_launch(e);
         if (rc) return settle(rc);
         int64_t n = 0;
-        do {
-            vo_engine_results(e, buf.data(), (int64_t)buf.size(), &n);
-            for (int64_t i = 0; i < n; ++i) {
-                const int64_t k = buf[i].tag;
-                if (k >= 0 && k < n_sets && !got[k]) {
-                    results[k] = buf[i];
-                    got[k] = 1;
-                    ++have;
-                } else {
-                    std::free(buf[i].simplices);
+        auto harvest = [&] {
+            do {
+                vo_engine_results(e, buf.data(), (int64_t)buf.size(), &n);
+                for (int64_t i = 0; i < n; ++i) {
+                    const int64_t k = buf[i].tag;
+                    if (k >= 0 && k < n_sets && !got[k]) {
+                        results[k] = buf[i];
+                        got[k] = 1;
+                        ++have;
+                    } else {
+                        std::free(buf[i].simplices);
+                    }
                 }
-            }
-        } while (n == (int64_t)buf.size());
+            } while (n == (int64_t)buf.size());
+        };
+        harvest();
+        rc = vo_engine_finish(e, &left);
+        if (rc) return settle(rc);
+        if (left == 0) harvest();
         // guard: a set waiting for space that nothing can free
         vo_engine_stats sst;
         vo_engine_stats_get(e, &sst);
@@ -1520,16 +1556,23 @@ void server_loop(vo_server* s) {
         }
         for (vo_engine* e : s->eng)
             if (!rc && (!e->queue.empty() || e->n_resident > 0)) rc = vo_engine_launch(e);
+        // results of the previous pass: their copies ran beside the replays just launched
+        std::vector<vo_set_result> got;
+        auto harvest = [&] {
+            for (vo_engine* e : s->eng) {
+                int64_t n = 0;
+                do {
+                    if (vo_engine_results(e, buf.data(), (int64_t)buf.size(), &n) && !rc) rc = VO_ERR_OOM;
+                    got.insert(got.end(), buf.begin(), buf.begin() + n);
+                } while (n == (int64_t)buf.size());
+            }
+        };
+        harvest();
         for (vo_engine* e : s->eng)
             if (!rc) rc = vo_engine_finish(e, nullptr);
-        std::vector<vo_set_result> got;
-        for (vo_engine* e : s->eng) {
-            int64_t n = 0;
-            do {
-                vo_engine_results(e, buf.data(), (int64_t)buf.size(), &n);
-                got.insert(got.end(), buf.begin(), buf.begin() + n);
-            } while (n == (int64_t)buf.size());
-        }
+        bool idle = true;
+        for (vo_engine* e : s->eng) idle &= e->queue.empty() && e->n_resident == 0;
+        if (idle || rc) harvest();          // nothing will launch next: copy now
         {
             std::lock_guard<std::mutex> lk(s->mu);
             for (auto& r : got) s->outbox.push_back(r);
