@@ -104,7 +104,9 @@ int vo_tri_stats(const vo_tri* t, vo_stats* out);
  * size so that this fraction of it wins (smaller windows aim at 0.35 .. 0.6).  "compact_ratio_milli" (default
  * 0 = never; measured no faster at n = 4M: every visit misses the caches either way): between rounds, when the record array holds more than this many records (x 1000) per live
  * simplex, the live ones move to a dense array in spatial order; "compact_min" (default 2^20): not before this
- * many records exist. */
+ * many records exist.  "xcd_regions" (0 default / 1): each round's window is sorted into 8 spatial regions and
+ * block b of the round kernels serves region b % 8, so that each of the 8 XCDs (blocks are dispatched to them
+ * in turn) caches one region's records; measured no faster at n = 4M, kept as an option. */
 int vo_tri_config(vo_tri* t, const char* key, int64_t value);
 
 /* The precision tiers one predicate at a time (no counterpart in the reference, whose in_sphere

@@ -35,3 +35,20 @@ def test_version_and_error_without_gpu():
     rc = L.vo_tri_create(0, 4, None, 0, ctypes.byref(out))
     assert rc == _lib.VO_ERR_INVALID and not out.value
     assert b"dim" in L.vo_last_error()
+
+
+def test_result_canonical_is_host_only_and_matches_the_oracle():
+    # vo_result_canonical needs no GPU: canonical T / E of a simplex list in the reference id
+    # space, checked against the oracle's own canonicalizer on random tuples
+    import numpy as np
+    from oracle import oracle as O
+    from voronoids_amd import batch as B
+    rng = np.random.default_rng(5)
+    for dim in (2, 3):
+        M = dim + 1
+        S = rng.integers(0, 2 * M + 40, size=(200, M)).astype(np.int32)
+        S = S[np.array([len(set(r)) == M for r in S])]
+        S = np.unique(np.sort(S, axis=1), axis=0)            # distinct simplices, as a triangulation has
+        T, E = B.canonical_from_simplices(S, dim)
+        ref = O.canonical_from_real(np.where(S >= 2 * M, S - 2 * M, -1), dim)
+        assert np.array_equal(T, ref.T) and np.array_equal(E, ref.E)

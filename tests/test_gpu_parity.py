@@ -198,3 +198,19 @@ def test_compaction_keeps_the_triangulation(dim):
     assert t.violations() == 0
     nv, ns, mx = t.counts()
     assert ns == (t.export()[0].shape[0]) and mx == st["created"] + dim + 1 - 1 + 1
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_xcd_region_windows_keep_the_triangulation(dim):
+    # windows sorted into 8 spatial regions, block b serving region b % 8, with frequent
+    # compactions: same triangulation as the oracle's
+    P = datasets.uniform(60000, dim, seed=130 + dim)
+    t = vo.DelaunayTree(P)
+    t.config("xcd_regions", 1)
+    t.config("compact_ratio_milli", 1500)
+    t.config("compact_min", 256)
+    t.add_points_to_tree(P, seed=3)
+    T, E = t.canonical()
+    ref = O.Tree.delaunay(P, O.MODE_EXACT).canonical()
+    assert np.array_equal(T, ref.T) and np.array_equal(E, ref.E)
+    assert t.violations() == 0 and t.stats()["compactions"] > 0

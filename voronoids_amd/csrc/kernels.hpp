@@ -119,7 +119,26 @@ struct Ctx {
     const BatchCtl* ctl;   // window size and key of the round
     int big_cap;           // capacity of the big list
     int pend_cap;          // capacity of the pending list
+    const int* region;     // XCD regions (null: off): [0..7] first slot, [8..15] slot count of region r
 };
+
+// Window slots served by this block.  With XCD regions the window is sorted by region (the
+// octant of the point, the top three bits of the Morton order that compaction sorts records
+// by) and block b serves region b % 8 only: blocks are dispatched to the eight XCDs in turn,
+// so each XCD's L2 and translation caches see one region's records.  Iterate with
+//   for (kb = it.base; kb < it.lim; kb += it.stride) { k = kb + part; valid = k < it.lim;
+//                                                    slot = it.off + k; ... }
+struct SlotIter {
+    int base, stride, lim, off;
+};
+__device__ __forceinline__ SlotIter slot_iter(const Ctx& c, int nwin, int per_block) {
+    if (c.region) {
+        const int r = blockIdx.x & 7;
+        return SlotIter{(int)(blockIdx.x >> 3) * per_block, (int)(gridDim.x >> 3) * per_block, c.region[8 + r],
+                        c.region[r]};
+    }
+    return SlotIter{(int)blockIdx.x * per_block, (int)gridDim.x * per_block, nwin, 0};
+}
 
 // window size and reservation key of the round: kernel arguments with one set, planned on
 // the device with several
@@ -561,9 +580,10 @@ __global__ void __launch_bounds__(256) k_cavity(Ctx c, int nwin, unsigned long l
     const int lt = threadIdx.x;
     LocalCnt lc = {};
     round_params(c, nwin, key_hi);
-    for (int wbase = blockIdx.x * blockDim.x; wbase < nwin; wbase += gridDim.x * blockDim.x) {
-    const int slot = wbase + threadIdx.x;
-    if (slot < nwin) {
+    const SlotIter it = slot_iter(c, nwin, blockDim.x);
+    for (int kb = it.base; kb < it.lim; kb += it.stride) {
+    const int slot = it.off + kb + threadIdx.x;
+    if (kb + (int)threadIdx.x < it.lim) {
         const int p = c.win[slot];
         const int pv = c.M + p;
         const typename V::F pf = V::ld(c.X32, pv);
@@ -691,12 +711,14 @@ __global__ void __launch_bounds__(256) k_commit(Ctx c, int nwin, unsigned long l
     constexpr int M = D + 1;
     LocalCnt lc = {};
     round_params(c, nwin, key_hi);
-    for (int wbase = blockIdx.x * blockDim.x; wbase < nwin; wbase += gridDim.x * blockDim.x) {
-    const int slot = wbase + threadIdx.x;
+    const SlotIter it = slot_iter(c, nwin, blockDim.x);
+    for (int kb = it.base; kb < it.lim; kb += it.stride) {
+    const int slot = it.off + kb + threadIdx.x;
+    const bool valid = kb + (int)threadIdx.x < it.lim;
     int st = ST_IDLE, p = 0, nk = 0, nf = 0;
     bool win = false;
     unsigned long long key = 0;
-    if (slot < nwin) {
+    if (valid) {
         st = c.status[slot];
         p = c.win[slot];
         key = key_hi | (unsigned)p;

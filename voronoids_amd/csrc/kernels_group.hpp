@@ -89,9 +89,11 @@ __global__ void __launch_bounds__(64, VO_CAVITY_WAVES) k_cavity_g(Ctx c, int nwi
     const bool lead = g.gl == 0;
     LocalCnt lc = {};
     round_params(c, nwin, key_hi);
-    for (int wbase = blockIdx.x * NG; wbase < nwin; wbase += gridDim.x * NG) {
-    const int slot = wbase + g.gid;
-    int st = slot < nwin ? ST_OK : ST_IDLE;
+    const SlotIter it = slot_iter(c, nwin, NG);
+    for (int kb = it.base; kb < it.lim; kb += it.stride) {
+    const int slot = it.off + kb + g.gid;
+    const bool valid = kb + g.gid < it.lim;
+    int st = valid ? ST_OK : ST_IDLE;
     int p = 0, pv = 0, t = 0;
     F pf{};
     int4 tt = make_int4(0, 0, 0, 0);
@@ -241,7 +243,7 @@ __global__ void __launch_bounds__(64, VO_CAVITY_WAVES) k_cavity_g(Ctx c, int nwi
             c.cav[(size_t)slot * c.kcap + (j)] = x;
         }
     }
-    if (slot < nwin && lead) {
+    if (valid && lead) {
         c.nk[slot] = nk;
         c.nf[slot] = nf;
         c.status[slot] = st;
@@ -264,12 +266,14 @@ __global__ void __launch_bounds__(64) k_commit_g(Ctx c, int nwin, unsigned long 
     const bool lead = g.gl == 0;
     LocalCnt lc = {};
     round_params(c, nwin, key_hi);
-    for (int wbase = blockIdx.x * NG; wbase < nwin; wbase += gridDim.x * NG) {
-    const int slot = wbase + g.gid;
+    const SlotIter it = slot_iter(c, nwin, NG);
+    for (int kb = it.base; kb < it.lim; kb += it.stride) {
+    const int slot = it.off + kb + g.gid;
+    const bool valid = kb + g.gid < it.lim;
     int st = ST_IDLE, p = 0, nk = 0, nf = 0;
     unsigned long long key = 0;
     bool ok = true;
-    if (slot < nwin) {
+    if (valid) {
         st = c.status[slot];
         p = c.win[slot];
         key = key_hi | (unsigned)p;

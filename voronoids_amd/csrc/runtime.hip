@@ -93,6 +93,10 @@ struct vo_tri {
     int64_t compact_min = 1 << 20;     // ... and at least this many records are allocated
     int64_t n_compact = 0;
     double ms_compact = 0;
+    int xcd = 0;                       // window sorted into 8 spatial regions, one per XCD
+    int* reg_meta = nullptr;           // [0..7] first slot, [8..15] count, [16..23] scatter cursors
+    int* win2 = nullptr;               // the window in region order
+    unsigned char* regs = nullptr;     // region of each window slot
     int64_t tcap = 0;
     int n_tets = 1;
     int64_t n_live = 1;
@@ -179,6 +183,7 @@ Ctx make_ctx(vo_tri* t) {
     c.ctl = nullptr;
     c.big_cap = t->win_cap;
     c.pend_cap = t->win_cap;
+    c.region = nullptr;
     return c;
 }
 
@@ -187,6 +192,49 @@ __global__ void k_window(int* win, const int* pend, int npend, int head, int nfr
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < npend) win[i] = pend[i];
     else if (i < npend + nfresh) win[i] = head + (i - npend);
+}
+
+// region of point p: the top three bits of its Morton code over the hint grid's box (3D: the
+// octant; 2D: y quarter and x half), the same bits that lead compaction's record order
+template <int D> __device__ __forceinline__ int point_region(const Ctx& c, int p) {
+    const typename VT<D>::F x = VT<D>::ld(c.X32, c.M + p);
+    const float ux = (x.x - c.glo[0]) * c.ginv, uy = (x.y - c.glo[1]) * c.ginv;
+    if constexpr (D == 3) {
+        const float uz = (x.z - c.glo[2]) * c.ginv;
+        return ((uz >= 0.5f) << 2) | ((uy >= 0.5f) << 1) | (ux >= 0.5f);
+    } else {
+        const int qy = (int)fminf(fmaxf(uy * 4.0f, 0.0f), 3.0f);
+        return ((qy >> 1) << 2) | ((ux >= 0.5f) << 1) | (qy & 1);
+    }
+}
+
+template <int D>
+__global__ void __launch_bounds__(256) k_region_count(Ctx c, const int* win, int nwin, unsigned char* regs, int* meta) {
+    __shared__ int s_n[8];
+    if (threadIdx.x < 8) s_n[threadIdx.x] = 0;
+    __syncthreads();
+    for (int i = blockIdx.x * 256 + threadIdx.x; i < nwin; i += gridDim.x * 256) {
+        const int r = point_region<D>(c, win[i]);
+        regs[i] = (unsigned char)r;
+        atomicAdd(&s_n[r], 1);
+    }
+    __syncthreads();
+    if (threadIdx.x < 8 && s_n[threadIdx.x]) atomicAdd(&meta[8 + threadIdx.x], s_n[threadIdx.x]);
+}
+
+__global__ void k_region_scan(int* meta) {
+    int acc = 0;
+    for (int r = 0; r < 8; ++r) {
+        meta[r] = acc;
+        meta[16 + r] = acc;
+        acc += meta[8 + r];
+    }
+}
+
+__global__ void __launch_bounds__(256) k_region_scatter(const int* win, int nwin, const unsigned char* regs, int* meta,
+                                                         int* win2) {
+    for (int i = blockIdx.x * 256 + threadIdx.x; i < nwin; i += gridDim.x * 256)
+        win2[atomicAdd(&meta[16 + regs[i]], 1)] = win[i];
 }
 
 template <int D>
@@ -300,7 +348,8 @@ int upload_super(vo_tri* t) {
 }
 
 void free_all(vo_tri* t) {
-    void* ptrs[] = {t->X64, t->X32, t->cp, t->vmap, t->rec, t->rec_spare, t->cmap, t->chist, t->win, t->pend[0], t->pend[1],
+    void* ptrs[] = {t->X64, t->X32, t->cp, t->vmap, t->rec, t->rec_spare, t->cmap, t->chist, t->reg_meta, t->win2,
+                    t->regs, t->win, t->pend[0], t->pend[1],
                     t->cav, t->fac, t->nk, t->nf, t->status, t->hint, t->st, t->cnt, t->q, t->cache, t->ws,
                     t->big, t->cav1, t->fac1, t->misc1};
     for (void* p : ptrs)
@@ -468,7 +517,7 @@ int compact(vo_tri* t) {
 
 #define VO_LAUNCH_G(DIM, GL)                                                                     \
     do {                                                                                         \
-        const dim3 grid_((nwin + 64 / GL - 1) / (64 / GL));                                      \
+        const dim3 grid_(blocks(64 / GL));                                                       \
         if (which) hipLaunchKernelGGL((k_commit_g<DIM, GL>), grid_, dim3(64), 0, t->s, c, nwin, key_hi); \
         else hipLaunchKernelGGL((k_cavity_g<DIM, GL>), grid_, dim3(64), 0, t->s, c, nwin, key_hi);       \
     } while (0)
@@ -489,8 +538,13 @@ int auto_group(const vo_tri* t, int nwin) {
 
 int launch_round(vo_tri* t, const Ctx& c, int nwin, unsigned long long key_hi, int which) {
     const int G = auto_group(t, nwin);
+    // with regions, a multiple of 8 blocks and slack for unequal region sizes
+    auto blocks = [&](int per_block) {
+        const int b = (nwin + per_block - 1) / per_block;
+        return c.region ? ((b + b / 8 + 8 + 7) & ~7) : b;
+    };
     if (G == 1) {
-        const int grid = (nwin + 255) / 256;
+        const int grid = blocks(256);
         if (t->D == 3) {
             if (which) hipLaunchKernelGGL(k_commit<3>, dim3(grid), dim3(256), 0, t->s, c, nwin, key_hi);
             else hipLaunchKernelGGL(k_cavity<3>, dim3(grid), dim3(256), 0, t->s, c, nwin, key_hi);
@@ -788,6 +842,19 @@ int vo_tri_run(vo_tri* t) {
         hipLaunchKernelGGL(k_reset_round, dim3(1), dim3(1), 0, t->s, t->st);
         const unsigned long long key_hi = (unsigned long long)(0xFFFFFFFFu - (uint32_t)t->rounds) << 32;
         Ctx c = make_ctx(t);
+        if (t->xcd) {
+            if (!t->reg_meta && ((rc = dev_alloc(&t->reg_meta, 24)) || (rc = dev_alloc(&t->win2, t->win_cap)) ||
+                                 (rc = dev_alloc(&t->regs, t->win_cap))))
+                return rc;
+            VO_HIP(hipMemsetAsync(t->reg_meta, 0, sizeof(int) * 24, t->s));
+            const int rb = std::max(1, std::min(1024, (nwin + 255) / 256));
+            if (D == 3) hipLaunchKernelGGL(k_region_count<3>, dim3(rb), dim3(256), 0, t->s, c, t->win, nwin, t->regs, t->reg_meta);
+            else hipLaunchKernelGGL(k_region_count<2>, dim3(rb), dim3(256), 0, t->s, c, t->win, nwin, t->regs, t->reg_meta);
+            hipLaunchKernelGGL(k_region_scan, dim3(1), dim3(1), 0, t->s, t->reg_meta);
+            hipLaunchKernelGGL(k_region_scatter, dim3(rb), dim3(256), 0, t->s, t->win, nwin, t->regs, t->reg_meta, t->win2);
+            c.win = t->win2;
+            c.region = t->reg_meta;
+        }
         VO_HIP(hipEventRecord(t->ev[2], t->s));
         if ((rc = launch_round(t, c, nwin, key_hi, 0))) return rc;
         VO_HIP(hipEventRecord(t->ev[3], t->s));
@@ -1024,6 +1091,9 @@ int vo_tri_config(vo_tri* t, const char* key, int64_t value) {
     } else if (k == "big_frac_lo_milli" || k == "big_frac_hi_milli") {
         if (value <= 0 || value >= 1000) return fail(VO_ERR_INVALID, k + " must be in 1..999");
         (k == "big_frac_lo_milli" ? t->big_frac_lo : t->big_frac_hi) = (double)value / 1000.0;
+    } else if (k == "xcd_regions") {
+        if (value != 0 && value != 1) return fail(VO_ERR_INVALID, "xcd_regions must be 0 or 1");
+        t->xcd = (int)value;
     } else if (k == "compact_min") {
         if (value < 16) return fail(VO_ERR_INVALID, "compact_min must be >= 16");
         t->compact_min = value;
