@@ -823,7 +823,7 @@ int admit(vo_engine* e) {
                            (long long)(sizeof(CEntry) / 8) * ((long long)e->cmask + 1), ~0ull);
         e->cache_stale = false;
     }
-    const int D = e->D, M = e->M;
+    const int D = e->D;
     int64_t npts = 0;
     for (HSet* h : adm) npts += h->n;
     const size_t desc_b = sizeof(AdmitDesc) * adm.size();
@@ -887,7 +887,6 @@ int admit(vo_engine* e) {
     VO_HIP(hipStreamSynchronize(e->s));
     e->stats.sets_admitted += nd;
     e->stats.max_sets_resident = std::max<int64_t>(e->stats.max_sets_resident, e->n_resident);
-    (void)M;
     return 0;
 }
 
