@@ -214,3 +214,52 @@ def test_xcd_region_windows_keep_the_triangulation(dim):
     ref = O.Tree.delaunay(P, O.MODE_EXACT).canonical()
     assert np.array_equal(T, ref.T) and np.array_equal(E, ref.E)
     assert t.violations() == 0 and t.stats()["compactions"] > 0
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_device_planned_rounds_match_host_planned(dim):
+    # the device planner applies the host's window rule: the same rounds and windows and the
+    # same triangulation for any number of rounds per chunk, and a second insert continues
+    # from the planner's state
+    P = datasets.uniform(120000, dim, seed=140 + dim)
+    Q = datasets.uniform(20000, dim, seed=150 + dim) * 0.5 + 0.25
+    ref = O.Tree.delaunay(np.vstack([P, Q]), O.MODE_EXACT).canonical()
+    seen = []
+    for pipelined, chunk in ((0, 4), (1, 1), (1, 4), (1, 7)):
+        t = vo.DelaunayTree(np.vstack([P, Q]))
+        t.config("pipelined", pipelined)
+        t.config("chunk_rounds", chunk)
+        t.add_points_to_tree(P, seed=8)
+        t.add_points_to_tree(Q, seed=9)
+        T, E = t.canonical()
+        assert np.array_equal(T, ref.T) and np.array_equal(E, ref.E), (pipelined, chunk)
+        st = t.stats()
+        seen.append((st["rounds"], st["points_inserted"], st["windows_total"], st["created"], st["killed"],
+                     st["launches_locate_cavity"]))
+        assert t.violations() == 0
+    assert all(x == seen[0] for x in seen), seen
+
+
+def test_device_planned_rounds_hand_over_exact_and_solo_rounds():
+    # points rounded onto a sphere / circle: rounds queue exact predicates and some cavities
+    # outgrow the per-point LDS lists (solo runs); the planner halts on each such round and the
+    # host finishes it, with the host loop's rounds, windows and result
+    rng = np.random.default_rng(160)
+    solo = 0
+    for dim in (3, 2):
+        u = rng.normal(size=(500, dim))
+        P = np.vstack([0.5 + 0.3 * u / np.linalg.norm(u, axis=1)[:, None], 0.5 + 0.05 * rng.random((60, dim))])
+        ref = O.Tree.delaunay(P, O.MODE_EXACT).canonical()
+        seen = []
+        for pipelined in (0, 1):
+            t = vo.DelaunayTree(P)
+            t.config("pipelined", pipelined)
+            t.add_points_to_tree(P, seed=11)
+            T, E = t.canonical()
+            assert np.array_equal(T, ref.T) and np.array_equal(E, ref.E), (dim, pipelined)
+            st = t.stats()
+            assert st["exact_evaluated"] > 0
+            seen.append((st["rounds"], st["points_inserted"], st["windows_total"], st["solo_runs"]))
+        assert seen[0] == seen[1], (dim, seen)
+        solo += seen[1][3]
+    assert solo > 0

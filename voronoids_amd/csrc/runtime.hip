@@ -70,6 +70,22 @@ int canonical_from_simplices(int M, const int32_t* s, int64_t nl, int32_t** T, i
 }  // namespace vo_host
 
 
+// Round planning on the device for one triangulation: after each round a one-thread kernel
+// absorbs the round's outcome (the host's window rule, the same arithmetic) and sizes the next
+// window, so that rounds run back to back while the host only watches snapshots.  A round that
+// needs the host (points re-run alone, queued exact predicates, an error) or more simplex
+// space halts the planner; the rounds after a halt are empty until the host takes over.
+enum : int { HALT_HOST = 1, HALT_SPACE = 2, HALT_DONE = 4, HALT_STALL = 8 };
+struct RunPlan {
+    BatchCtl ctl;              // nwin / key_hi of the planned round (read by the round kernels)
+    long long inserted, dups, windows, n_live, tet_cap;
+    double gamma, big_frac_lo, big_frac_hi;
+    int head, n_pts, npend, nfresh, nwin, rounds, stalled, cur;
+    int launched;              // a planned round ran and is not absorbed yet
+    int halt;                  // HALT_* bits
+    int win_cap, big_window, per_point;
+};
+
 struct vo_tri {
     int dev = 0, D = 3, M = 4, S = 4;
     hipStream_t s = nullptr;
@@ -132,6 +148,15 @@ struct vo_tri {
     double* ws = nullptr;
     int ws_cap = 0, ws_threads = 0;
     hipEvent_t ev[6] = {};
+    // device-planned rounds: planner state, pinned snapshots (one per chunk in flight) and
+    // per-round timing events (cavity start, commit start, commit end)
+    RunPlan* plan = nullptr;
+    RunPlan* plan_host = nullptr;
+    std::vector<hipEvent_t> rev;
+    hipEvent_t chunk_ev[2] = {};
+    int pipelined = 1;                 // 0: the host plans every round and waits for it
+    int chunk_rounds = 4;              // rounds submitted per chunk
+    int64_t stalled = 0;
     double gamma = 0.25;
     int filters = 3;
     int64_t big_window = 0;            // windows at least this large target big_frac_lo..hi winners (0: off)
@@ -273,6 +298,72 @@ __global__ void k_reset_round(RoundState* st) {
     st->n_big = 0;
 }
 
+// Absorb the last planned round and plan the next one (one thread).  Absorbing follows the
+// host's rule in finish_round: duplicates are the window points neither inserted nor sent
+// on; the window factor gamma grows when more than the target winning fraction won and
+// shrinks when fewer did; the next window is gamma x inserted points (at least 8, at most the
+// window capacity), the pending points first.
+__global__ void k_plan_one(RunPlan* P, RoundState* st) {
+    RunPlan& p = *P;
+    p.ctl.nwin = 0;
+    if (p.launched) {
+        if (st->n_big > 0 || st->n_q > 0 || st->err) {
+            p.halt |= HALT_HOST;
+            return;
+        }
+        const int nwin = p.nwin, w = st->n_winners, nx = st->n_next_pending;
+        const int consumed = nwin - nx - w;
+        p.dups += consumed;
+        p.inserted += w;
+        p.n_live += st->n_live_delta;
+        p.windows += nwin;
+        const double frac = nwin ? (double)w / nwin : 1.0;
+        const bool big = p.big_window > 0 && nwin >= p.big_window;
+        const double hi = big ? p.big_frac_hi : 0.6, lo = big ? p.big_frac_lo : 0.35;
+        if (frac > hi) p.gamma = fmin(p.gamma * 1.5, 4.0);
+        else if (frac < lo) p.gamma = fmax(p.gamma * 0.7, 0.002);
+        p.stalled = (w == 0 && consumed == 0) ? p.stalled + 1 : 0;
+        p.npend = nx;
+        p.cur ^= 1;
+        p.head += p.nfresh;
+        p.rounds++;
+        p.launched = 0;
+        if (p.stalled > 64) p.halt |= HALT_STALL;
+    }
+    if (p.halt) return;
+    if (p.head >= p.n_pts && p.npend == 0) {
+        p.halt |= HALT_DONE;
+        return;
+    }
+    const double target = fmax(8.0, p.gamma * (double)(p.inserted > 1 ? p.inserted : 1));
+    int nwin = (int)fmin(target, (double)p.win_cap);
+    if (p.npend > nwin) nwin = p.npend;
+    const int nfresh = min(p.n_pts - p.head, nwin - p.npend);
+    nwin = p.npend + nfresh;
+    if ((long long)st->n_tets + (long long)nwin * p.per_point + 1024 > p.tet_cap) {
+        p.halt |= HALT_SPACE;
+        return;
+    }
+    p.nfresh = nfresh;
+    p.nwin = nwin;
+    p.ctl.nwin = nwin;
+    p.ctl.npend = p.npend;
+    p.ctl.key_hi = (unsigned long long)(0xFFFFFFFFu - (uint32_t)p.rounds) << 32;
+    st->n_next_pending = 0;
+    st->n_winners = 0;
+    st->n_q = 0;
+    st->n_live_delta = 0;
+    st->n_big = 0;
+    p.launched = 1;
+}
+
+// the planned window: pending points, then fresh points head .. head + nfresh - 1
+__global__ void k_fill_one(const RunPlan* P, int* win, const int* pend) {
+    const int nwin = P->ctl.nwin, np = P->ctl.npend, head = P->head;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nwin; i += gridDim.x * blockDim.x)
+        win[i] = i < np ? pend[i] : head + (i - np);
+}
+
 
 
 // grow the simplex records to cap (copying the first n_tets); new records are all 0xFF bytes:
@@ -351,11 +442,16 @@ void free_all(vo_tri* t) {
     void* ptrs[] = {t->X64, t->X32, t->cp, t->vmap, t->rec, t->rec_spare, t->cmap, t->chist, t->reg_meta, t->win2,
                     t->regs, t->win, t->pend[0], t->pend[1],
                     t->cav, t->fac, t->nk, t->nf, t->status, t->hint, t->st, t->cnt, t->q, t->cache, t->ws,
-                    t->big, t->cav1, t->fac1, t->misc1};
+                    t->big, t->cav1, t->fac1, t->misc1, t->plan};
     for (void* p : ptrs)
         if (p) hipFree(p);
     if (t->st_host) hipHostFree(t->st_host);
+    if (t->plan_host) hipHostFree(t->plan_host);
     for (auto& e : t->ev)
+        if (e) hipEventDestroy(e);
+    for (auto& e : t->rev)
+        if (e) hipEventDestroy(e);
+    for (auto& e : t->chunk_ev)
         if (e) hipEventDestroy(e);
     if (t->s) hipStreamDestroy(t->s);
 }
@@ -536,8 +632,10 @@ int auto_group(const vo_tri* t, int nwin) {
     return 1;
 }
 
-int launch_round(vo_tri* t, const Ctx& c, int nwin, unsigned long long key_hi, int which) {
-    const int G = auto_group(t, nwin);
+// nwin sizes the grid (the kernels read the exact window from c.ctl when it is set); G = 0:
+// lanes per point chosen from nwin
+int launch_round(vo_tri* t, const Ctx& c, int nwin, unsigned long long key_hi, int which, int G = 0) {
+    if (!G) G = auto_group(t, nwin);
     // with regions, a multiple of 8 blocks and slack for unequal region sizes
     auto blocks = [&](int per_block) {
         const int b = (nwin + per_block - 1) / per_block;
@@ -626,6 +724,263 @@ __global__ void __launch_bounds__(64) k_eval_predicates(int type, const double* 
         tier[i] = t;
     }
 }
+
+namespace {
+
+// simplex records for n_tets + need, growing by at least half (grow_tets copies the live part)
+int ensure_records(vo_tri* t, int64_t need) {
+    if (need <= t->tcap) return 0;
+    const int64_t cap_max = (int64_t)1 << 29;
+    return grow_tets(t, std::max(need, std::min(t->tcap * 3 / 2, cap_max)));
+}
+
+// records a window point may create at most, as the planners reserve them
+int64_t window_records(int D) { return D == 3 ? 64 : 16; }
+
+// The host's side of a finished round whose state is in st_host: errors, points re-run alone,
+// the exact queue, then the counters and the window factor (k_plan_one applies the same rule
+// on the device).
+int finish_round(vo_tri* t, int nwin, int nfresh, FILE* trace, float ms_cav, float ms_com) {
+    int rc;
+    if ((rc = check_err(t))) return rc;
+    const int n_big = t->st_host->n_big;
+    if (n_big > 0 && (rc = run_solo(t, n_big))) return rc;
+    const RoundState& r = *t->st_host;
+    if (r.n_q > 0) {
+        if ((rc = run_exact(t, r.n_q))) return rc;
+        if ((rc = sync_state(t)) || (rc = check_err(t))) return rc;
+    }
+    t->n_tets = r.n_tets;
+    t->n_live += r.n_live_delta;
+    t->inserted += r.n_winners;
+    t->windows_total += nwin;
+    if (trace) {
+        Counters h;
+        hipMemcpy(&h, t->cnt, sizeof(h), hipMemcpyDeviceToHost);
+        std::fprintf(trace, "%lld %d %d %d %d %.4f %.4f %d %lld %d %d %llu %llu %llu %llu\n", (long long)t->rounds, nwin,
+                     t->npend, nfresh, r.n_winners, ms_cav, ms_com, r.n_q, (long long)t->inserted, n_big,
+                     r.n_next_pending, h.walk_steps, h.chain_hops, h.killed, h.created);
+    }
+    const int64_t consumed = (int64_t)nwin - r.n_next_pending - r.n_winners;   // duplicates
+    t->dups += consumed;
+    const double frac = nwin ? (double)r.n_winners / nwin : 1.0;
+    // keep about half of the window winning; windows past big_window are throughput-bound
+    // (a loser's cavity is wasted work, a round costs little beyond its points), so there
+    // the target winning fraction is higher
+    const bool big = t->big_window > 0 && nwin >= t->big_window;
+    const double hi = big ? t->big_frac_hi : 0.6, lo = big ? t->big_frac_lo : 0.35;
+    if (frac > hi) t->gamma = std::min(t->gamma * 1.5, 4.0);
+    else if (frac < lo) t->gamma = std::max(t->gamma * 0.7, 0.002);
+    // every round either inserts, drops a duplicate, or resolves queued predicates
+    t->stalled = (r.n_winners == 0 && consumed == 0 && r.n_q == 0) ? t->stalled + 1 : 0;
+    if (t->stalled > 64) return fail(VO_ERR_INTERNAL, "no progress for 64 rounds");
+    t->npend = r.n_next_pending;
+    t->cur ^= 1;
+    t->head += nfresh;
+    t->rounds++;
+    return 0;
+}
+
+// window of the next round by the host's rule: (nwin, nfresh)
+std::pair<int, int> host_window(const vo_tri* t) {
+    const double target = std::max(8.0, t->gamma * (double)std::max<int64_t>(t->inserted, 1));
+    int nwin = (int)std::min<double>(target, t->win_cap);
+    nwin = std::max(nwin, t->npend);
+    const int nfresh = (int)std::min<int64_t>(t->n_pts - t->head, nwin - t->npend);
+    return {t->npend + nfresh, nfresh};
+}
+
+// Rounds planned on the host, one at a time: plan, launch, wait for the round's state, finish.
+int run_host_planned(vo_tri* t, FILE* trace) {
+    const int D = t->D;
+    int rc = 0;
+    while (t->head < t->n_pts || t->npend > 0) {
+        if (t->npend > t->win_cap) return fail(VO_ERR_INTERNAL, "pending list beyond the window capacity");
+        const auto [nwin, nfresh] = host_window(t);
+        if ((rc = ensure_records(t, (int64_t)t->n_tets + (int64_t)nwin * window_records(D) + 1024))) return rc;
+        const int grid = (nwin + 255) / 256;
+        hipLaunchKernelGGL(k_window, dim3(grid), dim3(256), 0, t->s, t->win, t->pend[t->cur], t->npend, (int)t->head,
+                           nfresh);
+        hipLaunchKernelGGL(k_reset_round, dim3(1), dim3(1), 0, t->s, t->st);
+        const unsigned long long key_hi = (unsigned long long)(0xFFFFFFFFu - (uint32_t)t->rounds) << 32;
+        Ctx c = make_ctx(t);
+        if (t->xcd) {
+            if (!t->reg_meta && ((rc = dev_alloc(&t->reg_meta, 24)) || (rc = dev_alloc(&t->win2, t->win_cap)) ||
+                                 (rc = dev_alloc(&t->regs, t->win_cap))))
+                return rc;
+            VO_HIP(hipMemsetAsync(t->reg_meta, 0, sizeof(int) * 24, t->s));
+            const int rb = std::max(1, std::min(1024, (nwin + 255) / 256));
+            if (D == 3) hipLaunchKernelGGL(k_region_count<3>, dim3(rb), dim3(256), 0, t->s, c, t->win, nwin, t->regs, t->reg_meta);
+            else hipLaunchKernelGGL(k_region_count<2>, dim3(rb), dim3(256), 0, t->s, c, t->win, nwin, t->regs, t->reg_meta);
+            hipLaunchKernelGGL(k_region_scan, dim3(1), dim3(1), 0, t->s, t->reg_meta);
+            hipLaunchKernelGGL(k_region_scatter, dim3(rb), dim3(256), 0, t->s, t->win, nwin, t->regs, t->reg_meta, t->win2);
+            c.win = t->win2;
+            c.region = t->reg_meta;
+        }
+        VO_HIP(hipEventRecord(t->ev[2], t->s));
+        if ((rc = launch_round(t, c, nwin, key_hi, 0))) return rc;
+        VO_HIP(hipEventRecord(t->ev[3], t->s));
+        if ((rc = launch_round(t, c, nwin, key_hi, 1))) return rc;
+        VO_HIP(hipEventRecord(t->ev[1], t->s));
+        if ((rc = sync_state(t))) return rc;
+        float a = 0, b = 0;
+        hipEventElapsedTime(&a, t->ev[2], t->ev[3]);
+        hipEventElapsedTime(&b, t->ev[3], t->ev[1]);
+        t->ms_cav += a;
+        t->ms_com += b;
+        t->n_cav++;
+        t->n_com++;
+        if ((rc = finish_round(t, nwin, nfresh, trace, a, b))) return rc;
+        if (t->compact_ratio > 0 && t->n_tets >= t->compact_min && (double)t->n_tets > t->compact_ratio * (double)t->n_live &&
+            (rc = compact(t)))
+            return rc;
+    }
+    return 0;
+}
+
+// the device planner's state from the host's fields (nothing launched, no halt)
+RunPlan plan_from_host(const vo_tri* t) {
+    RunPlan p = {};
+    p.inserted = t->inserted;
+    p.dups = t->dups;
+    p.windows = t->windows_total;
+    p.n_live = t->n_live;
+    p.tet_cap = t->tcap;
+    p.gamma = t->gamma;
+    p.big_frac_lo = t->big_frac_lo;
+    p.big_frac_hi = t->big_frac_hi;
+    p.head = (int)t->head;
+    p.n_pts = (int)t->n_pts;
+    p.npend = t->npend;
+    p.rounds = (int)t->rounds;
+    p.stalled = (int)t->stalled;
+    p.cur = t->cur;
+    p.win_cap = t->win_cap;
+    p.big_window = (int)t->big_window;
+    p.per_point = (int)window_records(t->D);
+    return p;
+}
+
+void plan_to_host(vo_tri* t, const RunPlan& p) {
+    t->inserted = p.inserted;
+    t->dups = p.dups;
+    t->windows_total = p.windows;
+    t->n_live = p.n_live;
+    t->gamma = p.gamma;
+    t->head = p.head;
+    t->npend = p.npend;
+    t->rounds = p.rounds;
+    t->stalled = p.stalled;
+    t->cur = p.cur;
+}
+
+// cavity and commit kernel time of the R rounds of chunk slot `slot`
+void add_chunk_times(vo_tri* t, int slot, int R) {
+    for (int r = 0; r < R; ++r) {
+        hipEvent_t* e = &t->rev[(size_t)(slot * R + r) * 3];
+        float a = 0, b = 0;
+        hipEventElapsedTime(&a, e[0], e[1]);
+        hipEventElapsedTime(&b, e[1], e[2]);
+        t->ms_cav += a;
+        t->ms_com += b;
+    }
+}
+
+// Rounds planned on the device (k_plan_one), submitted in chunks of R with two chunks in
+// flight: the host reads a chunk's planner snapshot while the next chunk runs, and takes over
+// only when the planner halts (points re-run alone, queued exact predicates, an error, no
+// simplex space, or done).  Each round's grid is sized from the snapshot two chunks back,
+// scaled by the growth the window rule allows over that many rounds; the kernels stop at the
+// planned window.
+int run_device_planned(vo_tri* t) {
+    const int D = t->D, R = t->chunk_rounds;
+    int rc;
+    if (!t->plan) {
+        if ((rc = dev_alloc(&t->plan, 1))) return rc;
+        if (hipHostMalloc((void**)&t->plan_host, 2 * sizeof(RunPlan)) != hipSuccess) return fail(VO_ERR_OOM, "pinned");
+        for (auto& e : t->chunk_ev) VO_HIP(hipEventCreate(&e));
+    }
+    while ((int)t->rev.size() < 2 * R * 3) {
+        hipEvent_t e;
+        VO_HIP(hipEventCreate(&e));
+        t->rev.push_back(e);
+    }
+    // one reservation for the staged points (about 27 records per point in 3D, 6 in 2D);
+    // a planner halt grows it when a distribution needs more
+    {
+        const int64_t left = t->n_pts - t->head + t->npend;
+        const int64_t want = std::min<int64_t>((int64_t)t->n_tets + left * (D == 3 ? 30 : 8) + 4096, (int64_t)1 << 29);
+        if ((rc = ensure_records(t, want))) return rc;
+    }
+    while (t->head < t->n_pts || t->npend > 0) {
+        if (t->npend > t->win_cap) return fail(VO_ERR_INTERNAL, "pending list beyond the window capacity");
+        {
+            const int nwin = host_window(t).first;
+            if ((rc = ensure_records(t, (int64_t)t->n_tets + (int64_t)nwin * window_records(D) + 1024))) return rc;
+        }
+        t->plan_host[0] = plan_from_host(t);
+        VO_HIP(hipMemcpyAsync(t->plan, &t->plan_host[0], sizeof(RunPlan), hipMemcpyHostToDevice, t->s));
+        int cur = t->cur;
+        int k = 0, inflight = 0;
+        double base = host_window(t).first, gam = t->gamma;
+        bool halted = false;
+        while (!halted) {
+            // window estimates for the rounds of this chunk: typical (lanes per point) and high (grid)
+            const double grow = 1.0 + 0.5 * gam;
+            const double lag = inflight * R;
+            const int est_mid = (int)std::min<double>(t->win_cap, base * std::pow(grow, lag + 0.5 * R));
+            const int est_hi = (int)std::min<double>(t->win_cap, base * std::pow(grow, lag + R) + 64);
+            const int G = auto_group(t, est_mid);
+            const int slot = k & 1;
+            for (int r = 0; r < R; ++r) {
+                hipLaunchKernelGGL(k_plan_one, dim3(1), dim3(1), 0, t->s, t->plan, t->st);
+                hipLaunchKernelGGL(k_fill_one, dim3(std::max(1, std::min(1024, (est_hi + 255) / 256))), dim3(256), 0, t->s,
+                                   t->plan, t->win, t->pend[cur]);
+                Ctx c = make_ctx(t);
+                c.ctl = &t->plan->ctl;
+                c.pend_next = t->pend[cur ^ 1];
+                hipEvent_t* e = &t->rev[(size_t)(slot * R + r) * 3];
+                VO_HIP(hipEventRecord(e[0], t->s));
+                if ((rc = launch_round(t, c, est_hi, 0, 0, G))) return rc;
+                VO_HIP(hipEventRecord(e[1], t->s));
+                if ((rc = launch_round(t, c, est_hi, 0, 1, G))) return rc;
+                VO_HIP(hipEventRecord(e[2], t->s));
+                cur ^= 1;
+            }
+            VO_HIP(hipMemcpyAsync(&t->plan_host[slot], t->plan, sizeof(RunPlan), hipMemcpyDeviceToHost, t->s));
+            VO_HIP(hipEventRecord(t->chunk_ev[slot], t->s));
+            ++k;
+            if (++inflight < 2) continue;
+            const int hs = (k - 2) & 1;
+            VO_HIP(hipEventSynchronize(t->chunk_ev[hs]));
+            add_chunk_times(t, hs, R);
+            --inflight;
+            const RunPlan& sp = t->plan_host[hs];
+            if (sp.halt) {
+                halted = true;
+            } else {
+                base = std::max<double>(sp.npend, std::min<double>(t->win_cap, std::max(8.0, sp.gamma * (double)std::max<long long>(sp.inserted, 1))));
+                gam = sp.gamma;
+            }
+        }
+        // the newer chunk ends with no-op rounds; its snapshot is the planner's final state
+        VO_HIP(hipStreamSynchronize(t->s));
+        add_chunk_times(t, (k - 1) & 1, R);
+        const RunPlan P = t->plan_host[(k - 1) & 1];
+        if ((rc = sync_state(t))) return rc;
+        // launches counted for the rounds that ran (the empty rounds after a halt are not)
+        t->n_cav += P.rounds - t->rounds + P.launched;
+        t->n_com += P.rounds - t->rounds + P.launched;
+        plan_to_host(t, P);
+        t->n_tets = t->st_host->n_tets;
+        if (P.launched && (rc = finish_round(t, P.nwin, P.nfresh, nullptr, 0.f, 0.f))) return rc;
+        if ((rc = check_err(t))) return rc;
+        if (P.halt & HALT_STALL) return fail(VO_ERR_INTERNAL, "no progress for 64 rounds");
+    }
+    return 0;
+}
+
+}  // namespace
 
 // =================================================================== C ABI
 extern "C" {
@@ -812,104 +1167,21 @@ int vo_tri_stage(vo_tri* t, const double* pts, int64_t n, uint64_t seed) {
 int vo_tri_run(vo_tri* t) {
     if (!t) return fail(VO_ERR_INVALID, "null handle");
     VO_HIP(hipSetDevice(t->dev));
-    const int D = t->D;
     t->ms_cav = t->ms_com = t->ms_ex = t->ms_other = 0;
     t->n_cav = t->n_com = 0;
+    t->stalled = 0;
     const int64_t dups0 = t->dups;
     VO_HIP(hipEventRecord(t->ev[0], t->s));
-    int rc = 0;
-    int stalled = 0;
-    // VO_TRACE=<path>: append one line per round (diagnostics only)
+    // VO_TRACE=<path>: append one line per round (diagnostics only; rounds planned on the host)
     struct TraceFile {
         FILE* f = nullptr;
         ~TraceFile() { if (f) std::fclose(f); }
     } tf;
     if (const char* tp = std::getenv("VO_TRACE")) tf.f = std::fopen(tp, "a");
-    FILE* trace = tf.f;
-    while (t->head < t->n_pts || t->npend > 0) {
-        // window plan
-        const double target = std::max(8.0, t->gamma * (double)std::max<int64_t>(t->inserted, 1));
-        int nwin = (int)std::min<double>(target, t->win_cap);
-        if (t->npend > t->win_cap) return fail(VO_ERR_INTERNAL, "pending list beyond the window capacity");
-        nwin = std::max(nwin, t->npend);
-        const int nfresh = (int)std::min<int64_t>(t->n_pts - t->head, nwin - t->npend);
-        nwin = t->npend + nfresh;
-        const int64_t need = (int64_t)t->n_tets + (int64_t)nwin * (D == 3 ? 64 : 16) + 1024;
-        if (need > t->tcap && (rc = grow_tets(t, std::max<int64_t>(need, t->tcap * 3 / 2)))) return rc;
-        const int grid = (nwin + 255) / 256;
-        hipLaunchKernelGGL(k_window, dim3(grid), dim3(256), 0, t->s, t->win, t->pend[t->cur], t->npend, (int)t->head,
-                           nfresh);
-        hipLaunchKernelGGL(k_reset_round, dim3(1), dim3(1), 0, t->s, t->st);
-        const unsigned long long key_hi = (unsigned long long)(0xFFFFFFFFu - (uint32_t)t->rounds) << 32;
-        Ctx c = make_ctx(t);
-        if (t->xcd) {
-            if (!t->reg_meta && ((rc = dev_alloc(&t->reg_meta, 24)) || (rc = dev_alloc(&t->win2, t->win_cap)) ||
-                                 (rc = dev_alloc(&t->regs, t->win_cap))))
-                return rc;
-            VO_HIP(hipMemsetAsync(t->reg_meta, 0, sizeof(int) * 24, t->s));
-            const int rb = std::max(1, std::min(1024, (nwin + 255) / 256));
-            if (D == 3) hipLaunchKernelGGL(k_region_count<3>, dim3(rb), dim3(256), 0, t->s, c, t->win, nwin, t->regs, t->reg_meta);
-            else hipLaunchKernelGGL(k_region_count<2>, dim3(rb), dim3(256), 0, t->s, c, t->win, nwin, t->regs, t->reg_meta);
-            hipLaunchKernelGGL(k_region_scan, dim3(1), dim3(1), 0, t->s, t->reg_meta);
-            hipLaunchKernelGGL(k_region_scatter, dim3(rb), dim3(256), 0, t->s, t->win, nwin, t->regs, t->reg_meta, t->win2);
-            c.win = t->win2;
-            c.region = t->reg_meta;
-        }
-        VO_HIP(hipEventRecord(t->ev[2], t->s));
-        if ((rc = launch_round(t, c, nwin, key_hi, 0))) return rc;
-        VO_HIP(hipEventRecord(t->ev[3], t->s));
-        if ((rc = launch_round(t, c, nwin, key_hi, 1))) return rc;
-        VO_HIP(hipEventRecord(t->ev[1], t->s));
-        if ((rc = sync_state(t))) return rc;
-        float a = 0, b = 0;
-        hipEventElapsedTime(&a, t->ev[2], t->ev[3]);
-        hipEventElapsedTime(&b, t->ev[3], t->ev[1]);
-        t->ms_cav += a;
-        t->ms_com += b;
-        t->n_cav++;
-        t->n_com++;
-        if ((rc = check_err(t))) return rc;
-        const int n_big = t->st_host->n_big;
-        if (n_big > 0) {
-            if ((rc = run_solo(t, n_big))) return rc;
-        }
-        const RoundState& r = *t->st_host;
-        if (r.n_q > 0) {
-            if ((rc = run_exact(t, r.n_q))) return rc;
-            if ((rc = sync_state(t)) || (rc = check_err(t))) return rc;
-        }
-        t->n_tets = r.n_tets;
-        t->n_live += r.n_live_delta;
-        t->inserted += r.n_winners;
-        t->windows_total += nwin;
-        if (trace) {
-            Counters h;
-            hipMemcpy(&h, t->cnt, sizeof(h), hipMemcpyDeviceToHost);
-            std::fprintf(trace, "%lld %d %d %d %d %.4f %.4f %d %lld %d %d %llu %llu %llu %llu\n", (long long)t->rounds,
-                         nwin, t->npend, nfresh, r.n_winners, a, b, r.n_q, (long long)t->inserted, n_big,
-                         r.n_next_pending, h.walk_steps, h.chain_hops, h.killed, h.created);
-        }
-        const int64_t consumed = (int64_t)nwin - r.n_next_pending - r.n_winners;   // duplicates
-        t->dups += consumed;
-        const double frac = nwin ? (double)r.n_winners / nwin : 1.0;
-        // keep about half of the window winning; windows past big_window are throughput-bound
-        // (a loser's cavity is wasted work, a round costs little beyond its points), so there
-        // the target winning fraction is higher
-        const bool big = t->big_window > 0 && nwin >= t->big_window;
-        const double hi = big ? t->big_frac_hi : 0.6, lo = big ? t->big_frac_lo : 0.35;
-        if (frac > hi) t->gamma = std::min(t->gamma * 1.5, 4.0);
-        else if (frac < lo) t->gamma = std::max(t->gamma * 0.7, 0.002);
-        // every round either inserts, drops a duplicate, or resolves queued predicates
-        stalled = (r.n_winners == 0 && consumed == 0 && r.n_q == 0) ? stalled + 1 : 0;
-        if (stalled > 64) return fail(VO_ERR_INTERNAL, "no progress for 64 rounds");
-        t->npend = r.n_next_pending;
-        t->cur ^= 1;
-        t->head += nfresh;
-        t->rounds++;
-        if (t->compact_ratio > 0 && t->n_tets >= t->compact_min && (double)t->n_tets > t->compact_ratio * (double)t->n_live &&
-            (rc = compact(t)))
-            return rc;
-    }
+    // the per-round host decisions of compaction and XCD regions need the host loop
+    const bool host_loop = !t->pipelined || tf.f || t->xcd || t->compact_ratio > 0;
+    int rc = host_loop ? run_host_planned(t, tf.f) : run_device_planned(t);
+    if (rc) return rc;
     VO_HIP(hipEventRecord(t->ev[1], t->s));
     VO_HIP(hipEventSynchronize(t->ev[1]));
     float ms = 0;
@@ -1100,6 +1372,12 @@ int vo_tri_config(vo_tri* t, const char* key, int64_t value) {
     } else if (k == "compact_ratio_milli") {
         if (value != 0 && value < 1500) return fail(VO_ERR_INVALID, "compact_ratio_milli must be 0 or >= 1500");
         t->compact_ratio = (double)value / 1000.0;
+    } else if (k == "pipelined") {
+        if (value != 0 && value != 1) return fail(VO_ERR_INVALID, "pipelined must be 0 or 1");
+        t->pipelined = (int)value;
+    } else if (k == "chunk_rounds") {
+        if (value < 1 || value > 64) return fail(VO_ERR_INVALID, "chunk_rounds must be in 1..64");
+        t->chunk_rounds = (int)value;
     } else if (k == "local_order") {
         if (value < 0 || value > (1 << 20)) return fail(VO_ERR_INVALID, "local_order must be in 0..2^20");
         t->local_order = (int)value;
