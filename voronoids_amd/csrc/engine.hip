@@ -472,7 +472,7 @@ struct vo_engine {
     RoundState* st_host = nullptr;
     BatchCtl* ctl = nullptr;
     BatchCtl* ctl_host = nullptr;
-    Counters* cnt_host = nullptr;          // device counters as of the last replay (pinned)
+    Counters* cnt_host = nullptr;          // device counters (kCntStripes copies) as of the last replay (pinned)
     Counters* cnt = nullptr;
     QEntry* q = nullptr;
     CEntry* cache = nullptr;
@@ -1198,7 +1198,7 @@ int vo_engine_create(int hip_device, int dim, int64_t max_points, int64_t max_si
         (rc = dev_alloc(&E->status, E->win_cap)) || (rc = dev_alloc(&E->big, E->big_cap)) ||
         (rc = dev_alloc(&E->cav1, E->solo_cap)) || (rc = dev_alloc(&E->fac1, E->solo_cap)) ||
         (rc = dev_alloc(&E->misc1, 4)) || (rc = dev_alloc(&E->st, 1)) || (rc = dev_alloc(&E->ctl, 1)) ||
-        (rc = dev_alloc(&E->cnt, 1)) || (rc = dev_alloc(&E->q, E->qcap)) ||
+        (rc = dev_alloc(&E->cnt, kCntStripes)) || (rc = dev_alloc(&E->q, E->qcap)) ||
         (rc = dev_alloc(&E->cache, (size_t)E->cmask + 1)) || (rc = dev_alloc(&E->ws, (size_t)E->ws_threads * E->ws_cap)) ||
         (rc = dev_alloc(&E->done, E->nslots)))
         return bail(rc);
@@ -1208,7 +1208,7 @@ int vo_engine_create(int hip_device, int dim, int64_t max_points, int64_t max_si
         return bail(fail(VO_ERR_OOM, "pinned export buffer"));
     if (hipHostMalloc((void**)&E->st_host, sizeof(RoundState)) != hipSuccess ||
         hipHostMalloc((void**)&E->ctl_host, sizeof(BatchCtl)) != hipSuccess ||
-        hipHostMalloc((void**)&E->cnt_host, sizeof(Counters)) != hipSuccess)
+        hipHostMalloc((void**)&E->cnt_host, sizeof(Counters) * kCntStripes) != hipSuccess)
         return bail(fail(VO_ERR_OOM, "pinned"));
     // reservations start at the largest key; no set record is active
     hipLaunchKernelGGL(k_fill_u64, dim3(4096), dim3(256), 0, E->s, reinterpret_cast<unsigned long long*>(E->a.rec),
@@ -1226,7 +1226,7 @@ int vo_engine_create(int hip_device, int dim, int64_t max_points, int64_t max_si
     *E->ctl_host = c0;
     if (hipMemcpyAsync(E->st, E->st_host, sizeof(RoundState), hipMemcpyHostToDevice, E->s) != hipSuccess ||
         hipMemcpyAsync(E->ctl, E->ctl_host, sizeof(BatchCtl), hipMemcpyHostToDevice, E->s) != hipSuccess ||
-        hipMemsetAsync(E->cnt, 0, sizeof(Counters), E->s) != hipSuccess || hipStreamSynchronize(E->s) != hipSuccess)
+        hipMemsetAsync(E->cnt, 0, sizeof(Counters) * kCntStripes, E->s) != hipSuccess || hipStreamSynchronize(E->s) != hipSuccess)
         return bail(fail(VO_ERR_HIP, "engine init"));
     if (hipGetLastError() != hipSuccess) return bail(fail(VO_ERR_HIP, "engine init kernels"));
     {
@@ -1335,7 +1335,7 @@ int vo_engine_launch(vo_engine* e) {
     VO_HIP(hipEventRecord(e->ev1, e->s));
     VO_HIP(hipMemcpyAsync(e->st_host, e->st, sizeof(RoundState), hipMemcpyDeviceToHost, e->s));
     VO_HIP(hipMemcpyAsync(e->ctl_host, e->ctl, sizeof(BatchCtl), hipMemcpyDeviceToHost, e->s));
-    VO_HIP(hipMemcpyAsync(e->cnt_host, e->cnt, sizeof(Counters), hipMemcpyDeviceToHost, e->s));
+    VO_HIP(hipMemcpyAsync(e->cnt_host, e->cnt, sizeof(Counters) * kCntStripes, hipMemcpyDeviceToHost, e->s));
     e->launched = true;
     // the previous boundary's result copies overlap this replay
     const double tc = now_s();
@@ -1359,7 +1359,7 @@ int vo_engine_finish(vo_engine* e, int64_t* n_left) {
         e->stats.rounds += e->rounds_per_replay;
         e->last_nwin = e->ctl_host->nwin;
         // points inserted so far, all sets (device count): the progress measure of a replay
-        e->stats.points_inserted = (int64_t)e->cnt_host->winners;
+        e->stats.points_inserted = (int64_t)sum_counters(e->cnt_host).winners;
         if ((rc = check_err(e))) return rc;
         if (e->st_host->n_big > 0 && (rc = run_solo(e))) return rc;
         if (e->st_host->n_nospace > 0 && (rc = extend_ranges(e))) return rc;

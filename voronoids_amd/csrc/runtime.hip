@@ -755,8 +755,9 @@ int finish_round(vo_tri* t, int nwin, int nfresh, FILE* trace, float ms_cav, flo
     t->inserted += r.n_winners;
     t->windows_total += nwin;
     if (trace) {
-        Counters h;
-        hipMemcpy(&h, t->cnt, sizeof(h), hipMemcpyDeviceToHost);
+        std::vector<Counters> hs(kCntStripes);
+        hipMemcpy(hs.data(), t->cnt, sizeof(Counters) * kCntStripes, hipMemcpyDeviceToHost);
+        const Counters h = sum_counters(hs.data());
         std::fprintf(trace, "%lld %d %d %d %d %.4f %.4f %d %lld %d %d %llu %llu %llu %llu\n", (long long)t->rounds, nwin,
                      t->npend, nfresh, r.n_winners, ms_cav, ms_com, r.n_q, (long long)t->inserted, n_big,
                      r.n_next_pending, h.walk_steps, h.chain_hops, h.killed, h.created);
@@ -1078,14 +1079,14 @@ int vo_tri_create(int hip_device, int dim, const double* pts, int64_t n, vo_tri*
         (rc = dev_alloc(&t->pend[1], t->win_cap)) || (rc = dev_alloc(&t->cav, (size_t)t->win_cap * t->kcap)) ||
         (rc = dev_alloc(&t->fac, (size_t)t->win_cap * t->fcap)) || (rc = dev_alloc(&t->nk, t->win_cap)) ||
         (rc = dev_alloc(&t->nf, t->win_cap)) || (rc = dev_alloc(&t->status, t->win_cap)) ||
-        (rc = dev_alloc(&t->st, 1)) || (rc = dev_alloc(&t->cnt, 1)) || (rc = dev_alloc(&t->q, t->qcap)) ||
+        (rc = dev_alloc(&t->st, 1)) || (rc = dev_alloc(&t->cnt, kCntStripes)) || (rc = dev_alloc(&t->q, t->qcap)) ||
         (rc = dev_alloc(&t->cache, (size_t)t->cmask + 1)) || (rc = dev_alloc(&t->ws, (size_t)t->ws_threads * t->ws_cap)) ||
         (rc = dev_alloc(&t->big, t->win_cap)) || (rc = dev_alloc(&t->cav1, t->big_cap)) ||
         (rc = dev_alloc(&t->fac1, t->big_cap)) || (rc = dev_alloc(&t->misc1, 4)))
         return bail(rc);
     if (hipHostMalloc((void**)&t->st_host, sizeof(RoundState)) != hipSuccess) return bail(fail(VO_ERR_OOM, "pinned"));
     hipMemsetAsync(t->st, 0, sizeof(RoundState), t->s);
-    hipMemsetAsync(t->cnt, 0, sizeof(Counters), t->s);
+    hipMemsetAsync(t->cnt, 0, sizeof(Counters) * kCntStripes, t->s);
     hipMemsetAsync(t->cache, 0xFF, sizeof(CEntry) * ((size_t)t->cmask + 1), t->s);
     if ((rc = grow_points(t, std::max<int64_t>(n, 16)))) return bail(rc);
     if ((rc = upload_super(t))) return bail(rc);
@@ -1306,8 +1307,9 @@ int vo_tri_check(vo_tri* t, int64_t* n_violations) {
 int vo_tri_stats(const vo_tri* t, vo_stats* out) {
     if (!t || !out) return fail(VO_ERR_INVALID, "null argument");
     VO_HIP(hipSetDevice(t->dev));
-    Counters h;
-    VO_HIP(hipMemcpy(&h, t->cnt, sizeof(h), hipMemcpyDeviceToHost));
+    std::vector<Counters> hs(kCntStripes);
+    VO_HIP(hipMemcpy(hs.data(), t->cnt, sizeof(Counters) * kCntStripes, hipMemcpyDeviceToHost));
+    const Counters h = sum_counters(hs.data());
     std::memset(out, 0, sizeof(*out));
     out->rounds = t->rounds;
     out->points_inserted = t->inserted;
