@@ -47,15 +47,17 @@ struct alignas(256) Counters {
     unsigned long long max_killed, max_facets;
 };
 
+// Round totals.  The four that every wavefront of a round updates sit on lines of their own, so
+// that their atomics queue separately; the rarely touched rest share one line.
 struct RoundState {
-    int n_tets;            // simplices allocated (append-only high-water mark; one set only)
-    int n_next_pending;    // losers and deferred points appended for the next window
-    int n_winners;
-    int n_q;               // exact-queue fill (may exceed capacity: excess dropped)
-    unsigned int err;      // sticky ERR_* bits
-    int n_live_delta;      // sum over winners of (created - killed) this round
-    int n_big;             // points whose cavity outgrew the group kernel's LDS lists
-    int n_nospace;         // several sets: winners whose set had no simplex space left
+    alignas(256) int n_tets;          // simplices allocated (append-only high-water mark; one set only)
+    alignas(256) int n_next_pending;  // losers and deferred points appended for the next window
+    alignas(256) int n_winners;
+    alignas(256) int n_live_delta;    // sum over winners of (created - killed) this round
+    alignas(256) int n_q;             // exact-queue fill (may exceed capacity: excess dropped)
+    unsigned int err;                 // sticky ERR_* bits
+    int n_big;                        // points whose cavity outgrew the group kernel's LDS lists
+    int n_nospace;                    // several sets: winners whose set had no simplex space left
 };
 
 // Several independent point sets sharing one device arena (batched and served modes).
