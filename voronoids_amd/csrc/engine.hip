@@ -105,11 +105,12 @@ __device__ int block_excl_scan_1024(int v, int* s_w, int* tot) {
 // points, its pending points first; when the sum exceeds the window capacity every set's
 // fresh share is scaled down.
 __global__ void __launch_bounds__(kPlanThreads) k_plan(BatchCtl* ctl, SetRec* sets, int nslots, RoundState* st,
-                                                       int win_cap, int win_limit, int pend_cap) {
+                                                       int win_cap, int win_limit, int pend_cap, const Counters* cnt) {
     __shared__ int s_w[kPlanThreads / 64 + 1];
     __shared__ int s_npend;
     __shared__ float s_gamma;
     __shared__ unsigned long long s_sum;
+    round_totals(cnt, st);       // the last round's winners
     if (threadIdx.x == 0) {
         // (a count past the list's capacity has already raised ERR_PENDING: never read past it)
         const int npend = min(st->n_next_pending, pend_cap);
@@ -741,7 +742,7 @@ int get_graph(vo_engine* e, int G, hipGraphExec_t* out) {
         int* src = e->pend[r & 1];
         c.pend_next = e->pend[(r + 1) & 1];
         hipLaunchKernelGGL(k_plan, dim3(1), dim3(kPlanThreads), 0, e->s, e->ctl, e->a.sets, e->nslots, e->st,
-                           e->win_cap, e->win_limit, e->win_cap + e->big_cap);
+                           e->win_cap, e->win_limit, e->win_cap + e->big_cap, e->cnt);
         hipLaunchKernelGGL(k_fill, dim3(2048), dim3(256), 0, e->s, e->ctl, e->a.sets, e->nslots, src, c.pend_next,
                            e->win);
         launch_round_kernels(e, c, G, 0);

@@ -47,17 +47,21 @@ struct alignas(256) Counters {
     unsigned long long max_killed, max_facets;
 };
 
-// Round totals.  The four that every wavefront of a round updates sit on lines of their own, so
-// that their atomics queue separately; the rarely touched rest share one line.
+// Round totals.  The two that every wavefront of a round updates atomically sit on lines of
+// their own, so that their atomics queue separately; the rarely touched rest share one line.
+// n_winners and n_live_delta are not atomic targets: round_totals derives them from the
+// striped statistics after the round.
 struct RoundState {
     alignas(256) int n_tets;          // simplices allocated (append-only high-water mark; one set only)
     alignas(256) int n_next_pending;  // losers and deferred points appended for the next window
-    alignas(256) int n_winners;
-    alignas(256) int n_live_delta;    // sum over winners of (created - killed) this round
     alignas(256) int n_q;             // exact-queue fill (may exceed capacity: excess dropped)
     unsigned int err;                 // sticky ERR_* bits
     int n_big;                        // points whose cavity outgrew the group kernel's LDS lists
     int n_nospace;                    // several sets: winners whose set had no simplex space left
+    int n_winners;                    // winners of the round (round_totals)
+    int n_live_delta;                 // sum over its winners of (created - killed) (round_totals)
+    unsigned long long acc_winners;   // statistics totals at the last round_totals call
+    long long acc_live;
 };
 
 // Several independent point sets sharing one device arena (batched and served modes).
@@ -338,6 +342,41 @@ __device__ inline void flush_counters(Counters* c, const LocalCnt& l) {
         if (mk) atomicMax(&c->max_killed, (unsigned long long)mk);
         if (mf) atomicMax(&c->max_facets, (unsigned long long)mf);
     }
+}
+
+// Winners and the change in live simplices since the last call, from the striped statistics
+// (each commit wavefront adds its winners, created and killed simplices there), into
+// st->n_winners / st->n_live_delta.  Called by every thread of one block (block-uniform).
+__device__ inline void round_totals(const Counters* cnt, RoundState* st) {
+    __shared__ unsigned long long s_w[16];
+    __shared__ long long s_l[16];
+    unsigned long long w = 0;
+    long long l = 0;
+    for (int k = threadIdx.x; k < kCntStripes; k += blockDim.x) {
+        w += cnt[k].winners;
+        l += (long long)cnt[k].created - (long long)cnt[k].killed;
+    }
+    w = wave_sum(w);
+    l = wave_sum_i64(l);
+    const int wv = threadIdx.x / 64, nw = (blockDim.x + 63) / 64;
+    if (__lane_id() == 0 && wv < 16) {
+        s_w[wv] = w;
+        s_l[wv] = l;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long W = 0;
+        long long L = 0;
+        for (int i = 0; i < nw && i < 16; ++i) {
+            W += s_w[i];
+            L += s_l[i];
+        }
+        st->n_winners = (int)(W - st->acc_winners);
+        st->n_live_delta = (int)(L - st->acc_live);
+        st->acc_winners = W;
+        st->acc_live = L;
+    }
+    __syncthreads();
 }
 
 // exclusive prefix sum across the 64 lanes; all lanes must call
@@ -868,13 +907,6 @@ __global__ void __launch_bounds__(256) k_commit(Ctx c, int nwin, unsigned long l
         if (pos < c.pend_cap) c.pend_next[pos] = p;
         else atomicOr(&c.st->err, ERR_PENDING);
     }
-    const unsigned long long wm = __ballot(win);
-    if (__lane_id() == 0) {
-        const int wc = __popcll(wm);
-        if (wc) atomicAdd(&c.st->n_winners, wc);
-    }
-    const long long delta = wave_sum_i64(win ? (long long)(nf - nk) : 0ll);
-    if (__lane_id() == 0 && delta) atomicAdd(&c.st->n_live_delta, (int)delta);
     }
     flush_counters(c.cnt, lc);
 }

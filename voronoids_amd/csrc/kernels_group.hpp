@@ -404,10 +404,6 @@ __global__ void __launch_bounds__(64) k_commit_g(Ctx c, int nwin, unsigned long 
         if (pos < c.pend_cap) c.pend_next[pos] = p;
         else atomicOr(&c.st->err, ERR_PENDING);
     }
-    const unsigned long long wm = __ballot(build && lead);
-    if (g.lane == 0 && wm) atomicAdd(&c.st->n_winners, __popcll(wm));
-    const long long delta = wave_sum_i64(build && lead ? (long long)(nf - nk) : 0ll);
-    if (g.lane == 0 && delta) atomicAdd(&c.st->n_live_delta, (int)delta);
     __builtin_amdgcn_wave_barrier();       // s_fac is reused by the next slots
     }
     flush_counters(c.cnt, lc);

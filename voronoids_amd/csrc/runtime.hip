@@ -303,11 +303,20 @@ __global__ void k_reset_round(RoundState* st) {
 // on; the window factor gamma grows when more than the target winning fraction won and
 // shrinks when fewer did; the next window is gamma x inserted points (at least 8, at most the
 // window capacity), the pending points first.
-__global__ void k_plan_one(RunPlan* P, RoundState* st) {
+__global__ void __launch_bounds__(kCntStripes) k_round_totals(const Counters* cnt, RoundState* st) {
+    round_totals(cnt, st);
+}
+
+__global__ void __launch_bounds__(kCntStripes) k_plan_one(RunPlan* P, RoundState* st, const Counters* cnt) {
+    __shared__ int s_absorb;
     RunPlan& p = *P;
+    if (threadIdx.x == 0) s_absorb = p.launched && !(st->n_big > 0 || st->n_q > 0 || st->err);
+    __syncthreads();
+    if (s_absorb) round_totals(cnt, st);
+    if (threadIdx.x != 0) return;
     p.ctl.nwin = 0;
     if (p.launched) {
-        if (st->n_big > 0 || st->n_q > 0 || st->err) {
+        if (!s_absorb) {
             p.halt |= HALT_HOST;
             return;
         }
@@ -750,6 +759,9 @@ int finish_round(vo_tri* t, int nwin, int nfresh, FILE* trace, float ms_cav, flo
         if ((rc = run_exact(t, r.n_q))) return rc;
         if ((rc = sync_state(t)) || (rc = check_err(t))) return rc;
     }
+    // the round's winners (solo runs included) from the statistics
+    hipLaunchKernelGGL(k_round_totals, dim3(1), dim3(kCntStripes), 0, t->s, t->cnt, t->st);
+    if ((rc = sync_state(t))) return rc;
     t->n_tets = r.n_tets;
     t->n_live += r.n_live_delta;
     t->inserted += r.n_winners;
@@ -934,7 +946,7 @@ int run_device_planned(vo_tri* t) {
             const int G = auto_group(t, est_mid);
             const int slot = k & 1;
             for (int r = 0; r < R; ++r) {
-                hipLaunchKernelGGL(k_plan_one, dim3(1), dim3(1), 0, t->s, t->plan, t->st);
+                hipLaunchKernelGGL(k_plan_one, dim3(1), dim3(kCntStripes), 0, t->s, t->plan, t->st, t->cnt);
                 hipLaunchKernelGGL(k_fill_one, dim3(std::max(1, std::min(1024, (est_hi + 255) / 256))), dim3(256), 0, t->s,
                                    t->plan, t->win, t->pend[cur]);
                 Ctx c = make_ctx(t);
