@@ -106,7 +106,10 @@ int vo_tri_stats(const vo_tri* t, vo_stats* out);
  * simplex, the live ones move to a dense array in spatial order; "compact_min" (default 2^20): not before this
  * many records exist.  "xcd_regions" (0 default / 1): each round's window is sorted into 8 spatial regions and
  * block b of the round kernels serves region b % 8, so that each of the 8 XCDs (blocks are dispatched to them
- * in turn) caches one region's records; measured no faster at n = 4M, kept as an option. */
+ * in turn) caches one region's records; measured no faster at n = 4M, kept as an option.  "pipelined" (1
+ * default / 0): rounds planned on the device and submitted in chunks, the host taking over only when the
+ * planner halts; 0 plans and waits for every round on the host (compaction, XCD regions and VO_TRACE always
+ * do); both give the same rounds and result.  "chunk_rounds" (default 4, 1..64): rounds per submitted chunk. */
 int vo_tri_config(vo_tri* t, const char* key, int64_t value);
 
 /* The precision tiers one predicate at a time (no counterpart in the reference, whose in_sphere
