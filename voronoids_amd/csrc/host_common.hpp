@@ -111,7 +111,8 @@ inline int record_bytes(int D) { return D == 3 ? 128 : 64; }
 // super vertices translated by c, reservations at the largest key
 inline void root_record(int D, const int tet0[4], int v0, const double sv[4][3], const double* c, unsigned char* out) {
     const int B = record_bytes(D), M = D + 1;
-    const int V = 0, A = D == 3 ? 16 : 12, X = D == 3 ? 32 : 24, K = D == 3 ? 96 : 48;
+    // the device's record layout (Rec<D> in kernels.hpp): coordinates packed in vertex order
+    const int V = 0, A = D == 3 ? 16 : 12, X = D == 3 ? 32 : 24, K = D == 3 ? 80 : 48;
     std::memset(out, 0, (size_t)B);
     int* v = reinterpret_cast<int*>(out + V);
     int* a = reinterpret_cast<int*>(out + A);
@@ -119,7 +120,7 @@ inline void root_record(int D, const int tet0[4], int v0, const double sv[4][3],
     for (int k = 0; k < M; ++k) {
         v[k] = v0 + tet0[k];
         a[k] = -1;
-        for (int j = 0; j < D; ++j) x[(D == 3 ? 4 : 2) * k + j] = (float)(sv[tet0[k]][j] - c[j]);
+        for (int j = 0; j < D; ++j) x[D * k + j] = (float)(sv[tet0[k]][j] - c[j]);
     }
     std::memset(out + K, 0xFF, 16);
 }

@@ -204,11 +204,15 @@ __device__ __forceinline__ void set_comp(int4& a, int k, int v) {
     if (k == 0) a.x = v; else if (k == 1) a.y = v; else if (k == 2) a.z = v; else a.w = v;
 }
 
-// simplex record layout: byte offsets of v (int x M), a (int x M), x (fp32 coordinates),
-// kres, nres; 3D: v int4 | a int4 | x 4 x float4 | kres | nres | 16 B pad;
-// 2D: v int3 | a int3 | x 3 x float2 | kres | nres
+// simplex record layout: byte offsets of v (int x M), a (int x M), x (fp32 coordinates, packed
+// in vertex order), kres, nres; 3D: v int4 | a int4 | x 12 floats | kres | nres | 32 B pad;
+// 2D: v int3 | a int3 | x 6 floats | kres | nres.  Adjacency and coordinates are contiguous and
+// 16 B aligned (3D: bytes 16..79, 2D with the vertices: 0..47), so that a walk step or a cavity
+// test reads them with four (3D) / three (2D) 16 B loads: several loads of one line by one lane
+// are separate requests, and at full load the request rate, not the line rate, bounds the
+// random gathers.  kres and nres share one 16 B load.
 template <int D> struct Rec;
-template <> struct Rec<3> { static constexpr int B = 128, V = 0, A = 16, X = 32, K = 96, N = 104; };
+template <> struct Rec<3> { static constexpr int B = 128, V = 0, A = 16, X = 32, K = 80, N = 88; };
 template <> struct Rec<2> { static constexpr int B = 64, V = 0, A = 12, X = 24, K = 48, N = 56; };
 
 template <int D> __device__ __forceinline__ char* rp(const Ctx& c, int t) { return c.rec + (size_t)t * Rec<D>::B; }
@@ -483,10 +487,18 @@ __device__ __forceinline__ void enqueue(const Ctx& c, uint32_t type, const uint3
 // Tier 1 reads the simplex's own coordinate record tx[t] (the fp32 coordinates of its
 // vertices, in vertex order); tiers 2 and 3 need vertex ids and load tv[t] only then.
 
+// 3D coordinates: 12 packed floats in three float4 <-> four float4 with w = 0
+__device__ __forceinline__ void unpack_x3(float4 f0, float4 f1, float4 f2, float4* a) {
+    a[0] = make_float4(f0.x, f0.y, f0.z, 0.f);
+    a[1] = make_float4(f0.w, f1.x, f1.y, 0.f);
+    a[2] = make_float4(f1.z, f1.w, f2.x, 0.f);
+    a[3] = make_float4(f2.y, f2.z, f2.w, 0.f);
+}
+
 template <int D> __device__ __forceinline__ void load_tx(const Ctx& c, int t, typename VT<D>::F* a) {
     if constexpr (D == 3) {
         const float4* q = reinterpret_cast<const float4*>(rp<D>(c, t) + Rec<D>::X);
-        a[0] = q[0]; a[1] = q[1]; a[2] = q[2]; a[3] = q[3];
+        unpack_x3(q[0], q[1], q[2], a);
     } else {
         const float2* q = reinterpret_cast<const float2*>(rp<D>(c, t) + Rec<D>::X);
         a[0] = q[0]; a[1] = q[1]; a[2] = q[2];
@@ -496,11 +508,35 @@ template <int D> __device__ __forceinline__ void load_tx(const Ctx& c, int t, ty
 template <int D> __device__ __forceinline__ void store_tx(const Ctx& c, int t, const typename VT<D>::F* a) {
     if constexpr (D == 3) {
         float4* q = reinterpret_cast<float4*>(rp<D>(c, t) + Rec<D>::X);
-        q[0] = a[0]; q[1] = a[1]; q[2] = a[2]; q[3] = a[3];
+        q[0] = make_float4(a[0].x, a[0].y, a[0].z, a[1].x);
+        q[1] = make_float4(a[1].y, a[1].z, a[2].x, a[2].y);
+        q[2] = make_float4(a[2].z, a[3].x, a[3].y, a[3].z);
     } else {
         float2* q = reinterpret_cast<float2*>(rp<D>(c, t) + Rec<D>::X);
         q[0] = a[0]; q[1] = a[1]; q[2] = a[2];
     }
+}
+
+// adjacency and coordinates of simplex t in one run of 16 B loads (2D: with the vertices)
+template <int D> __device__ __forceinline__ void ld_ax(const Ctx& c, int t, int4& adj, typename VT<D>::F* a) {
+    if constexpr (D == 3) {
+        const float4* q = reinterpret_cast<const float4*>(rp<D>(c, t) + Rec<D>::A);
+        const float4 f0 = q[0], f1 = q[1], f2 = q[2], f3 = q[3];
+        adj = make_int4(__float_as_int(f0.x), __float_as_int(f0.y), __float_as_int(f0.z), __float_as_int(f0.w));
+        unpack_x3(f1, f2, f3, a);
+    } else {
+        const int4* q = reinterpret_cast<const int4*>(rp<D>(c, t));
+        const int4 w0 = q[0], w1 = q[1], w2 = q[2];
+        adj = make_int4(w0.w, w1.x, w1.y, -1);
+        a[0] = make_float2(__int_as_float(w1.z), __int_as_float(w1.w));
+        a[1] = make_float2(__int_as_float(w2.x), __int_as_float(w2.y));
+        a[2] = make_float2(__int_as_float(w2.z), __int_as_float(w2.w));
+    }
+}
+
+// kres and nres of simplex t in one 16 B load
+template <int D> __device__ __forceinline__ ulonglong2 ld_res(const Ctx& c, int t) {
+    return *reinterpret_cast<const ulonglong2*>(rp<D>(c, t) + Rec<D>::K);
 }
 
 // tiers 2 and 3 of the in-sphere test of vertex pv against simplex v
@@ -590,6 +626,38 @@ __device__ __forceinline__ void star_record(const typename VT<D>::F* ox, int i, 
     } else {
         nx[1] = i == 0 ? ox[1] : (i == 1 ? ox[2] : ox[0]);
         nx[2] = i == 0 ? ox[2] : (i == 1 ? ox[0] : ox[1]);
+    }
+}
+
+// Coordinates of the new simplex replace_first(v(t), i, p) written to record N straight from
+// t's packed record (3D: scalar selects on the packed floats, no private array)
+template <int D>
+__device__ __forceinline__ void write_star_coords(const Ctx& c, int t, int i, typename VT<D>::F pf, int N) {
+    if constexpr (D == 3) {
+        const float4* q = reinterpret_cast<const float4*>(rp<D>(c, t) + Rec<D>::X);
+        const float4 f0 = q[0], f1 = q[1], f2 = q[2];
+        // old vertex k = (X[3k], X[3k+1], X[3k+2]) of the packed floats
+        const float X[12] = {f0.x, f0.y, f0.z, f0.w, f1.x, f1.y, f1.z, f1.w, f2.x, f2.y, f2.z, f2.w};
+        // new vertices 1..3 come from old vertices s1, s2, s3 (replace_first's even permutation)
+        const int s1 = i == 0 ? 1 : 0;
+        const int s2 = i == 0 ? 2 : (i == 1 ? 3 : (i == 2 ? 1 : 2));
+        const int s3 = i == 0 ? 3 : (i == 1 ? 2 : (i == 2 ? 3 : 1));
+        float o[9];
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            o[j] = s1 == 0 ? X[j] : (s1 == 1 ? X[3 + j] : (s1 == 2 ? X[6 + j] : X[9 + j]));
+            o[3 + j] = s2 == 0 ? X[j] : (s2 == 1 ? X[3 + j] : (s2 == 2 ? X[6 + j] : X[9 + j]));
+            o[6 + j] = s3 == 0 ? X[j] : (s3 == 1 ? X[3 + j] : (s3 == 2 ? X[6 + j] : X[9 + j]));
+        }
+        float4* w = reinterpret_cast<float4*>(rp<D>(c, N) + Rec<D>::X);
+        w[0] = make_float4(pf.x, pf.y, pf.z, o[0]);
+        w[1] = make_float4(o[1], o[2], o[3], o[4]);
+        w[2] = make_float4(o[5], o[6], o[7], o[8]);
+    } else {
+        typename VT<D>::F ox[4], nx[4];
+        load_tx<D>(c, t, ox);
+        star_record<D>(ox, i, pf, nx);
+        store_tx<D>(c, N, nx);
     }
 }
 

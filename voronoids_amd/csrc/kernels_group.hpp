@@ -44,10 +44,16 @@ template <int G> struct Group {
 #define VO_GROUP_FC3 256
 #endif
 #ifndef VO_CAVITY_WAVES
-#define VO_CAVITY_WAVES 1
+#define VO_CAVITY_WAVES 3
 #endif
 template <int D> constexpr int kGroupKC = D == 3 ? VO_GROUP_KC3 : 64;    // cavity entries per point
 template <int D> constexpr int kGroupFC = D == 3 ? VO_GROUP_FC3 : 96;    // boundary facets per point
+// Cavity members whose adjacency the breadth-first search keeps in LDS: it arrives with the
+// member's coordinates when the member is tested (one run of loads), so expanding the member
+// costs no further request; members past this index read their adjacency from the record.
+// About 6 KB (3D) / 3 KB (2D) per wavefront, so that LDS does not limit the waves per CU.
+template <int D, int G> constexpr int kGroupAC =
+    (D == 3 ? 384 : 192) / (64 / G) < kGroupKC<D> ? (D == 3 ? 384 : 192) / (64 / G) : kGroupKC<D>;
 
 
 // LDS list scans, four entries per ds_read_b128 (entries at or past n are ignored: they may
@@ -85,6 +91,9 @@ __global__ void __launch_bounds__(64, VO_CAVITY_WAVES) k_cavity_g(Ctx c, int nwi
     static_assert(KC % 4 == 0, "int4 scans");
     __shared__ __align__(16) int s_cav[NG][KC];
     __shared__ __align__(16) int s_tmp[64];
+    constexpr int AC = kGroupAC<D, G>;
+    __shared__ __align__(16) int4 s_adj[NG][AC];
+    __shared__ __align__(16) int4 s_na[64];           // adjacency of each lane's candidate
     const Group<G> g;
     const bool lead = g.gl == 0;
     LocalCnt lc = {};
@@ -118,12 +127,13 @@ __global__ void __launch_bounds__(64, VO_CAVITY_WAVES) k_cavity_g(Ctx c, int nwi
     int came = -1;
     uint32_t rng = mix32((uint32_t)p * 0x9E3779B9u + 0x632BE5ABu);
     F a[4];
-    if (walking) load_tx<D>(c, t, a);
+    int4 aa = make_int4(-1, -1, -1, -1);
+    if (walking) ld_ax<D>(c, t, aa, a);
     for (int step = 0; __any(walking); ++step) {
         int s = 1, nb = -1;
         bool test = false;
         if (walking) {
-            const int4 aa = ld_a<D>(c, t);
+            if (lead) s_adj[g.gid][0] = aa;    // the simplex the walk ends in is member 0
             rng = rng * 1664525u + 1013904223u;
             const int r = (int)((rng >> 16) % M);
             int k = r + g.gl;
@@ -152,7 +162,7 @@ __global__ void __launch_bounds__(64, VO_CAVITY_WAVES) k_cavity_g(Ctx c, int nwi
             } else {
                 came = t;
                 t = nb_sel >> 2;
-                load_tx<D>(c, t, a);
+                ld_ax<D>(c, t, aa, a);
             }
         }
     }
@@ -186,16 +196,19 @@ __global__ void __launch_bounds__(64, VO_CAVITY_WAVES) k_cavity_g(Ctx c, int nwi
         int tet = 0, face = 0, n = -1, s = 0;
         bool hull = false, test = false;
         if (valid) {
-            tet = s_cav[g.gid][lo + i / M];
+            const int mi = lo + i / M;
+            tet = s_cav[g.gid][mi];
             face = i % M;
-            const int nb = *a_ptr<D>(c, tet, face);
+            const int nb = mi < AC ? reinterpret_cast<const int*>(s_adj[g.gid])[mi * 4 + face] : *a_ptr<D>(c, tet, face);
             hull = nb < 0;
             if (!hull) {
                 n = nb >> 2;
                 test = !lds_contains(s_cav[g.gid], nk, n);
                 if (test) {
                     F an[4];
-                    load_tx<D>(c, n, an);
+                    int4 na;
+                    ld_ax<D>(c, n, na, an);
+                    s_na[g.lane] = na;
                     s = conflict_at<D>(c, an, n, pv, pf, lc);
                 }
             }
@@ -216,7 +229,11 @@ __global__ void __launch_bounds__(64, VO_CAVITY_WAVES) k_cavity_g(Ctx c, int nwi
                 st = ST_OVERFLOW;          // re-run alone with global-memory lists
                 bfs = false;
             } else {
-                if (isnew) s_cav[g.gid][nk + g.below(newm)] = n;
+                if (isnew) {
+                    const int at = nk + g.below(newm);
+                    s_cav[g.gid][at] = n;
+                    if (at < AC) s_adj[g.gid][at] = s_na[g.lane];
+                }
                 if (isfac) {
                     // facets go straight to the point's global list: only the killed list,
                     // which every candidate is checked against, occupies LDS
@@ -254,7 +271,7 @@ __global__ void __launch_bounds__(64, VO_CAVITY_WAVES) k_cavity_g(Ctx c, int nwi
 }
 
 template <int D, int G>
-__global__ void __launch_bounds__(64) k_commit_g(Ctx c, int nwin, unsigned long long key_hi) {
+__global__ void __launch_bounds__(64, 5) k_commit_g(Ctx c, int nwin, unsigned long long key_hi) {
     constexpr int M = D + 1;
     constexpr int NG = Group<G>::NG;
     constexpr int FC = kGroupFC<D>;
@@ -282,7 +299,8 @@ __global__ void __launch_bounds__(64) k_commit_g(Ctx c, int nwin, unsigned long 
             nf = c.nf[slot];
             for (int j = g.gl; j < nk && ok; j += G) {
                 const int x = c.cav[(size_t)slot * c.kcap + (j)];
-                ok = *kres_p<D>(c, x) == key && *nres_p<D>(c, x) > key;
+                const ulonglong2 r = ld_res<D>(c, x);
+                ok = r.x == key && r.y > key;
             }
             for (int j = g.gl; j < nf && ok; j += G) {
                 const int fe = c.fac[(size_t)slot * c.fcap + (j)];
@@ -309,10 +327,7 @@ __global__ void __launch_bounds__(64) k_commit_g(Ctx c, int nwin, unsigned long 
             const int outside = *a_ptr<D>(c, t, i);
             const int N = base + j;
             st_v<D>(c, N, nv);
-            typename VT<D>::F ox[4], nx[4];
-            load_tx<D>(c, t, ox);
-            star_record<D>(ox, i, pf, nx);
-            store_tx<D>(c, N, nx);
+            write_star_coords<D>(c, t, i, pf, N);
             *a_ptr<D>(c, N, 0) = outside;
             if (outside >= 0) *a_ptr<D>(c, outside >> 2, outside & 3) = N << 2;
         }
