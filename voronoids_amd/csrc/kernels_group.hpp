@@ -297,16 +297,20 @@ __global__ void __launch_bounds__(64, 5) k_commit_g(Ctx c, int nwin, unsigned lo
         if (st == ST_OK) {
             nk = c.nk[slot];
             nf = c.nf[slot];
-            for (int j = g.gl; j < nk && ok; j += G) {
-                const int x = c.cav[(size_t)slot * c.kcap + (j)];
+            // members hold their kill reservation and no lower outside reservation; outside
+            // neighbors are killed by no lower key.  Member and facet checks share one pass with
+            // unconditional loads (indices clamped to the lists, never empty for an OK point):
+            // three dependent loads per pass
+            const int n_it = nk > nf ? nk : nf;
+            for (int j = g.gl; j < n_it; j += G) {
+                const int jk = j < nk ? j : nk - 1, jf = j < nf ? j : nf - 1;
+                const int x = c.cav[(size_t)slot * c.kcap + jk];
+                const int fe = c.fac[(size_t)slot * c.fcap + jf];
                 const ulonglong2 r = ld_res<D>(c, x);
-                ok = r.x == key && r.y > key;
-            }
-            for (int j = g.gl; j < nf && ok; j += G) {
-                const int fe = c.fac[(size_t)slot * c.fcap + (j)];
-                s_fac[g.gid][j] = fe;
                 const int nb = *a_ptr<D>(c, fe >> 2, fe & 3);
-                if (nb >= 0) ok = *kres_p<D>(c, nb >> 2) > key;
+                const unsigned long long ko = *kres_p<D>(c, nb >= 0 ? nb >> 2 : x);
+                if (j < nf) s_fac[g.gid][j] = fe;
+                ok = ok & (j >= nk || ((r.x == key) & (r.y > key))) & (j >= nf || nb < 0 || ko > key);
             }
         }
     }
@@ -341,6 +345,7 @@ __global__ void __launch_bounds__(64, 5) k_commit_g(Ctx c, int nwin, unsigned lo
             const int fe = s_fac[g.gid][j];
             const int t = fe >> 2, i = fe & 3;
             const int4 tvt = ld_v<D>(c, t);
+            int4 TA = ld_a<D>(c, t);
             const int4 nv = replace_first<D>(tvt, i, pv);
             const int cv = comp(nv, k);
             int T = t;
@@ -348,16 +353,21 @@ __global__ void __launch_bounds__(64, 5) k_commit_g(Ctx c, int nwin, unsigned lo
             int e = index_of<D>(TV, cv);
             int y = comp(tvt, i);
             for (int guard = 0;; ++guard) {
-                const int nbe = *a_ptr<D>(c, T, e);
-                if (nbe < 0 || *kres_p<D>(c, nbe >> 2) != key) break;
-                if (guard > 4096) { atomicOr(&c.st->err, ERR_WALK); break; }
+                const int nbe = comp(TA, e);
+                if (nbe < 0) break;
+                // reservation, vertices and adjacency of the next simplex in one round trip
                 const int T2 = nbe >> 2;
+                const unsigned long long kr = *kres_p<D>(c, T2);
                 const int4 TV2 = ld_v<D>(c, T2);
+                const int4 TA2 = ld_a<D>(c, T2);
+                if (kr != key) break;
+                if (guard > 4096) { atomicOr(&c.st->err, ERR_WALK); break; }
                 const int w = comp(TV2, nbe & 3);
                 e = index_of<D>(TV2, y);
                 y = w;
                 T = T2;
                 TV = TV2;
+                TA = TA2;
             }
             const int target = (T << 2) | e;
             const int jj = lds_find<0>(s_fac[g.gid], nf, target);
