@@ -679,14 +679,16 @@ int ensure_stage(vo_engine* e, size_t bytes) {
     return 0;
 }
 
+// as the one-set driver's auto_group: latency-bound small windows get more lanes per point,
+// large ones four (the thread kernels only on request, "group_lanes" = 1)
 int lanes_group(const vo_engine* e, int64_t nwin) {
     if (e->group) return e->group;
-    const int64_t target = e->lanes_target ? e->lanes_target : (e->D == 3 ? (1 << 20) : (1 << 16));
-    static const int sizes[] = {32, 16, 8, 4};
+    const int64_t target = e->lanes_target ? e->lanes_target : (e->D == 3 ? (1 << 21) : (1 << 16));
+    static const int sizes[] = {32, 16, 8};
     const int gmax = e->D == 3 ? 32 : 16;
     for (int g : sizes)
         if (g <= gmax && nwin * g <= target) return g;
-    return 1;
+    return 4;
 }
 
 int gindex(int G) { return G == 1 ? 0 : G == 4 ? 1 : G == 8 ? 2 : G == 16 ? 3 : 4; }

@@ -629,16 +629,18 @@ int compact(vo_tri* t) {
 
 // which = 0: locate + cavity + reservation, 1: check + commit
 // lanes per point for this round: small windows are latency-bound (more lanes per point
-// shorten each point's dependent-load chain), large windows are throughput-bound (one lane
-// per point keeps every SIMD lane busy)
+// shorten each point's dependent-load chain); large windows are throughput-bound, where four
+// lanes per point (face tests in parallel, LDS lists, one run of loads per simplex) beat one
+// thread per point (measured: 2D n = 10M 114 -> 108 ms), so the choice stops at 4 unless
+// "group_lanes" = 1 asks for the thread kernels
 int auto_group(const vo_tri* t, int nwin) {
     if (t->group) return t->group;
-    const int64_t target = t->lanes_target ? t->lanes_target : (t->D == 3 ? (1 << 20) : (1 << 16));
-    static const int sizes[] = {32, 16, 8, 4};
+    const int64_t target = t->lanes_target ? t->lanes_target : (t->D == 3 ? (1 << 21) : (1 << 16));
+    static const int sizes[] = {32, 16, 8};
     const int gmax = t->D == 3 ? 32 : 16;
     for (int g : sizes)
         if (g <= gmax && (int64_t)nwin * g <= target) return g;
-    return 1;
+    return 4;
 }
 
 // nwin sizes the grid (the kernels read the exact window from c.ctl when it is set); G = 0:
