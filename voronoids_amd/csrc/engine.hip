@@ -31,6 +31,7 @@
 #include <map>
 #include <memory>
 #include <mutex>
+#include <set>
 #include <string>
 #include <thread>
 #include <vector>
@@ -142,6 +143,7 @@ __global__ void __launch_bounds__(kPlanThreads) k_plan(BatchCtl* ctl, SetRec* se
     const int npend = s_npend;
     const float g = s_gamma;
     const int minw = ctl->min_window;
+    nslots = min(nslots, ctl->slots);
     unsigned long long part = 0;
     for (int s = threadIdx.x; s < nslots; s += kPlanThreads) {
         SetRec& r = sets[s];
@@ -187,6 +189,7 @@ __global__ void __launch_bounds__(kPlanThreads) k_plan(BatchCtl* ctl, SetRec* se
 __global__ void __launch_bounds__(256) k_fill(const BatchCtl* ctl, SetRec* sets, int nslots, const int* pend,
                                               int* pend_next, int* win) {
     const int npend = ctl->npend, tail = ctl->tail;
+    nslots = min(nslots, ctl->slots);
     for (int i = blockIdx.x * 256 + threadIdx.x; i < npend; i += gridDim.x * 256) win[i] = pend[i];
     for (int i = blockIdx.x * 256 + threadIdx.x; i < tail; i += gridDim.x * 256) pend_next[i] = pend[npend + i];
     for (int s = blockIdx.x; s < nslots; s += gridDim.x) {
@@ -201,6 +204,7 @@ __global__ void __launch_bounds__(256) k_fill(const BatchCtl* ctl, SetRec* sets,
 
 // completed sets (every point inserted or dropped as a duplicate) leave the rounds
 __global__ void k_collect(BatchCtl* ctl, SetRec* sets, int nslots, DoneRec* done) {
+    nslots = min(nslots, ctl->slots);
     for (int s = blockIdx.x * blockDim.x + threadIdx.x; s < nslots; s += gridDim.x * blockDim.x) {
         SetRec& r = sets[s];
         if ((r.flags & SET_ACTIVE) && r.inserted + r.dups == r.n) {
@@ -212,6 +216,9 @@ __global__ void k_collect(BatchCtl* ctl, SetRec* sets, int nslots, DoneRec* done
 }
 
 __global__ void k_clear_queue(RoundState* st) { st->n_q = 0; }
+
+// slots that may be in use, set before each replay (the planner, fill and collection scan only these)
+__global__ void k_set_slots(BatchCtl* ctl, int slots) { ctl->slots = slots; }
 
 __global__ void k_boundary_reset(BatchCtl* ctl, RoundState* st, unsigned int round) {
     ctl->n_done = 0;
@@ -487,7 +494,8 @@ struct vo_engine {
     size_t out_cap = 0;                     // int32 elements
     // host bookkeeping
     RangeAlloc valloc, talloc, halloc;
-    std::vector<int> free_slots;
+    std::set<int> free_slots;          // the lowest free slot is taken first, so that slots in use stay dense
+    std::set<int> used_slots;
     std::deque<std::unique_ptr<HSet>> queue;
     std::vector<std::unique_ptr<HSet>> resident;   // by slot
     int64_t n_resident = 0;
@@ -807,8 +815,9 @@ int admit(vo_engine* e) {
         h->hcells = hc;
         h->hlevels = L;
         h->ranges.push_back({t0, tl, -1});
-        h->slot = e->free_slots.back();
-        e->free_slots.pop_back();
+        h->slot = *e->free_slots.begin();
+        e->free_slots.erase(e->free_slots.begin());
+        e->used_slots.insert(h->slot);
         h->t_admit = now_s();
         h->round_admit = e->ctl_host->round;
         adm.push_back(h);
@@ -1134,7 +1143,8 @@ int collect(vo_engine* e, int n_done) {
             e->valloc.release(h->v0, e->M + h->n);
             e->halloc.release(h->h0, h->hcells);
             for (auto& rg : h->ranges) e->talloc.release(rg.start, rg.len);
-            e->free_slots.push_back(d.slot);
+            e->free_slots.insert(d.slot);
+            e->used_slots.erase(d.slot);
             e->n_resident--;
             e->stats.sets_done++;
         }
@@ -1180,7 +1190,7 @@ int vo_engine_create(int hip_device, int dim, int64_t max_points, int64_t max_si
     E->cmask = (1u << 22) - 1;
     E->gamma0 = dim == 3 ? 0.2f : 0.6f;
     E->resident.resize(E->nslots);
-    for (int s = E->nslots - 1; s >= 0; --s) E->free_slots.push_back(s);
+    for (int s = 0; s < E->nslots; ++s) E->free_slots.insert(s);
     auto bail = [&](int rc) {
         engine_free(E);
         return rc;
@@ -1333,6 +1343,8 @@ int vo_engine_launch(vo_engine* e) {
     const int G = lanes_group(e, std::min<int64_t>(pred, e->win_limit));
     hipGraphExec_t g;
     if ((rc = get_graph(e, G, &g))) return rc;
+    hipLaunchKernelGGL(k_set_slots, dim3(1), dim3(1), 0, e->s, e->ctl, e->used_slots.empty() ? 0 : *e->used_slots.rbegin() + 1);
+    VO_HIP(hipGetLastError());
     VO_HIP(hipEventRecord(e->ev0, e->s));
     VO_HIP(hipGraphLaunch(g, e->s));
     VO_HIP(hipEventRecord(e->ev1, e->s));

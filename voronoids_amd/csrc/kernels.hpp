@@ -68,18 +68,23 @@ struct RoundState {
 // Set s owns a contiguous vertex block (its d+1 super vertices, then its points in priority
 // order), one or more simplex ranges handed out by the host, and its own hint grid.
 enum : int { SET_ACTIVE = 1, SET_NEED_SPACE = 2, SET_DONE = 4 };
+// The five fields that every commit wavefront of a set updates atomically (the allocation
+// cursor and four outcome counters) sit on lines of their own: atomics to one line queue one
+// behind another, and a large set is served by thousands of wavefronts per round.
 struct SetRec {
-    int cursor, end;       // simplex allocation in the current range: next free, one past last
-    int inserted, dups;    // points inserted / dropped as duplicates
+    alignas(128) int cursor; // simplex allocation in the current range: next free
+    int end;               // one past the last slot of the current range
     int head, last;        // fresh (never windowed) points: head .. last-1 (point indices)
-    int pend;              // points of the set sent to the pending list in the current round
     int flags;             // SET_*
     int hint_off, hlevels; // hint grid of the set: offset in the hint arena, finest level
     float glo[3], ginv;    // grid origin and 1 / extent in the set's translated frame
-    int live;              // live simplices of the set
     int n;                 // points of the set
     int fresh, woff;       // planner: fresh points windowed this round and their offset
     int fail_lo;           // least slot of a failed request in the current range (slots below it are all written)
+    alignas(128) int inserted; // points inserted
+    alignas(128) int dups;     // points dropped as duplicates
+    alignas(128) int pend;     // points of the set sent to the pending list in the current round
+    alignas(128) int live;     // live simplices of the set
 };
 
 // round control planned on the device when several sets share the arrays: the kernels read
@@ -94,7 +99,7 @@ struct BatchCtl {
     int n_active;
     int min_window;        // least fresh window of a set per round
     int big_window;        // windows at least this large aim at a higher winning fraction
-    int pad;
+    int slots;             // set slots that may be in use: one past the highest (host, before each replay)
 };
 
 struct Ctx {
