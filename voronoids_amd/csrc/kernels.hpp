@@ -69,11 +69,12 @@ struct RoundState {
 // order), one or more simplex ranges handed out by the host, and its own hint grid.
 enum : int { SET_ACTIVE = 1, SET_NEED_SPACE = 2, SET_DONE = 4 };
 // The five fields that every commit wavefront of a set updates atomically (the allocation
-// cursor and four outcome counters) sit on lines of their own: atomics to one line queue one
-// behind another, and a large set is served by thousands of wavefronts per round.
+// cursor and four outcome counters) sit 256 B apart, each alone on its line, and apart from the
+// fields the kernels read: atomics to one line queue one behind another (loads of that line
+// too), and a large set is served by thousands of wavefronts per round.
 struct SetRec {
-    alignas(128) int cursor; // simplex allocation in the current range: next free
-    int end;               // one past the last slot of the current range
+    alignas(256) int cursor; // simplex allocation in the current range: next free
+    alignas(256) int end;  // one past the last slot of the current range
     int head, last;        // fresh (never windowed) points: head .. last-1 (point indices)
     int flags;             // SET_*
     int hint_off, hlevels; // hint grid of the set: offset in the hint arena, finest level
@@ -81,10 +82,10 @@ struct SetRec {
     int n;                 // points of the set
     int fresh, woff;       // planner: fresh points windowed this round and their offset
     int fail_lo;           // least slot of a failed request in the current range (slots below it are all written)
-    alignas(128) int inserted; // points inserted
-    alignas(128) int dups;     // points dropped as duplicates
-    alignas(128) int pend;     // points of the set sent to the pending list in the current round
-    alignas(128) int live;     // live simplices of the set
+    alignas(256) int inserted; // points inserted
+    alignas(256) int dups;     // points dropped as duplicates
+    alignas(256) int pend;     // points of the set sent to the pending list in the current round
+    alignas(256) int live;     // live simplices of the set
 };
 
 // round control planned on the device when several sets share the arrays: the kernels read
