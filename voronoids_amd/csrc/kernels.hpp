@@ -101,6 +101,8 @@ struct BatchCtl {
     int min_window;        // least fresh window of a set per round
     int big_window;        // windows at least this large aim at a higher winning fraction
     int slots;             // set slots that may be in use: one past the highest (host, before each replay)
+    int virt, vhead;       // virt = 1: the window is not written out: slots below npend are the
+                           // pending list (Ctx::win), slot i >= npend is point vhead + i - npend
 };
 
 struct Ctx {
@@ -165,6 +167,19 @@ __device__ __forceinline__ void round_params(const Ctx& c, int& nwin, unsigned l
         nwin = c.ctl->nwin;
         key_hi = c.ctl->key_hi;
     }
+}
+
+// the round's window: point of slot i
+struct WinView {
+    const int* w;
+    int np, head;
+};
+__device__ __forceinline__ WinView win_view(const Ctx& c) {
+    if (c.ctl && c.ctl->virt) return WinView{c.win, c.ctl->npend, c.ctl->vhead};
+    return WinView{c.win, 0x7fffffff, 0};
+}
+__device__ __forceinline__ int win_point(const WinView& v, int slot) {
+    return slot < v.np ? v.w[slot] : v.head + (slot - v.np);
 }
 
 // hint grid of a point's set
@@ -726,11 +741,12 @@ __global__ void __launch_bounds__(256) k_cavity(Ctx c, int nwin, unsigned long l
     const int lt = threadIdx.x;
     LocalCnt lc = {};
     round_params(c, nwin, key_hi);
+    const WinView wv = win_view(c);
     const SlotIter it = slot_iter(c, nwin, blockDim.x);
     for (int kb = it.base; kb < it.lim; kb += it.stride) {
     const int slot = it.off + kb + threadIdx.x;
     if (kb + (int)threadIdx.x < it.lim) {
-        const int p = c.win[slot];
+        const int p = win_point(wv, slot);
         const int pv = c.M + p;
         const typename V::F pf = V::ld(c.X32, pv);
         const HintView hv = hint_view(c, p);
@@ -857,6 +873,7 @@ __global__ void __launch_bounds__(256) k_commit(Ctx c, int nwin, unsigned long l
     constexpr int M = D + 1;
     LocalCnt lc = {};
     round_params(c, nwin, key_hi);
+    const WinView wv = win_view(c);
     const SlotIter it = slot_iter(c, nwin, blockDim.x);
     for (int kb = it.base; kb < it.lim; kb += it.stride) {
     const int slot = it.off + kb + threadIdx.x;
@@ -866,7 +883,7 @@ __global__ void __launch_bounds__(256) k_commit(Ctx c, int nwin, unsigned long l
     unsigned long long key = 0;
     if (valid) {
         st = c.status[slot];
-        p = c.win[slot];
+        p = win_point(wv, slot);
         key = key_hi | (unsigned)p;
         if (st == ST_OK) {
             nk = c.nk[slot];

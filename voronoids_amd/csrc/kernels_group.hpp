@@ -98,6 +98,7 @@ __global__ void __launch_bounds__(64, VO_CAVITY_WAVES) k_cavity_g(Ctx c, int nwi
     const bool lead = g.gl == 0;
     LocalCnt lc = {};
     round_params(c, nwin, key_hi);
+    const WinView wv = win_view(c);
     const SlotIter it = slot_iter(c, nwin, NG);
     for (int kb = it.base; kb < it.lim; kb += it.stride) {
     const int slot = it.off + kb + g.gid;
@@ -108,7 +109,7 @@ __global__ void __launch_bounds__(64, VO_CAVITY_WAVES) k_cavity_g(Ctx c, int nwi
     int4 tt = make_int4(0, 0, 0, 0);
     unsigned long long key = 0;
     if (st == ST_OK) {
-        p = c.win[slot];
+        p = win_point(wv, slot);
         pv = c.M + p;
         key = key_hi | (unsigned)p;
         pf = V::ld(c.X32, pv);
@@ -283,6 +284,7 @@ __global__ void __launch_bounds__(64, 5) k_commit_g(Ctx c, int nwin, unsigned lo
     const bool lead = g.gl == 0;
     LocalCnt lc = {};
     round_params(c, nwin, key_hi);
+    const WinView wv = win_view(c);
     const SlotIter it = slot_iter(c, nwin, NG);
     for (int kb = it.base; kb < it.lim; kb += it.stride) {
     const int slot = it.off + kb + g.gid;
@@ -292,7 +294,7 @@ __global__ void __launch_bounds__(64, 5) k_commit_g(Ctx c, int nwin, unsigned lo
     bool ok = true;
     if (valid) {
         st = c.status[slot];
-        p = c.win[slot];
+        p = win_point(wv, slot);
         key = key_hi | (unsigned)p;
         if (st == ST_OK) {
             nk = c.nk[slot];

@@ -357,6 +357,8 @@ __global__ void __launch_bounds__(kCntStripes) k_plan_one(RunPlan* P, RoundState
     p.nwin = nwin;
     p.ctl.nwin = nwin;
     p.ctl.npend = p.npend;
+    p.ctl.virt = 1;            // pending points, then fresh points head .. (not written out)
+    p.ctl.vhead = p.head;
     p.ctl.key_hi = (unsigned long long)(0xFFFFFFFFu - (uint32_t)p.rounds) << 32;
     st->n_next_pending = 0;
     st->n_winners = 0;
@@ -366,12 +368,6 @@ __global__ void __launch_bounds__(kCntStripes) k_plan_one(RunPlan* P, RoundState
     p.launched = 1;
 }
 
-// the planned window: pending points, then fresh points head .. head + nfresh - 1
-__global__ void k_fill_one(const RunPlan* P, int* win, const int* pend) {
-    const int nwin = P->ctl.nwin, np = P->ctl.npend, head = P->head;
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nwin; i += gridDim.x * blockDim.x)
-        win[i] = i < np ? pend[i] : head + (i - np);
-}
 
 
 
@@ -949,10 +945,10 @@ int run_device_planned(vo_tri* t) {
             const int slot = k & 1;
             for (int r = 0; r < R; ++r) {
                 hipLaunchKernelGGL(k_plan_one, dim3(1), dim3(kCntStripes), 0, t->s, t->plan, t->st, t->cnt);
-                hipLaunchKernelGGL(k_fill_one, dim3(std::max(1, std::min(1024, (est_hi + 255) / 256))), dim3(256), 0, t->s,
-                                   t->plan, t->win, t->pend[cur]);
+                // the window is virtual: the kernels read the pending list, then count fresh points
                 Ctx c = make_ctx(t);
                 c.ctl = &t->plan->ctl;
+                c.win = t->pend[cur];
                 c.pend_next = t->pend[cur ^ 1];
                 hipEvent_t* e = &t->rev[(size_t)(slot * R + r) * 3];
                 VO_HIP(hipEventRecord(e[0], t->s));
