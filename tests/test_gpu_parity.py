@@ -263,3 +263,38 @@ def test_device_planned_rounds_hand_over_exact_and_solo_rounds():
         assert seen[0] == seen[1], (dim, seen)
         solo += seen[1][3]
     assert solo > 0
+
+
+def test_full_size_3d_4m_invariants():
+    # the headline workload at full size (3D uniform n = 4,000,000): the exact device check
+    # (local Delaunay property of every face, adjacency and orientation consistency), vertex
+    # count, the super simplex's four hull faces, and the same rounds and windows from the
+    # device planner as from the host loop
+    n = 4_000_000
+    P = datasets.uniform(n, 3, seed=0)
+    seen = []
+    for pipelined in (1, 0):
+        t = vo.DelaunayTree(P)
+        t.config("pipelined", pipelined)
+        t.add_points_to_tree(P, seed=0)
+        assert t.violations() == 0
+        nv, ns, mx = t.counts()
+        st = t.stats()
+        assert nv == n + 4 and st["points_inserted"] == n and st["points_duplicate"] == 0
+        assert mx == st["created"] + 4          # d + 1 after DelaunayTree::new, plus one per created simplex
+        seen.append((st["rounds"], st["windows_total"], st["created"], st["killed"], ns))
+        if pipelined:
+            _, nb = t.export()
+            assert np.count_nonzero(nb < 0) == 4
+        t.close()
+    assert seen[0] == seen[1], seen
+
+
+def test_full_size_2d_10m_invariants():
+    # 2D uniform n = 10,000,000: exact device check and the triangle count of DT(P u S)
+    n = 10_000_000
+    t = vo.delaunay(datasets.uniform(n, 2, seed=0))
+    nv, ns, _ = t.counts()
+    assert nv == n + 3 and ns == 2 * n + 1
+    assert t.violations() == 0
+    t.close()
