@@ -298,3 +298,20 @@ def test_full_size_2d_10m_invariants():
     assert nv == n + 3 and ns == 2 * n + 1
     assert t.violations() == 0
     t.close()
+
+
+def test_full_size_3d_4m_order_and_engine_independent():
+    # DT(P u S) is unique for points in general position, so at the headline size two insertion
+    # orders (seeds) and the multi-set engine must give the same simplex set (order-free hash of
+    # the sorted vertex tuples in the reference id space)
+    from voronoids_amd.batch import simplex_set_hash
+    P = datasets.uniform(4_000_000, 3, seed=0)
+    hashes = []
+    for seed in (0, 1):
+        t = vo.delaunay(P, seed=seed)
+        s, _ = t.export()
+        hashes.append(simplex_set_hash(s))
+        t.close()
+    r = vo.batch_triangulate([P], seed=2)[0]
+    hashes.append(simplex_set_hash(r.simplices))
+    assert hashes[0] == hashes[1] == hashes[2], hashes
