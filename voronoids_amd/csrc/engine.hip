@@ -718,6 +718,10 @@ void launch_round_kernels(vo_engine* e, const Ctx& c, int G, int which) {
             if (which) hipLaunchKernelGGL(k_commit<2>, dim3(tblocks), dim3(256), 0, e->s, c, 0, 0ull);
             else hipLaunchKernelGGL(k_cavity<2>, dim3(tblocks), dim3(256), 0, e->s, c, 0, 0ull);
         }
+    } else if (e->D == 3 && which && G == 32) {
+        // as the one-set driver: a whole wavefront per point for the commit of small windows
+        const int cblocks = std::max(1, std::min(8192, e->win_cap));
+        hipLaunchKernelGGL((k_commit_g<3, 64>), dim3(cblocks), dim3(64), 0, e->s, c, 0, 0ull);
     } else if (e->D == 3) {
         switch (G) {
             case 4: VO_ENGINE_G(3, 4, which); break;

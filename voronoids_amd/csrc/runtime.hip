@@ -629,11 +629,13 @@ int compact(vo_tri* t) {
 // lanes per point (face tests in parallel, LDS lists, one run of loads per simplex) beat one
 // thread per point (measured: 2D n = 10M 114 -> 108 ms), so the choice stops at 4 unless
 // "group_lanes" = 1 asks for the thread kernels
-int auto_group(const vo_tri* t, int nwin) {
+// The 3D commit of small windows takes up to a whole wavefront per point (facets and ridge
+// rotations over 64 lanes): measured faster than 32 lanes for the commit, slower for the cavity.
+int auto_group(const vo_tri* t, int nwin, int which) {
     if (t->group) return t->group;
     const int64_t target = t->lanes_target ? t->lanes_target : (t->D == 3 ? (1 << 21) : (1 << 16));
-    static const int sizes[] = {32, 16, 8};
-    const int gmax = t->D == 3 ? 32 : 16;
+    static const int sizes[] = {64, 32, 16, 8};
+    const int gmax = t->D == 3 ? (which ? 64 : 32) : 16;
     for (int g : sizes)
         if (g <= gmax && (int64_t)nwin * g <= target) return g;
     return 4;
@@ -642,7 +644,7 @@ int auto_group(const vo_tri* t, int nwin) {
 // nwin sizes the grid (the kernels read the exact window from c.ctl when it is set); G = 0:
 // lanes per point chosen from nwin
 int launch_round(vo_tri* t, const Ctx& c, int nwin, unsigned long long key_hi, int which, int G = 0) {
-    if (!G) G = auto_group(t, nwin);
+    if (!G) G = auto_group(t, nwin, which);
     // with regions, a multiple of 8 blocks and slack for unequal region sizes
     auto blocks = [&](int per_block) {
         const int b = (nwin + per_block - 1) / per_block;
@@ -657,6 +659,8 @@ int launch_round(vo_tri* t, const Ctx& c, int nwin, unsigned long long key_hi, i
             if (which) hipLaunchKernelGGL(k_commit<2>, dim3(grid), dim3(256), 0, t->s, c, nwin, key_hi);
             else hipLaunchKernelGGL(k_cavity<2>, dim3(grid), dim3(256), 0, t->s, c, nwin, key_hi);
         }
+    } else if (t->D == 3 && which && G == 64) {
+        hipLaunchKernelGGL((k_commit_g<3, 64>), dim3(blocks(1)), dim3(64), 0, t->s, c, nwin, key_hi);
     } else if (t->D == 3) {
         switch (G) {
             case 4: VO_LAUNCH_G(3, 4); break;
@@ -941,7 +945,7 @@ int run_device_planned(vo_tri* t) {
             const double lag = inflight * R;
             const int est_mid = (int)std::min<double>(t->win_cap, base * std::pow(grow, lag + 0.5 * R));
             const int est_hi = (int)std::min<double>(t->win_cap, base * std::pow(grow, lag + R) + 64);
-            const int G = auto_group(t, est_mid);
+            const int Gc = auto_group(t, est_mid, 0), Gm = auto_group(t, est_mid, 1);
             const int slot = k & 1;
             for (int r = 0; r < R; ++r) {
                 hipLaunchKernelGGL(k_plan_one, dim3(1), dim3(kCntStripes), 0, t->s, t->plan, t->st, t->cnt);
@@ -952,9 +956,9 @@ int run_device_planned(vo_tri* t) {
                 c.pend_next = t->pend[cur ^ 1];
                 hipEvent_t* e = &t->rev[(size_t)(slot * R + r) * 3];
                 VO_HIP(hipEventRecord(e[0], t->s));
-                if ((rc = launch_round(t, c, est_hi, 0, 0, G))) return rc;
+                if ((rc = launch_round(t, c, est_hi, 0, 0, Gc))) return rc;
                 VO_HIP(hipEventRecord(e[1], t->s));
-                if ((rc = launch_round(t, c, est_hi, 0, 1, G))) return rc;
+                if ((rc = launch_round(t, c, est_hi, 0, 1, Gm))) return rc;
                 VO_HIP(hipEventRecord(e[2], t->s));
                 cur ^= 1;
             }
