@@ -306,12 +306,33 @@ def test_full_size_3d_4m_order_and_engine_independent():
     # the sorted vertex tuples in the reference id space)
     from voronoids_amd.batch import simplex_set_hash
     P = datasets.uniform(4_000_000, 3, seed=0)
-    hashes = []
+    hashes, canon = [], []
     for seed in (0, 1):
         t = vo.delaunay(P, seed=seed)
         s, _ = t.export()
         hashes.append(simplex_set_hash(s))
+        canon.append(t.canonical())          # sorted on the device
         t.close()
     r = vo.batch_triangulate([P], seed=2)[0]
     hashes.append(simplex_set_hash(r.simplices))
     assert hashes[0] == hashes[1] == hashes[2], hashes
+    assert np.array_equal(canon[0][0], canon[1][0]) and np.array_equal(canon[0][1], canon[1][1])
+    assert canon[0][0].shape[1] == 4 and canon[0][1].shape[1] == 2
+
+
+@pytest.mark.parametrize("dim,n", [(3, 1), (3, 5), (2, 1), (2, 7), (3, 250_000), (2, 600_000)])
+def test_device_canonical_equals_host_canonical(dim, n):
+    # the canonical form is sorted on the device (radix sorts); the host routine the engine's
+    # results use must give the same arrays, from tiny sets (no real simplex) to many tiles
+    from voronoids_amd.batch import canonical_from_simplices
+    P = datasets.uniform(n, dim, seed=170 + n % 97)
+    t = vo.delaunay(P, seed=3)
+    T, E = t.canonical()
+    s, _ = t.export()
+    Th, Eh = canonical_from_simplices(s, dim)
+    assert T.shape == Th.shape and E.shape == Eh.shape
+    assert np.array_equal(T, Th) and np.array_equal(E, Eh)
+    if n > 1000:
+        assert T.shape[0] > 0 and E.shape[0] > 0
+        assert np.all(np.diff(E[:, 0]) >= 0)
+    t.close()

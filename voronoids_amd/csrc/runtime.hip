@@ -23,6 +23,7 @@
 #include "kernels.hpp"
 #include "kernels_group.hpp"
 #include "compact.hpp"
+#include "canon.hpp"
 
 #pragma clang fp contract(off)
 
@@ -997,6 +998,76 @@ int run_device_planned(vo_tri* t) {
 
 }  // namespace
 
+namespace {
+
+// RAII device buffer for the canonical export's temporaries
+template <class T> struct DevBuf {
+    T* p = nullptr;
+    ~DevBuf() { if (p) hipFree(p); }
+};
+
+// exclusive scan of n ints (device), total of the inputs to *total
+int device_excl_scan(hipStream_t s, const int* in, int* out, int64_t n, int64_t* total) {
+    if (n <= 0) {
+        *total = 0;
+        return 0;
+    }
+    const int64_t nb = (n + 255) / 256;
+    DevBuf<int> bsum, boff;
+    int rc;
+    if ((rc = dev_alloc(&bsum.p, (size_t)nb)) || (rc = dev_alloc(&boff.p, (size_t)nb))) return rc;
+    hipLaunchKernelGGL(k_scan_blocks, dim3((unsigned)nb), dim3(256), 0, s, in, out, bsum.p, n);
+    VO_HIP(hipGetLastError());
+    int64_t t2 = 0;
+    if (nb > 1) {
+        if ((rc = device_excl_scan(s, bsum.p, boff.p, nb, &t2))) return rc;
+        hipLaunchKernelGGL(k_scan_add, dim3((unsigned)nb), dim3(256), 0, s, out, boff.p, n);
+        VO_HIP(hipGetLastError());
+    }
+    int last_in = 0, last_out = 0;
+    VO_HIP(hipMemcpyAsync(&last_out, out + n - 1, sizeof(int), hipMemcpyDeviceToHost, s));
+    VO_HIP(hipMemcpyAsync(&last_in, in + n - 1, sizeof(int), hipMemcpyDeviceToHost, s));
+    VO_HIP(hipStreamSynchronize(s));
+    *total = (int64_t)last_out + last_in;
+    return 0;
+}
+
+// stable LSD radix sort of n 64-bit keys (only the low `bits` bits are compared) with an
+// optional 32-bit payload; keys/vals and the temporaries swap roles each pass, so the result is
+// in keys/vals on return
+int radix_sort(hipStream_t s, unsigned long long*& keys, uint32_t*& vals, unsigned long long*& ktmp, uint32_t*& vtmp,
+               int64_t n, int bits) {
+    if (n <= 1) return 0;
+    const int64_t ntiles = (n + kRadixTile - 1) / kRadixTile;
+    DevBuf<int> hist, off;
+    int rc;
+    if ((rc = dev_alloc(&hist.p, (size_t)(256 * ntiles))) || (rc = dev_alloc(&off.p, (size_t)(256 * ntiles)))) return rc;
+    for (int shift = 0; shift < bits; shift += 8) {
+        hipLaunchKernelGGL(k_radix_hist, dim3((unsigned)ntiles), dim3(kRadixThreads), 0, s, keys, n, shift, hist.p, (int)ntiles);
+        VO_HIP(hipGetLastError());
+        int64_t tot = 0;
+        if ((rc = device_excl_scan(s, hist.p, off.p, 256 * ntiles, &tot))) return rc;
+        if (tot != n) return fail(VO_ERR_INTERNAL, "radix sort: histogram total differs from the key count");
+        hipLaunchKernelGGL(k_radix_scatter, dim3((unsigned)ntiles), dim3(kRadixThreads), 0, s, keys, vals, ktmp, vtmp, n,
+                           shift, off.p, (int)ntiles);
+        VO_HIP(hipGetLastError());
+        std::swap(keys, ktmp);
+        std::swap(vals, vtmp);
+    }
+    return 0;
+}
+
+int bit_length(int64_t x) {
+    int b = 0;
+    while (x > 0) {
+        ++b;
+        x >>= 1;
+    }
+    return b;
+}
+
+}  // namespace
+
 // =================================================================== C ABI
 extern "C" {
 
@@ -1269,19 +1340,100 @@ int vo_tri_export(const vo_tri* t, int32_t* simplices, int32_t* neighbors) {
     return 0;
 }
 
+// Canonical form on the device (canon.hpp): real simplices as sorted tuples in lexicographic
+// order (two stable radix sorts: columns 2-3, then columns 0-1), real edges sorted and unique.
+// The same output as canonical_from_simplices on the host, which the engine's results use.
+static int canonical_device(const vo_tri* t, int32_t** T, int64_t* n_T, int32_t** E, int64_t* n_E) {
+    const int M = t->M;
+    const hipStream_t s = t->s;
+    DevBuf<int32_t> ds, dn;
+    int64_t nl = 0;
+    int rc = export_device(t, &ds.p, &dn.p, &nl);
+    if (rc) return rc;
+    hipFree(dn.p);
+    dn.p = nullptr;
+    const int first = 2 * M;
+    const int nb = std::max(1, bit_length((int64_t)t->input_of.size() - 1));
+    const unsigned grid = (unsigned)std::max<int64_t>(1, std::min<int64_t>(4096, (nl + 255) / 256));
+    // ---- simplices
+    int64_t nt = 0;
+    DevBuf<int32_t> dT;
+    {
+        DevBuf<int4> rows, crow;
+        DevBuf<int> flag, pos;
+        if ((rc = dev_alloc(&rows.p, (size_t)std::max<int64_t>(nl, 1))) || (rc = dev_alloc(&flag.p, (size_t)std::max<int64_t>(nl, 1))) ||
+            (rc = dev_alloc(&pos.p, (size_t)std::max<int64_t>(nl, 1))))
+            return rc;
+        if (M == 4) hipLaunchKernelGGL(k_canon_rows<4>, dim3(grid), dim3(256), 0, s, ds.p, nl, first, rows.p, flag.p);
+        else hipLaunchKernelGGL(k_canon_rows<3>, dim3(grid), dim3(256), 0, s, ds.p, nl, first, rows.p, flag.p);
+        VO_HIP(hipGetLastError());
+        if ((rc = device_excl_scan(s, flag.p, pos.p, nl, &nt))) return rc;
+        const size_t nt1 = (size_t)std::max<int64_t>(nt, 1);
+        DevBuf<unsigned long long> klo, khi, ktmp;
+        DevBuf<uint32_t> perm, ptmp;
+        if ((rc = dev_alloc(&crow.p, nt1)) || (rc = dev_alloc(&klo.p, nt1)) || (rc = dev_alloc(&khi.p, nt1)) ||
+            (rc = dev_alloc(&ktmp.p, nt1)) || (rc = dev_alloc(&perm.p, nt1)) || (rc = dev_alloc(&ptmp.p, nt1)) ||
+            (rc = dev_alloc(&dT.p, nt1 * M)))
+            return rc;
+        if (M == 4) hipLaunchKernelGGL(k_canon_keys<4>, dim3(grid), dim3(256), 0, s, rows.p, flag.p, pos.p, nl, nb, crow.p, klo.p, khi.p);
+        else hipLaunchKernelGGL(k_canon_keys<3>, dim3(grid), dim3(256), 0, s, rows.p, flag.p, pos.p, nl, nb, crow.p, klo.p, khi.p);
+        hipLaunchKernelGGL(k_iota_u32, dim3(grid), dim3(256), 0, s, perm.p, nt);
+        VO_HIP(hipGetLastError());
+        // least significant columns first, then the leading two (stable): lexicographic order
+        if ((rc = radix_sort(s, klo.p, perm.p, ktmp.p, ptmp.p, nt, M == 4 ? 2 * nb : nb))) return rc;
+        hipLaunchKernelGGL(k_gather_u64, dim3(grid), dim3(256), 0, s, khi.p, perm.p, klo.p, nt);
+        VO_HIP(hipGetLastError());
+        if ((rc = radix_sort(s, klo.p, perm.p, ktmp.p, ptmp.p, nt, 2 * nb))) return rc;
+        if (M == 4) hipLaunchKernelGGL(k_canon_out_rows<4>, dim3(grid), dim3(256), 0, s, crow.p, perm.p, nt, dT.p);
+        else hipLaunchKernelGGL(k_canon_out_rows<3>, dim3(grid), dim3(256), 0, s, crow.p, perm.p, nt, dT.p);
+        VO_HIP(hipGetLastError());
+    }
+    // ---- edges
+    int64_t ne = 0;
+    DevBuf<int32_t> dE;
+    {
+        const int P = M * (M - 1) / 2;
+        const int64_t nk = nl * P;
+        const size_t nk1 = (size_t)std::max<int64_t>(nk, 1);
+        DevBuf<unsigned long long> keys, ktmp;
+        DevBuf<int> flag, pos;
+        uint32_t* nov = nullptr;
+        uint32_t* novt = nullptr;
+        if ((rc = dev_alloc(&keys.p, nk1)) || (rc = dev_alloc(&ktmp.p, nk1)) || (rc = dev_alloc(&flag.p, nk1)) ||
+            (rc = dev_alloc(&pos.p, nk1)))
+            return rc;
+        if (M == 4) hipLaunchKernelGGL(k_canon_edge_keys<4>, dim3(grid), dim3(256), 0, s, ds.p, nl, first, nb, keys.p);
+        else hipLaunchKernelGGL(k_canon_edge_keys<3>, dim3(grid), dim3(256), 0, s, ds.p, nl, first, nb, keys.p);
+        VO_HIP(hipGetLastError());
+        if ((rc = radix_sort(s, keys.p, nov, ktmp.p, novt, nk, 2 * nb))) return rc;
+        const unsigned egrid = (unsigned)std::max<int64_t>(1, std::min<int64_t>(8192, (nk + 255) / 256));
+        hipLaunchKernelGGL(k_canon_edge_flags, dim3(egrid), dim3(256), 0, s, keys.p, nk, flag.p);
+        VO_HIP(hipGetLastError());
+        if ((rc = device_excl_scan(s, flag.p, pos.p, nk, &ne))) return rc;
+        if ((rc = dev_alloc(&dE.p, (size_t)std::max<int64_t>(ne, 1) * 2))) return rc;
+        hipLaunchKernelGGL(k_canon_edge_out, dim3(egrid), dim3(256), 0, s, keys.p, flag.p, pos.p, nk, nb, dE.p);
+        VO_HIP(hipGetLastError());
+    }
+    *T = (int32_t*)std::malloc(sizeof(int32_t) * (size_t)std::max<int64_t>(1, nt * M));
+    *E = (int32_t*)std::malloc(sizeof(int32_t) * (size_t)std::max<int64_t>(1, ne * 2));
+    if (!*T || !*E) {
+        std::free(*T);
+        std::free(*E);
+        *T = *E = nullptr;
+        return fail(VO_ERR_OOM, "host allocation");
+    }
+    if (nt) VO_HIP(hipMemcpyAsync(*T, dT.p, sizeof(int32_t) * (size_t)nt * M, hipMemcpyDeviceToHost, s));
+    if (ne) VO_HIP(hipMemcpyAsync(*E, dE.p, sizeof(int32_t) * (size_t)ne * 2, hipMemcpyDeviceToHost, s));
+    VO_HIP(hipStreamSynchronize(s));
+    *n_T = nt;
+    *n_E = ne;
+    return 0;
+}
+
 int vo_tri_canonical(const vo_tri* t, int32_t** T, int64_t* n_T, int32_t** E, int64_t* n_E) {
     if (!t || !T || !n_T || !E || !n_E) return fail(VO_ERR_INVALID, "null argument");
     VO_HIP(hipSetDevice(t->dev));
-    int32_t *ds, *dn;
-    int64_t nl;
-    int rc = export_device(t, &ds, &dn, &nl);
-    if (rc) return rc;
-    const int M = t->M;
-    std::vector<int32_t> s((size_t)nl * M);
-    VO_HIP(hipMemcpy(s.data(), ds, sizeof(int32_t) * s.size(), hipMemcpyDeviceToHost));
-    hipFree(ds);
-    hipFree(dn);
-    return canonical_from_simplices(M, s.data(), nl, T, n_T, E, n_E);
+    return canonical_device(t, T, n_T, E, n_E);
 }
 
 int vo_tri_check(vo_tri* t, int64_t* n_violations) {
