@@ -351,6 +351,9 @@ def main() -> int:
     ap.add_argument("--no-precision", action="store_true", help="skip the precision-tier measurement")
     ap.add_argument("--only", choices=["batched", "served", "precision"], help="run only that measurement")
     ap.add_argument("--check", action="store_true", help="run the exact device check after the last step")
+    ap.add_argument("--kernel-timing-steps", type=int, default=-1, metavar="K",
+                    help="further steps with HIP events around every round's kernels, for the roofline "
+                         "(default: as many as --steps; 0: none, e.g. under a profiler)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run device-to-device copies of BYTES (for rocprofv3 counter scaling)")
     args = ap.parse_args()
@@ -373,21 +376,27 @@ def main() -> int:
     n, dim = P.shape[0], P.shape[1]
     print(f"# workload {args.workload}: {w['label']}", file=sys.stderr, flush=True)
 
-    runs = []
+    # timed steps as shipped (no per-round events), then as many steps with HIP events around
+    # every round's cavity and commit kernels: the per-kernel times of the roofline
+    runs, runs_t = [], []
     last = None
-    for i in range(args.warmup + args.steps):
+    kts = args.steps if args.kernel_timing_steps < 0 else args.kernel_timing_steps
+    for i in range(args.warmup + args.steps + kts):
+        timed_kernels = i >= args.warmup + args.steps
         t = vo.DelaunayTree(P)
+        if timed_kernels:
+            t.config("round_timing", 1)
         t.stage(P, seed=0)
         t0 = time.perf_counter()
         t.run()                                   # synchronizes the device before returning
         wall = time.perf_counter() - t0
         st = t.stats()
-        phase = "warmup" if i < args.warmup else "step"
+        phase = "warmup" if i < args.warmup else ("kernel-timed step" if timed_kernels else "step")
         print(f"# {phase} {i}: run {st['ms_run']:.1f} ms (wall {wall * 1e3:.1f} ms), rounds {st['rounds']}, "
               f"cavity {st['ms_locate_cavity']:.1f} ms, commit {st['ms_commit']:.1f} ms",
               file=sys.stderr, flush=True)
         if i >= args.warmup:
-            runs.append(st)
+            (runs_t if timed_kernels else runs).append(st)
         if last is not None:
             last.close()
         last = t
@@ -398,13 +407,13 @@ def main() -> int:
 
     ms = float(np.mean([r["ms_run"] for r in runs]))
     value = n / (ms / 1e3)
-    st = runs[-1]
+    st = (runs_t or runs)[-1]
     alg = algorithmic_bytes(dim, n, st)
-    kt = {"k_cavity": float(np.mean([r["ms_locate_cavity"] for r in runs])),
-          "k_commit": float(np.mean([r["ms_commit"] for r in runs]))}
+    kt = {"k_cavity": float(np.mean([r["ms_locate_cavity"] for r in runs_t])) if runs_t else 0.0,
+          "k_commit": float(np.mean([r["ms_commit"] for r in runs_t])) if runs_t else 0.0}
     launches = {"k_cavity": st["launches_locate_cavity"], "k_commit": st["launches_commit"]}
     dom = max(kt, key=kt.get)
-    achieved = alg[dom] / (kt[dom] / 1e3) / 1e9
+    achieved = alg[dom] / (kt[dom] / 1e3) / 1e9 if kt[dom] > 0 else None
     traffic = measured_traffic(args.workload, dom)
     tot_calls = [st["tier1"][i] + st["tier2"][i] + st["tier3"][i] + st["deferred"][i] for i in range(2)]
     tiers = {name: {"fp32": st["tier1"][i] / max(tot_calls[i], 1), "fp64_filter": st["tier2"][i] / max(tot_calls[i], 1),
@@ -419,11 +428,14 @@ def main() -> int:
         "dtype": "f64", "data": "synthetic",
         "config": {"workload": w["label"], "n": n, "dim": dim, "timed": "vo_tri_run, inputs resident in HBM"},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS,
+                     "frac": None if achieved is None else achieved / HBM_PEAK_GBS,
                      "traffic": None if traffic is None else traffic["bytes_per_launch"],
                      "traffic_source": traffic,
                      "alg_bytes_per_launch": alg[dom] / max(launches[dom], 1),
-                     "avg_launch_ms": kt[dom] / max(launches[dom], 1), "launches": launches[dom]},
+                     "avg_launch_ms": kt[dom] / max(launches[dom], 1), "launches": launches[dom],
+                     "kernel_times": (f"HIP events around every round's kernels in {len(runs_t)} further steps "
+                                      f"({float(np.mean([r['ms_run'] for r in runs_t])):.1f} ms per step with the events)")
+                                     if runs_t else None},
         "kernels_ms": kt, "rounds": st["rounds"], "tiers": tiers,
         "events": {k: st[k] for k in ("points_inserted", "killed", "created", "walk_steps", "chain_hops",
                                       "max_killed", "max_facets", "windows_total", "exact_evaluated")},

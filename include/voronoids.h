@@ -109,7 +109,9 @@ int vo_tri_stats(const vo_tri* t, vo_stats* out);
  * in turn) caches one region's records; measured no faster at n = 4M, kept as an option.  "pipelined" (1
  * default / 0): rounds planned on the device and submitted in chunks, the host taking over only when the
  * planner halts; 0 plans and waits for every round on the host (compaction, XCD regions and VO_TRACE always
- * do); both give the same rounds and result.  "chunk_rounds" (default 4, 1..64): rounds per submitted chunk. */
+ * do); both give the same rounds and result.  "chunk_rounds" (default 4, 1..64): rounds per submitted chunk.
+ * "round_timing" (0 default / 1): HIP events around every round's cavity and commit kernels, reported in
+ * vo_stats ms_locate_cavity / ms_commit (the host loop always times; the events cost about 3% at n = 4M). */
 int vo_tri_config(vo_tri* t, const char* key, int64_t value);
 
 /* The precision tiers one predicate at a time (no counterpart in the reference, whose in_sphere

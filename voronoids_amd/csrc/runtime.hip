@@ -157,6 +157,7 @@ struct vo_tri {
     hipEvent_t chunk_ev[2] = {};
     int pipelined = 1;                 // 0: the host plans every round and waits for it
     int chunk_rounds = 4;              // rounds submitted per chunk
+    int round_timing = 0;              // events around each round's kernels (cost: ~3% at n = 4M)
     int64_t stalled = 0;
     double gamma = 0.25;
     int filters = 3;
@@ -892,6 +893,7 @@ void plan_to_host(vo_tri* t, const RunPlan& p) {
 
 // cavity and commit kernel time of the R rounds of chunk slot `slot`
 void add_chunk_times(vo_tri* t, int slot, int R) {
+    if (!t->round_timing) return;
     for (int r = 0; r < R; ++r) {
         hipEvent_t* e = &t->rev[(size_t)(slot * R + r) * 3];
         float a = 0, b = 0;
@@ -956,11 +958,11 @@ int run_device_planned(vo_tri* t) {
                 c.win = t->pend[cur];
                 c.pend_next = t->pend[cur ^ 1];
                 hipEvent_t* e = &t->rev[(size_t)(slot * R + r) * 3];
-                VO_HIP(hipEventRecord(e[0], t->s));
+                if (t->round_timing) VO_HIP(hipEventRecord(e[0], t->s));
                 if ((rc = launch_round(t, c, est_hi, 0, 0, Gc))) return rc;
-                VO_HIP(hipEventRecord(e[1], t->s));
+                if (t->round_timing) VO_HIP(hipEventRecord(e[1], t->s));
                 if ((rc = launch_round(t, c, est_hi, 0, 1, Gm))) return rc;
-                VO_HIP(hipEventRecord(e[2], t->s));
+                if (t->round_timing) VO_HIP(hipEventRecord(e[2], t->s));
                 cur ^= 1;
             }
             VO_HIP(hipMemcpyAsync(&t->plan_host[slot], t->plan, sizeof(RunPlan), hipMemcpyDeviceToHost, t->s));
@@ -1543,6 +1545,9 @@ int vo_tri_config(vo_tri* t, const char* key, int64_t value) {
     } else if (k == "pipelined") {
         if (value != 0 && value != 1) return fail(VO_ERR_INVALID, "pipelined must be 0 or 1");
         t->pipelined = (int)value;
+    } else if (k == "round_timing") {
+        if (value != 0 && value != 1) return fail(VO_ERR_INVALID, "round_timing must be 0 or 1");
+        t->round_timing = (int)value;
     } else if (k == "chunk_rounds") {
         if (value < 1 || value > 64) return fail(VO_ERR_INVALID, "chunk_rounds must be in 1..64");
         t->chunk_rounds = (int)value;
