@@ -304,15 +304,22 @@ __global__ void __launch_bounds__(64, 5) k_commit_g(Ctx c, int nwin, unsigned lo
             // unconditional loads (indices clamped to the lists, never empty for an OK point):
             // three dependent loads per pass
             const int n_it = nk > nf ? nk : nf;
-            for (int j = g.gl; j < n_it; j += G) {
-                const int jk = j < nk ? j : nk - 1, jf = j < nf ? j : nf - 1;
-                const int x = c.cav[(size_t)slot * c.kcap + jk];
-                const int fe = c.fac[(size_t)slot * c.fcap + jf];
-                const ulonglong2 r = ld_res<D>(c, x);
-                const int nb = *a_ptr<D>(c, fe >> 2, fe & 3);
-                const unsigned long long ko = *kres_p<D>(c, nb >= 0 ? nb >> 2 : x);
-                if (j < nf) s_fac[g.gid][j] = fe;
-                ok = ok & (j >= nk || ((r.x == key) & (r.y > key))) & (j >= nf || nb < 0 || ko > key);
+            // two entries per lane and pass, their loads interleaved (with few lanes per point,
+            // as in large windows, the passes would otherwise run one after another)
+            for (int j0 = g.gl; j0 < n_it; j0 += 2 * G) {
+                const int j1 = j0 + G;
+                const int jk0 = j0 < nk ? j0 : nk - 1, jf0 = j0 < nf ? j0 : nf - 1;
+                const int jk1 = j1 < nk ? j1 : nk - 1, jf1 = j1 < nf ? j1 : nf - 1;
+                const int x0 = c.cav[(size_t)slot * c.kcap + jk0], x1 = c.cav[(size_t)slot * c.kcap + jk1];
+                const int fe0 = c.fac[(size_t)slot * c.fcap + jf0], fe1 = c.fac[(size_t)slot * c.fcap + jf1];
+                const ulonglong2 r0 = ld_res<D>(c, x0), r1 = ld_res<D>(c, x1);
+                const int nb0 = *a_ptr<D>(c, fe0 >> 2, fe0 & 3), nb1 = *a_ptr<D>(c, fe1 >> 2, fe1 & 3);
+                const unsigned long long ko0 = *kres_p<D>(c, nb0 >= 0 ? nb0 >> 2 : x0);
+                const unsigned long long ko1 = *kres_p<D>(c, nb1 >= 0 ? nb1 >> 2 : x1);
+                if (j0 < nf) s_fac[g.gid][j0] = fe0;
+                if (j1 < nf) s_fac[g.gid][j1] = fe1;
+                ok = ok & (j0 >= nk || ((r0.x == key) & (r0.y > key))) & (j0 >= nf || nb0 < 0 || ko0 > key);
+                ok = ok & (j1 >= nk || ((r1.x == key) & (r1.y > key))) & (j1 >= nf || nb1 < 0 || ko1 > key);
             }
         }
     }
