@@ -85,6 +85,9 @@ struct RunPlan {
     int launched;              // a planned round ran and is not absorbed yet
     int halt;                  // HALT_* bits
     int win_cap, big_window, per_point;
+    int exact_dev;             // queued exact predicates are evaluated in the stream (k_exact after
+                               // each commit) instead of halting for the host
+    int nq_done;               // the halted round's queue count, evaluated in the stream
 };
 
 struct vo_tri {
@@ -158,6 +161,7 @@ struct vo_tri {
     int pipelined = 1;                 // 0: the host plans every round and waits for it
     int chunk_rounds = 4;              // rounds submitted per chunk
     int round_timing = 0;              // events around each round's kernels (cost: ~3% at n = 4M)
+    int exact_dev = 0;                 // this run has queued exact predicates: evaluate them in the stream
     int64_t stalled = 0;
     double gamma = 0.25;
     int filters = 3;
@@ -309,13 +313,25 @@ __global__ void __launch_bounds__(kCntStripes) k_round_totals(const Counters* cn
     round_totals(cnt, st);
 }
 
+__device__ void plan_step(RunPlan& p, RoundState* st, bool absorb);
+
 __global__ void __launch_bounds__(kCntStripes) k_plan_one(RunPlan* P, RoundState* st, const Counters* cnt) {
     __shared__ int s_absorb;
     RunPlan& p = *P;
-    if (threadIdx.x == 0) s_absorb = p.launched && !(st->n_big > 0 || st->n_q > 0 || st->err);
+    if (threadIdx.x == 0) s_absorb = p.launched && !(st->n_big > 0 || (st->n_q > 0 && !p.exact_dev) || st->err);
     __syncthreads();
     if (s_absorb) round_totals(cnt, st);
     if (threadIdx.x != 0) return;
+    plan_step(p, st, s_absorb);
+    // halted with the last queue already evaluated in the stream: the empty rounds that follow
+    // must not evaluate it again (the host reads the count from nq_done)
+    if (p.halt && p.exact_dev && st->n_q > 0) {
+        p.nq_done = st->n_q;
+        st->n_q = 0;
+    }
+}
+
+__device__ void plan_step(RunPlan& p, RoundState* st, bool s_absorb) {
     p.ctl.nwin = 0;
     if (p.launched) {
         if (!s_absorb) {
@@ -333,7 +349,7 @@ __global__ void __launch_bounds__(kCntStripes) k_plan_one(RunPlan* P, RoundState
         const double hi = big ? p.big_frac_hi : 0.6, lo = big ? p.big_frac_lo : 0.35;
         if (frac > hi) p.gamma = fmin(p.gamma * 1.5, 4.0);
         else if (frac < lo) p.gamma = fmax(p.gamma * 0.7, 0.002);
-        p.stalled = (w == 0 && consumed == 0) ? p.stalled + 1 : 0;
+        p.stalled = (w == 0 && consumed == 0 && st->n_q == 0) ? p.stalled + 1 : 0;
         p.npend = nx;
         p.cur ^= 1;
         p.head += p.nfresh;
@@ -753,13 +769,13 @@ int64_t window_records(int D) { return D == 3 ? 64 : 16; }
 // The host's side of a finished round whose state is in st_host: errors, points re-run alone,
 // the exact queue, then the counters and the window factor (k_plan_one applies the same rule
 // on the device).
-int finish_round(vo_tri* t, int nwin, int nfresh, FILE* trace, float ms_cav, float ms_com) {
+int finish_round(vo_tri* t, int nwin, int nfresh, FILE* trace, float ms_cav, float ms_com, bool exact_done = false) {
     int rc;
     if ((rc = check_err(t))) return rc;
     const int n_big = t->st_host->n_big;
     if (n_big > 0 && (rc = run_solo(t, n_big))) return rc;
     const RoundState& r = *t->st_host;
-    if (r.n_q > 0) {
+    if (r.n_q > 0 && !exact_done) {      // (exact_done: evaluated in the stream after the commit)
         if ((rc = run_exact(t, r.n_q))) return rc;
         if ((rc = sync_state(t)) || (rc = check_err(t))) return rc;
     }
@@ -875,6 +891,7 @@ RunPlan plan_from_host(const vo_tri* t) {
     p.win_cap = t->win_cap;
     p.big_window = (int)t->big_window;
     p.per_point = (int)window_records(t->D);
+    p.exact_dev = t->exact_dev;
     return p;
 }
 
@@ -963,6 +980,17 @@ int run_device_planned(vo_tri* t) {
                 if (t->round_timing) VO_HIP(hipEventRecord(e[1], t->s));
                 if ((rc = launch_round(t, c, est_hi, 0, 1, Gm))) return rc;
                 if (t->round_timing) VO_HIP(hipEventRecord(e[2], t->s));
+                if (t->exact_dev) {
+                    // the round's queued predicates (count read from the round state; none: the
+                    // kernel returns at once), before the planner absorbs the round
+                    if (D == 3)
+                        hipLaunchKernelGGL(k_exact<4>, dim3(t->ws_threads / 64), dim3(64), 0, t->s, t->q, -1, t->X64, t->ws,
+                                           t->ws_cap, t->cache, t->cmask, t->st, t->cnt, t->qcap);
+                    else
+                        hipLaunchKernelGGL(k_exact<2>, dim3(t->ws_threads / 64), dim3(64), 0, t->s, t->q, -1, t->X64, t->ws,
+                                           t->ws_cap, t->cache, t->cmask, t->st, t->cnt, t->qcap);
+                    VO_HIP(hipGetLastError());
+                }
                 cur ^= 1;
             }
             VO_HIP(hipMemcpyAsync(&t->plan_host[slot], t->plan, sizeof(RunPlan), hipMemcpyDeviceToHost, t->s));
@@ -991,7 +1019,11 @@ int run_device_planned(vo_tri* t) {
         t->n_com += P.rounds - t->rounds + P.launched;
         plan_to_host(t, P);
         t->n_tets = t->st_host->n_tets;
-        if (P.launched && (rc = finish_round(t, P.nwin, P.nfresh, nullptr, 0.f, 0.f))) return rc;
+        // a round that queued exact predicates: the rest of this run evaluates them in the stream
+        const bool exact_done = P.exact_dev != 0;
+        if (exact_done && P.nq_done > 0) t->st_host->n_q = P.nq_done;
+        if (P.launched && t->st_host->n_q > 0) t->exact_dev = 1;
+        if (P.launched && (rc = finish_round(t, P.nwin, P.nfresh, nullptr, 0.f, 0.f, exact_done))) return rc;
         if ((rc = check_err(t))) return rc;
         if (P.halt & HALT_STALL) return fail(VO_ERR_INTERNAL, "no progress for 64 rounds");
     }
@@ -1258,6 +1290,7 @@ int vo_tri_run(vo_tri* t) {
     t->ms_cav = t->ms_com = t->ms_ex = t->ms_other = 0;
     t->n_cav = t->n_com = 0;
     t->stalled = 0;
+    t->exact_dev = 0;
     const int64_t dups0 = t->dups;
     VO_HIP(hipEventRecord(t->ev[0], t->s));
     // VO_TRACE=<path>: append one line per round (diagnostics only; rounds planned on the host)
