@@ -1,0 +1,49 @@
+"""The C++ facade (include/voronoids.hpp) over the C ABI.
+
+CPU: the header compiles with every member of DelaunayTree<3, 4> and <2, 3> instantiated.
+GPU: tests/cpp/_bin/test_delaunay_tree (built by tests/cpp/Makefile from build()) runs the
+reference's tree tests (tests/test_delaunay_tree.rs) through the facade and checks each result
+against the oracle; it exits non-zero on any failed check.
+"""
+import os
+import shutil
+import subprocess
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+BIN = os.path.join(ROOT, "tests", "cpp", "_bin", "test_delaunay_tree")
+
+_INSTANTIATE = r"""
+#include "voronoids.hpp"
+template class voronoids::delaunay_tree::DelaunayTree<3, 4>;
+template class voronoids::delaunay_tree::DelaunayTree<2, 3>;
+template class voronoids::delaunay_tree::TreeUpdate<3, 4>;
+template class voronoids::delaunay_tree::TreeUpdate<2, 3>;
+template voronoids::delaunay_tree::DelaunayTree<3, 4> voronoids::delaunay_tree::delaunay<3>(
+    const std::vector<std::array<double, 3>>&, uint64_t, int);
+template voronoids::delaunay_tree::DelaunayTree<2, 3> voronoids::delaunay_tree::delaunay<2>(
+    const std::vector<std::array<double, 2>>&, uint64_t, int);
+"""
+
+
+def test_facade_header_compiles(tmp_path):
+    cxx = shutil.which("g++")
+    if cxx is None:
+        pytest.skip("g++ not found")
+    src = tmp_path / "inst.cpp"
+    src.write_text(_INSTANTIATE)
+    r = subprocess.run([cxx, "-std=c++17", "-fsyntax-only", "-Wall", "-Wextra", "-Werror",
+                        "-I", os.path.join(ROOT, "include"), str(src)], capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+
+
+@pytest.mark.gpu
+def test_cpp_delaunay_tree_facade():
+    assert os.path.exists(BIN), "tests/cpp/_bin/test_delaunay_tree is missing: run __graft_entry__.build()"
+    print(f"running {BIN}", flush=True)
+    r = subprocess.run([BIN], capture_output=True, text=True, timeout=300)
+    print(r.stdout, flush=True)
+    assert r.returncode == 0, r.stdout + r.stderr
+    for name in ("test_delaunay_tree_3d", "test_delaunay_tree_2d", "test_delaunay_function", "test_errors"):
+        assert f"{name} passed" in r.stdout

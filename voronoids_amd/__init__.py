@@ -109,14 +109,15 @@ class DelaunayTree:
 
     # -- construction -----------------------------------------------------------------
     def add_points_to_tree(self, vertices, seed: int = 0) -> None:
-        p = _as_points(vertices, self.dim)
-        self._n_input += p.shape[0]
-        _lib.check(_lib.lib().vo_tri_insert(self._h, p.ctypes.data_as(_dp), p.shape[0], seed))
+        self.stage(vertices, seed)
+        self.run()
 
     def stage(self, vertices, seed: int = 0) -> None:
+        """a failed stage consumes no vertex ids; a successful one consumes one per point, also
+        for points the run later rejects as duplicates"""
         p = _as_points(vertices, self.dim)
-        self._n_input += p.shape[0]
         _lib.check(_lib.lib().vo_tri_stage(self._h, p.ctypes.data_as(_dp), p.shape[0], seed))
+        self._n_input += p.shape[0]
 
     def run(self) -> None:
         _lib.check(_lib.lib().vo_tri_run(self._h))

@@ -1247,8 +1247,10 @@ int vo_tri_stage(vo_tri* t, const double* pts, int64_t n, uint64_t seed) {
     std::vector<double> x64((size_t)n * S, 0.0);
     std::vector<float> x32((size_t)n * S, 0.0f);
     std::vector<int> vm(n);
-    const int64_t base_in = (int64_t)t->input_of.size();
-    t->input_of.resize(base_in + n);
+    // ids are committed (n_pts, input_of) only once every upload and allocation has succeeded,
+    // so that a failed stage leaves the handle as it was
+    const int64_t base_in = t->n_pts;
+    std::vector<int32_t> in_of(n);
     for (int64_t i = 0; i < n; ++i) {
         const int64_t src = perm[i];
         for (int k = 0; k < D; ++k) {
@@ -1256,7 +1258,7 @@ int vo_tri_stage(vo_tri* t, const double* pts, int64_t n, uint64_t seed) {
             x64[i * S + k] = x;
             x32[i * S + k] = (float)(x - t->c[k]);
         }
-        t->input_of[base_in + i] = (int32_t)(base_in + src);
+        in_of[i] = (int32_t)(base_in + src);
         vm[i] = (int)(2 * M + base_in + src);
     }
     const int64_t v0 = M + t->n_pts;
@@ -1264,11 +1266,11 @@ int vo_tri_stage(vo_tri* t, const double* pts, int64_t n, uint64_t seed) {
     VO_HIP(hipMemcpyAsync(t->X32 + v0 * S, x32.data(), sizeof(float) * x32.size(), hipMemcpyHostToDevice, t->s));
     VO_HIP(hipMemcpyAsync(t->vmap + v0, vm.data(), sizeof(int) * n, hipMemcpyHostToDevice, t->s));
     VO_HIP(hipMemsetAsync(t->cp + t->n_pts, 0xFF, sizeof(int) * n, t->s));
-    t->n_pts += n;
+    const int64_t n_after = t->n_pts + n;
     // walk-start hints: grids of 2^l cells per axis over the bounding sphere's box, finest
     // level about 4 points a cell; level 0 holds the initial simplex
     if (!t->hint) {
-        const int L = hint_levels(D, t->n_pts);
+        const int L = hint_levels(D, n_after);
         t->hlevels = L;
         const int64_t cells = hint_cells(D, L);
         for (int k = 0; k < 3; ++k) t->glo[k] = (float)(-t->r);
@@ -1278,9 +1280,11 @@ int vo_tri_stage(vo_tri* t, const double* pts, int64_t n, uint64_t seed) {
         VO_HIP(hipMemsetAsync(t->hint, 0, sizeof(int), t->s));
     }
     // simplex capacity for everything staged (uniform data creates ~27 (3D) / 6 (2D) per point)
-    const int64_t want = (int64_t)t->n_tets + (D == 3 ? 34 : 8) * (t->n_pts - t->inserted - t->dups) + 4096;
+    const int64_t want = (int64_t)t->n_tets + (D == 3 ? 34 : 8) * (n_after - t->inserted - t->dups) + 4096;
     if ((rc = grow_tets(t, want))) return rc;
     VO_HIP(hipStreamSynchronize(t->s));
+    t->n_pts = n_after;
+    t->input_of.insert(t->input_of.end(), in_of.begin(), in_of.end());
     return 0;
 }
 
