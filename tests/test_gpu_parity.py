@@ -227,6 +227,7 @@ def test_device_planned_rounds_match_host_planned(dim):
     seen = []
     for pipelined, chunk in ((0, 4), (1, 1), (1, 4), (1, 7)):
         t = vo.DelaunayTree(np.vstack([P, Q]))
+        t.config("abort_early", 0)              # the schedule that does not depend on timing
         t.config("pipelined", pipelined)
         t.config("chunk_rounds", chunk)
         t.add_points_to_tree(P, seed=8)
@@ -253,6 +254,7 @@ def test_device_planned_rounds_hand_over_exact_and_solo_rounds():
         seen = []
         for pipelined in (0, 1):
             t = vo.DelaunayTree(P)
+            t.config("abort_early", 0)
             t.config("pipelined", pipelined)
             t.add_points_to_tree(P, seed=11)
             T, E = t.canonical()
@@ -268,13 +270,14 @@ def test_device_planned_rounds_hand_over_exact_and_solo_rounds():
 def test_full_size_3d_4m_invariants():
     # the headline workload at full size (3D uniform n = 4,000,000): the exact device check
     # (local Delaunay property of every face, adjacency and orientation consistency), vertex
-    # count, the super simplex's four hull faces, and the same rounds and windows from the
-    # device planner as from the host loop
+    # count, the super simplex's four hull faces, and (without early abort, whose winners depend
+    # on timing) the same rounds and windows from the device planner as from the host loop
     n = 4_000_000
     P = datasets.uniform(n, 3, seed=0)
     seen = []
-    for pipelined in (1, 0):
+    for pipelined, abort in ((1, 1), (1, 0), (0, 0)):
         t = vo.DelaunayTree(P)
+        t.config("abort_early", abort)
         t.config("pipelined", pipelined)
         t.add_points_to_tree(P, seed=0)
         assert t.violations() == 0
@@ -282,7 +285,8 @@ def test_full_size_3d_4m_invariants():
         st = t.stats()
         assert nv == n + 4 and st["points_inserted"] == n and st["points_duplicate"] == 0
         assert mx == st["created"] + 4          # d + 1 after DelaunayTree::new, plus one per created simplex
-        seen.append((st["rounds"], st["windows_total"], st["created"], st["killed"], ns))
+        if not abort:
+            seen.append((st["rounds"], st["windows_total"], st["created"], st["killed"], ns))
         if pipelined:
             _, nb = t.export()
             assert np.count_nonzero(nb < 0) == 4
@@ -336,3 +340,31 @@ def test_device_canonical_equals_host_canonical(dim, n):
         assert T.shape[0] > 0 and E.shape[0] > 0
         assert np.all(np.diff(E[:, 0]) >= 0)
     t.close()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("dim", [3, 2])
+def test_abort_early_same_triangulation(dim):
+    # early abort changes which points win a round, never the result: uniform points and points
+    # rounded onto a sphere (exact predicates, solo runs) give the oracle's triangulation with
+    # abort_early 0 and 1, and the engine agrees with both
+    rng = np.random.default_rng(170 + dim)
+    u = rng.normal(size=(3000, dim))
+    S = 0.5 + 0.3 * u / np.linalg.norm(u, axis=1)[:, None]
+    for P in (datasets.uniform(150000, dim, seed=180 + dim), np.vstack([S, 0.5 + 0.05 * rng.random((200, dim))])):
+        ref = O.Tree.delaunay(P, O.MODE_EXACT).canonical()
+        for abort in (0, 1):
+            t = vo.DelaunayTree(P)
+            t.config("abort_early", abort)
+            t.add_points_to_tree(P, seed=3)
+            T, E = t.canonical()
+            assert np.array_equal(T, ref.T) and np.array_equal(E, ref.E), (dim, abort, P.shape)
+            assert t.violations() == 0
+            t.close()
+        parts = [P[:40000], P[40000:]] if P.shape[0] > 40000 else [P]
+        refs = [O.Tree.delaunay(part, O.MODE_EXACT).canonical() for part in parts]
+        for abort in (0, 1):
+            res = vo.batch_triangulate(parts, seed=4, abort_early=abort)
+            for ref_k, r in zip(refs, res):
+                T_k, E_k = r.canonical()
+                assert np.array_equal(T_k, ref_k.T) and np.array_equal(E_k, ref_k.E), (dim, abort, "engine")

@@ -510,6 +510,7 @@ struct vo_engine {
     int min_window = 8;
     int tpp_milli = 0;
     int filters = 3;
+    int abort_early = 1;               // as vo_tri_config "abort_early"
     float gamma0 = 0.2f;
     bool launched = false;
     int last_nwin = 0;
@@ -658,6 +659,7 @@ Ctx engine_ctx(vo_engine* e) {
     c.cmask = e->cmask;
     c.filters = e->filters;
     c.solo = 0;
+    c.abort_early = e->abort_early;
     c.M = e->M;
     c.win_cap = e->win_cap;
     c.kcap = e->kcap;
@@ -1301,6 +1303,10 @@ int vo_engine_config(vo_engine* e, const char* key, int64_t value) {
     } else if (k == "filters") {
         if (value < 0 || value > 3) return fail(VO_ERR_INVALID, "filters must be a mask of 1 (fp32) and 2 (fp64)");
         e->filters = (int)value;
+        drop_graphs(e);
+    } else if (k == "abort_early") {
+        if (value != 0 && value != 1) return fail(VO_ERR_INVALID, "abort_early must be 0 or 1");
+        e->abort_early = (int)value;
         drop_graphs(e);
     } else if (k == "tets_per_point_milli") {
         if (value < 1000 || value > 1000000) return fail(VO_ERR_INVALID, "tets_per_point_milli must be in 1000..1e6");

@@ -28,7 +28,9 @@
 
 namespace vo {
 
-enum : int { ST_OK = 0, ST_DEFER = 1, ST_DUP = 2, ST_OVERFLOW = 3, ST_BROKEN = 4, ST_IDLE = 5 };
+// ST_LOST: the point stopped its cavity search at a simplex that a higher-priority point of the
+// round had already reserved (it cannot win the round; it is retried in the next one)
+enum : int { ST_OK = 0, ST_DEFER = 1, ST_DUP = 2, ST_OVERFLOW = 3, ST_BROKEN = 4, ST_IDLE = 5, ST_LOST = 6 };
 enum : uint32_t { ERR_OVERFLOW = 1, ERR_WALK = 2, ERR_CHAIN = 4, ERR_TETCAP = 8, ERR_HULL = 16, ERR_CACHE = 32,
                   ERR_PRECISION = 64, ERR_PENDING = 128 };
 
@@ -130,6 +132,8 @@ struct Ctx {
     uint32_t cmask;
     int filters;           // bit 0: FP32 tier on, bit 1: FP64 filter on (both on in production)
     int solo;              // the window holds one point: cavity membership is stamped in kres
+    int abort_early;       // lane-group cavity search: reserve each member as it is found and stop
+                           // once a reservation shows a higher-priority point of the round
     int M;                 // vertex id of point p is M + p
     int win_cap, kcap, fcap;
     int tet_cap;
@@ -975,7 +979,7 @@ __global__ void __launch_bounds__(256) k_commit(Ctx c, int nwin, unsigned long l
     }
     // cavities too large for the per-point lists are re-run alone after the round
     const bool big = st == ST_OVERFLOW && !c.solo;
-    bool again = (st == ST_OK && !win) || st == ST_DEFER;
+    bool again = (st == ST_OK && !win) || st == ST_DEFER || st == ST_LOST;
     {
         const unsigned long long bm = __ballot(big);
         int bpos = 0;

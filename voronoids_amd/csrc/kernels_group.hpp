@@ -182,10 +182,18 @@ __global__ void __launch_bounds__(64, VO_CAVITY_WAVES) k_cavity_g(Ctx c, int nwi
         else if (s0 <= 0) { st = ST_DUP; if (lead) lc.dups++; }
     }
     // ---- cavity: level-synchronous breadth-first search
+    // With abort_early every member is reserved when it is found; the value the reservation
+    // replaced (kold) is examined one pass later, once that pass's loads have returned, so the
+    // check adds no wait.  A value below the point's key is a higher-priority point of this round
+    // (earlier rounds have larger keys): the point cannot win and stops searching.
     int nk = 0, nf = 0, lo = 0, hi = 0, ci = 0;
     bool bfs = st == ST_OK;
+    unsigned long long kold = ~0ull;
     if (bfs) {
-        if (lead) s_cav[g.gid][0] = t;
+        if (lead) {
+            s_cav[g.gid][0] = t;
+            if (c.abort_early) kold = atomicMin(kres_p<D>(c, t), key);
+        }
         nk = 1;
         hi = 1;
     }
@@ -215,6 +223,7 @@ __global__ void __launch_bounds__(64, VO_CAVITY_WAVES) k_cavity_g(Ctx c, int nwi
             }
         }
         const unsigned long long und = g.ballot(test && s == SIGN_UNDECIDED);
+        const unsigned long long lostm = g.ballot(bfs && kold < key);
         bool isnew = test && s != SIGN_UNDECIDED && s > 0;
         s_tmp[g.lane] = isnew ? n : -1;
         __builtin_amdgcn_wave_barrier();
@@ -223,7 +232,10 @@ __global__ void __launch_bounds__(64, VO_CAVITY_WAVES) k_cavity_g(Ctx c, int nwi
         const unsigned long long newm = g.ballot(isnew), facm = g.ballot(isfac);
         const int cnew = __popcll(newm), cfac = __popcll(facm);
         if (bfs) {
-            if (und) {
+            if (lostm) {
+                st = ST_LOST;
+                bfs = false;
+            } else if (und) {
                 st = ST_DEFER;
                 bfs = false;
             } else if (nk + cnew > KC || nf + cfac > FC) {
@@ -234,6 +246,7 @@ __global__ void __launch_bounds__(64, VO_CAVITY_WAVES) k_cavity_g(Ctx c, int nwi
                     const int at = nk + g.below(newm);
                     s_cav[g.gid][at] = n;
                     if (at < AC) s_adj[g.gid][at] = s_na[g.lane];
+                    if (c.abort_early) kold = atomicMin(kres_p<D>(c, n), key);
                 }
                 if (isfac) {
                     // facets go straight to the point's global list: only the killed list,
@@ -257,7 +270,7 @@ __global__ void __launch_bounds__(64, VO_CAVITY_WAVES) k_cavity_g(Ctx c, int nwi
     if (st == ST_OK) {
         for (int j = g.gl; j < nk; j += G) {
             const int x = s_cav[g.gid][j];
-            atomicMin(kres_p<D>(c, x), key);
+            if (!c.abort_early) atomicMin(kres_p<D>(c, x), key);
             c.cav[(size_t)slot * c.kcap + (j)] = x;
         }
     }
@@ -416,7 +429,7 @@ __global__ void __launch_bounds__(64, 5) k_commit_g(Ctx c, int nwin, unsigned lo
     }
     // cavities too large for the LDS lists are re-run alone after the round
     const bool big = lead && st == ST_OVERFLOW;
-    bool again = lead && ((st == ST_OK && !build) || st == ST_DEFER);
+    bool again = lead && ((st == ST_OK && !build) || st == ST_DEFER || st == ST_LOST);
     {
         const unsigned long long bm = __ballot(big);
         int bpos = 0;

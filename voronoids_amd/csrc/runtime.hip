@@ -161,6 +161,7 @@ struct vo_tri {
     int pipelined = 1;                 // 0: the host plans every round and waits for it
     int chunk_rounds = 4;              // rounds submitted per chunk
     int round_timing = 0;              // events around each round's kernels (cost: ~3% at n = 4M)
+    int abort_early = 1;               // losing cavity searches stop at the first reservation they lose
     int exact_dev = 0;                 // this run has queued exact predicates: evaluate them in the stream
     int64_t stalled = 0;
     double gamma = 0.25;
@@ -205,6 +206,7 @@ Ctx make_ctx(vo_tri* t) {
     c.M = t->M;
     c.filters = t->filters;
     c.solo = 0;
+    c.abort_early = t->abort_early;
     c.win_cap = t->win_cap;
     c.kcap = t->kcap;
     c.fcap = t->fcap;
@@ -1585,6 +1587,9 @@ int vo_tri_config(vo_tri* t, const char* key, int64_t value) {
     } else if (k == "round_timing") {
         if (value != 0 && value != 1) return fail(VO_ERR_INVALID, "round_timing must be 0 or 1");
         t->round_timing = (int)value;
+    } else if (k == "abort_early") {
+        if (value != 0 && value != 1) return fail(VO_ERR_INVALID, "abort_early must be 0 or 1");
+        t->abort_early = (int)value;
     } else if (k == "chunk_rounds") {
         if (value < 1 || value > 64) return fail(VO_ERR_INVALID, "chunk_rounds must be in 1..64");
         t->chunk_rounds = (int)value;
