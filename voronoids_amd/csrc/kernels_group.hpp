@@ -284,6 +284,69 @@ __global__ void __launch_bounds__(64, VO_CAVITY_WAVES) k_cavity_g(Ctx c, int nwi
     flush_counters(c.cnt, lc);
 }
 
+// Partner of ridge k of facet j of a winner's cavity boundary (fv: the new simplices' vertex
+// tuples, p first, then the facet's D vertices): the ridge is the facet's vertices other than
+// fv[j][k]; the boundary of a cavity is a closed manifold, so exactly one other facet holds all
+// of the ridge.  Returns that facet and, in kk, the index (1..D) of its vertex off the ridge,
+// which is the face of its new simplex that the two new simplices share; -1 if none is found.
+// 3D: the tuple in LDS is the new simplex (p, f1, f2, f3); 2D: the facet (f1, f2) only
+// The scan reads four tuples per step (indices clamped to the list, so every read is in it)
+// before comparing, one LDS round trip per four facets.
+__device__ __forceinline__ int match_ridge(const int4* fv, int nf, int j, int k, int& kk) {
+    const int4 a = fv[j];
+    const int u = k == 1 ? a.z : a.y;
+    const int w = k == 3 ? a.z : a.w;
+    for (int j0 = 0; j0 < nf; j0 += 4) {
+        int4 o[4];
+#pragma unroll
+        for (int r = 0; r < 4; ++r) o[r] = fv[j0 + r < nf ? j0 + r : nf - 1];
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int jj = j0 + r;
+            const bool hy = (o[r].y == u) | (o[r].y == w), hz = (o[r].z == u) | (o[r].z == w);
+            const bool hw = (o[r].w == u) | (o[r].w == w);
+            if (jj < nf && jj != j && (int)hy + (int)hz + (int)hw == 2) {
+                kk = !hy ? 1 : (!hz ? 2 : 3);
+                return jj;
+            }
+        }
+    }
+    return -1;
+}
+__device__ __forceinline__ int match_ridge(const int2* fv, int nf, int j, int k, int& kk) {
+    const int2 a = fv[j];
+    const int u = k == 1 ? a.y : a.x;
+    for (int j0 = 0; j0 < nf; j0 += 4) {
+        int2 o[4];
+#pragma unroll
+        for (int r = 0; r < 4; ++r) o[r] = fv[j0 + r < nf ? j0 + r : nf - 1];
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int jj = j0 + r;
+            if (jj < nf && jj != j && ((o[r].x == u) | (o[r].y == u))) {
+                kk = o[r].x == u ? 2 : 1;
+                return jj;
+            }
+        }
+    }
+    return -1;
+}
+template <int D> struct FacetTuple { using T = int4; };
+template <> struct FacetTuple<2> { using T = int2; };
+__device__ __forceinline__ void facet_tuple(int4& out, int4 nv) { out = nv; }
+__device__ __forceinline__ void facet_tuple(int2& out, int4 nv) { out = make_int2(nv.y, nv.z); }
+
+// Facet tuples a group keeps in LDS for matching ridges (4 KB per block in 3D, 2 KB in 2D); a
+// winner with more facets links its internal faces by rotating around each ridge through its
+// killed simplices.
+#ifndef VO_RIDGE_MATCH3
+#define VO_RIDGE_MATCH3 1
+#endif
+template <int D, int G> constexpr int kGroupMC =
+    (D == 3 && !VO_RIDGE_MATCH3) ? 0
+    : ((D == 3 ? 4096 / (16 * (64 / G)) : 2048 / (8 * (64 / G))) < kGroupFC<D>
+           ? (D == 3 ? 4096 / (16 * (64 / G)) : 2048 / (8 * (64 / G))) : kGroupFC<D>);
+
 template <int D, int G>
 __global__ void __launch_bounds__(64, 5) k_commit_g(Ctx c, int nwin, unsigned long long key_hi) {
     constexpr int M = D + 1;
@@ -291,8 +354,10 @@ __global__ void __launch_bounds__(64, 5) k_commit_g(Ctx c, int nwin, unsigned lo
     constexpr int FC = kGroupFC<D>;
     static_assert(FC % 4 == 0, "int4 scans");
     constexpr int DW = (FC * D + 31) / 32;     // one bit per (facet, ridge): internal face linked
+    constexpr int MC = kGroupMC<D, G>;
     __shared__ __align__(16) int s_fac[NG][FC];
     __shared__ unsigned s_done[NG][DW];
+    __shared__ __align__(16) typename FacetTuple<D>::T s_fv[NG][MC > 0 ? MC : 1];
     const Group<G> g;
     const bool lead = g.gl == 0;
     LocalCnt lc = {};
@@ -356,12 +421,30 @@ __global__ void __launch_bounds__(64, 5) k_commit_g(Ctx c, int nwin, unsigned lo
             write_star_coords<D>(c, t, i, pf, N);
             *a_ptr<D>(c, N, 0) = outside;
             if (outside >= 0) *a_ptr<D>(c, outside >> 2, outside & 3) = N << 2;
+            if (nf <= MC) facet_tuple(s_fv[g.gid][j], nv);
         }
-        // phase 2: internal faces, one lane per (facet, ridge); each rotation links both sides
-        // of the face it finds, and the other side's lane skips it when it comes later
+        // phase 2: internal faces, one lane per (facet, ridge); each link is written from both
+        // sides by the lane that finds it, and the other side's lane skips it when it comes later
         for (int w = g.gl; w < DW; w += G) s_done[g.gid][w] = 0u;
         __builtin_amdgcn_wave_barrier();
-        for (int q = g.gl; q < nf * D; q += G) {
+        // matched in LDS: the facet sharing the ridge is the partner (no memory traffic)
+        for (int q = g.gl; nf <= MC && q < nf * D; q += G) {
+            if (s_done[g.gid][q >> 5] & (1u << (q & 31))) continue;
+            const int j = q / D, k = 1 + q % D;
+            int kk = 1;
+            const int jj = match_ridge(s_fv[g.gid], nf, j, k, kk);
+            if (jj < 0) {
+                atomicOr(&c.st->err, ERR_WALK);
+                *a_ptr<D>(c, base + j, k) = -1;
+                continue;
+            }
+            *a_ptr<D>(c, base + j, k) = ((base + jj) << 2) | kk;
+            *a_ptr<D>(c, base + jj, kk) = ((base + j) << 2) | k;
+            const int qq = jj * D + kk - 1;
+            atomicOr(&s_done[g.gid][qq >> 5], 1u << (qq & 31));
+        }
+        // too many facets for the LDS tuples: rotate around the ridge through the killed simplices
+        for (int q = g.gl; nf > MC && q < nf * D; q += G) {
             if (s_done[g.gid][q >> 5] & (1u << (q & 31))) continue;
             const int j = q / D, k = 1 + q % D;
             const int fe = s_fac[g.gid][j];
