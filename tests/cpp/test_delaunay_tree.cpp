@@ -2,12 +2,13 @@
 //
 // test_delaunay_tree_3d / test_delaunay_tree_2d follow the reference's own tests
 // (tests/test_delaunay_tree.rs:6-37 and :39-60) step by step: new, max_simplex_id, single
-// insert_point calls with ids n_points + i, add_points_to_tree, check_delaunay.  The points come
-// from std::mt19937_64 seeded with 0 through uniform_real_distribution(0, 1), not from rand's
-// StdRng (ChaCha12, not available here), so the point sets differ from the reference's; both are
-// uniform in the unit cube / square.  On top of the reference's checks every result is compared
-// with the CPU oracle (oracle/liboracle.so, exact predicates, sequential insertion of the same
-// points): canonical simplices and edges, and per-vertex incidence counts of the vertices() view.
+// insert_point calls with ids n_points + i, add_points_to_tree, check_delaunay.  The points are
+// drawn as the reference draws them, StdRng::seed_from_u64(0) and Uniform::from(0.0..1.0) in the
+// same order, through include/voronoids_rand.hpp (rand 0.8 restated: its ChaCha core is checked
+// against RFC vectors, its seed expansion and float mapping are not checkable here).  On top of
+// the reference's checks every result is compared with the CPU oracle (oracle/liboracle.so,
+// exact predicates, sequential insertion of the same points): canonical simplices and edges, and
+// per-vertex incidence counts of the vertices() view.
 #include <algorithm>
 #include <chrono>
 #include <cmath>
@@ -21,6 +22,7 @@
 
 #include "oracle.h"
 #include "voronoids.hpp"
+#include "voronoids_rand.hpp"
 
 using voronoids::delaunay_tree::Canonical;
 using voronoids::delaunay_tree::DelaunayTree;
@@ -155,9 +157,19 @@ static void check_canonical_equal(const Canonical& a, const Canonical& b, const 
     CHECK(eq);
 }
 
+// n points of N coordinates from the reference's generator, point by point, coordinate by coordinate
+template <std::size_t N>
+static std::vector<std::array<double, N>> reference_points(voronoids::rand_compat::StdRng& rng, std::size_t n) {
+    const voronoids::rand_compat::UniformF64 dist(0.0, 1.0);
+    std::vector<std::array<double, N>> v(n);
+    for (auto& p : v)
+        for (auto& x : p) x = dist.sample(rng);
+    return v;
+}
+
 static void test_delaunay_tree_3d() {
-    std::mt19937_64 rng(0);
-    const auto vertices = uniform_points<3>(rng, 1000);
+    auto rng = voronoids::rand_compat::StdRng::seed_from_u64(0);
+    const auto vertices = reference_points<3>(rng, 1000);
     DelaunayTree<3, 4> delaunay_tree(vertices);
     CHECK(delaunay_tree.max_simplex_id() == 4);
     const std::size_t n_points = delaunay_tree.num_vertices();
@@ -166,7 +178,7 @@ static void test_delaunay_tree_3d() {
         const TreeUpdate<3, 4> update(n_points + i, vertices[i], delaunay_tree);
         delaunay_tree.insert_point(update);
     }
-    const auto vertices2 = uniform_points<3>(rng, 1000);
+    const auto vertices2 = reference_points<3>(rng, 1000);
     delaunay_tree.add_points_to_tree(vertices2);
     CHECK(delaunay_tree.check_delaunay());
     CHECK(delaunay_tree.num_vertices() == 8 + 1100);
@@ -187,15 +199,16 @@ static void test_delaunay_tree_2d() {
     const std::vector<std::array<double, 2>> vertices = {{0.3, 0.1}, {1.0, 0.2}, {0.1, 1.0}, {0.5, 0.5}};
     DelaunayTree<2, 3> delaunay_tree(vertices);
     CHECK(delaunay_tree.max_simplex_id() == 3);
-    std::mt19937_64 rng(0);
-    std::uniform_real_distribution<double> dist(0.0, 1.0);
+    auto rng = voronoids::rand_compat::StdRng::seed_from_u64(0);
+    const voronoids::rand_compat::UniformF64 dist(0.0, 1.0);
     const std::size_t n_points = delaunay_tree.num_vertices();
     CHECK(n_points == 6);
     std::vector<double> inserted;
     double total_ms = 0.0, worst_ms = 0.0;
     for (std::size_t i = 0; i < 1000; ++i) {
         const auto start = std::chrono::steady_clock::now();
-        const std::array<double, 2> point = {dist(rng), dist(rng)};
+        const double x = dist.sample(rng), y = dist.sample(rng);   // the reference's evaluation order
+        const std::array<double, 2> point = {x, y};
         const TreeUpdate<2, 3> update(n_points + i, point, delaunay_tree);
         delaunay_tree.insert_point(update);
         const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - start).count();

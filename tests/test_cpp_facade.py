@@ -1,6 +1,8 @@
 """The C++ facade (include/voronoids.hpp) over the C ABI.
 
-CPU: the header compiles with every member of DelaunayTree<3, 4> and <2, 3> instantiated.
+CPU: the header compiles with every member of DelaunayTree<3, 4> and <2, 3> instantiated, and
+the rand 0.8 streams of the reference's tests (include/voronoids_rand.hpp) pass their
+known-answer tests.
 GPU: tests/cpp/_bin/test_delaunay_tree (built by tests/cpp/Makefile from build()) runs the
 reference's tree tests (tests/test_delaunay_tree.rs) through the facade and checks each result
 against the oracle; it exits non-zero on any failed check.
@@ -36,6 +38,20 @@ def test_facade_header_compiles(tmp_path):
     r = subprocess.run([cxx, "-std=c++17", "-fsyntax-only", "-Wall", "-Wextra", "-Werror",
                         "-I", os.path.join(ROOT, "include"), str(src)], capture_output=True, text=True)
     assert r.returncode == 0, r.stderr
+
+
+def test_rand_compat_known_answers(tmp_path):
+    # include/voronoids_rand.hpp: ChaCha against RFC 8439 / RFC 7539 vectors, buffer boundaries
+    cxx = shutil.which("g++")
+    if cxx is None:
+        pytest.skip("g++ not found")
+    exe = tmp_path / "rand_compat"
+    r = subprocess.run([cxx, "-std=c++17", "-O2", "-Wall", "-Wextra", "-Werror", "-I", os.path.join(ROOT, "include"),
+                        "-o", str(exe), os.path.join(ROOT, "tests", "cpp", "test_rand_compat.cpp")],
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    r = subprocess.run([str(exe)], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0 and "rand_compat: ok" in r.stdout, r.stdout + r.stderr
 
 
 @pytest.mark.gpu
