@@ -116,7 +116,8 @@ int vo_tri_stats(const vo_tri* t, vo_stats* out);
  * finds it and stops as soon as a reservation shows a higher-priority point of the round (that point
  * cannot win; it retries next round).  Same triangulation either way; with 1 the set of winners of a
  * round depends on timing, so round counts and simplex ids may differ from run to run (0: the same
- * rounds, windows and ids on every run; measured 157 -> 147 ms at 3D n = 4M). */
+ * rounds, windows and ids on every run; measured 157 -> 147 ms at 3D n = 4M).  "min_window" (default 64,
+ * 1..65536): least points per round (the window is max(min_window, gamma x inserted)). */
 int vo_tri_config(vo_tri* t, const char* key, int64_t value);
 
 /* The precision tiers one predicate at a time (no counterpart in the reference, whose in_sphere
