@@ -43,6 +43,9 @@ template <int G> struct Group {
 #ifndef VO_GROUP_FC3
 #define VO_GROUP_FC3 256
 #endif
+#ifndef VO_COMMIT_WAVES
+#define VO_COMMIT_WAVES 5
+#endif
 #ifndef VO_CAVITY_WAVES
 #define VO_CAVITY_WAVES 3
 #endif
@@ -342,13 +345,16 @@ __device__ __forceinline__ void facet_tuple(int2& out, int4 nv) { out = make_int
 #ifndef VO_RIDGE_MATCH3
 #define VO_RIDGE_MATCH3 1
 #endif
+#ifndef VO_MATCH_BYTES3
+#define VO_MATCH_BYTES3 4096
+#endif
 template <int D, int G> constexpr int kGroupMC =
     (D == 3 && !VO_RIDGE_MATCH3) ? 0
-    : ((D == 3 ? 4096 / (16 * (64 / G)) : 2048 / (8 * (64 / G))) < kGroupFC<D>
-           ? (D == 3 ? 4096 / (16 * (64 / G)) : 2048 / (8 * (64 / G))) : kGroupFC<D>);
+    : ((D == 3 ? VO_MATCH_BYTES3 / (16 * (64 / G)) : 2048 / (8 * (64 / G))) < kGroupFC<D>
+           ? (D == 3 ? VO_MATCH_BYTES3 / (16 * (64 / G)) : 2048 / (8 * (64 / G))) : kGroupFC<D>);
 
 template <int D, int G>
-__global__ void __launch_bounds__(64, 5) k_commit_g(Ctx c, int nwin, unsigned long long key_hi) {
+__global__ void __launch_bounds__(64, VO_COMMIT_WAVES) k_commit_g(Ctx c, int nwin, unsigned long long key_hi) {
     constexpr int M = D + 1;
     constexpr int NG = Group<G>::NG;
     constexpr int FC = kGroupFC<D>;
