@@ -368,3 +368,30 @@ def test_abort_early_same_triangulation(dim):
             for ref_k, r in zip(refs, res):
                 T_k, E_k = r.canonical()
                 assert np.array_equal(T_k, ref_k.T) and np.array_equal(E_k, ref_k.E), (dim, abort, "engine")
+
+
+_TRANSFORMS = {
+    "offset1e6": lambda U: U + 1e6,
+    "offset1e9": lambda U: U + 1e9,
+    "neg_offset": lambda U: U - 123456.789,
+    "scale1e-9": lambda U: U * 1e-9,
+    "scale1e-6": lambda U: U * 1e-6,
+    "scale1e6": lambda U: U * 1e6,
+    "scale1e12": lambda U: U * 1e12,
+    "aniso": lambda U: U * np.array([1.0, 1e-4, 1e-4][:U.shape[1]]),
+}
+
+
+@pytest.mark.parametrize("dim", [3, 2])
+@pytest.mark.parametrize("name", sorted(_TRANSFORMS))
+def test_translated_and_scaled_inputs_match_oracle(dim, name):
+    # the FP32 tier works on coordinates translated to the bounding sphere's center and the filters'
+    # error bounds scale with the input: far offsets, tiny and huge scales and a 1e4:1 box give the
+    # exact oracle's triangulation (the 3D box sends thousands of predicates to the exact tier)
+    P = _TRANSFORMS[name](datasets.uniform(20000, dim, seed=300 + dim))
+    t = vo.delaunay(P, seed=1)
+    T, E = t.canonical()
+    ref = O.Tree.delaunay(P, O.MODE_EXACT).canonical()
+    assert np.array_equal(T, ref.T) and np.array_equal(E, ref.E), (dim, name)
+    assert t.violations() == 0
+    t.close()
