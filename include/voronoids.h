@@ -101,7 +101,7 @@ int vo_tri_stats(const vo_tri* t, vo_stats* out);
  * (the same order with consecutive chunks of position/k points sorted along the Morton curve; set before
  * vo_tri_stage).  "big_window" (default 16384 in 3D, 65536 in 2D; 0 = off), "big_frac_lo_milli" /
  * "big_frac_hi_milli" (default 550 / 800): rounds whose window is at least big_window steer the window
- * size so that this fraction of it wins (smaller windows aim at 0.35 .. 0.6).  "compact_ratio_milli" (default
+ * size so that this fraction of it wins (smaller windows aim at "small_frac_lo_milli" .. "small_frac_hi_milli", default 350 / 600).  "compact_ratio_milli" (default
  * 0 = never; measured no faster at n = 4M: every visit misses the caches either way): between rounds, when the record array holds more than this many records (x 1000) per live
  * simplex, the live ones move to a dense array in spatial order; "compact_min" (default 2^20): not before this
  * many records exist.  "xcd_regions" (0 default / 1): each round's window is sorted into 8 spatial regions and

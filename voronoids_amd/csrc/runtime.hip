@@ -80,7 +80,7 @@ enum : int { HALT_HOST = 1, HALT_SPACE = 2, HALT_DONE = 4, HALT_STALL = 8 };
 struct RunPlan {
     BatchCtl ctl;              // nwin / key_hi of the planned round (read by the round kernels)
     long long inserted, dups, windows, n_live, tet_cap;
-    double gamma, big_frac_lo, big_frac_hi;
+    double gamma, big_frac_lo, big_frac_hi, small_frac_lo, small_frac_hi;
     int head, n_pts, npend, nfresh, nwin, rounds, stalled, cur;
     int launched;              // a planned round ran and is not absorbed yet
     int halt;                  // HALT_* bits
@@ -170,6 +170,7 @@ struct vo_tri {
     int filters = 3;
     int64_t big_window = 0;            // windows at least this large target big_frac_lo..hi winners (0: off)
     double big_frac_lo = 0.55, big_frac_hi = 0.8;
+    double small_frac_lo = 0.35, small_frac_hi = 0.6;   // the same for smaller windows
     int local_order = 0;               // priority order: random, Morton-sorted in chunks of position/this (0: off)
     int group = 0;                     // lanes per window point (0 = chosen per round, 1 = one thread)
     int64_t lanes_target = 0;          // auto group size: largest G with window * G <= this (0 = default)
@@ -350,7 +351,7 @@ __device__ void plan_step(RunPlan& p, RoundState* st, bool s_absorb) {
         p.windows += nwin;
         const double frac = nwin ? (double)w / nwin : 1.0;
         const bool big = p.big_window > 0 && nwin >= p.big_window;
-        const double hi = big ? p.big_frac_hi : 0.6, lo = big ? p.big_frac_lo : 0.35;
+        const double hi = big ? p.big_frac_hi : p.small_frac_hi, lo = big ? p.big_frac_lo : p.small_frac_lo;
         if (frac > hi) p.gamma = fmin(p.gamma * 1.5, 4.0);
         else if (frac < lo) p.gamma = fmax(p.gamma * 0.7, 0.002);
         p.stalled = (w == 0 && consumed == 0 && st->n_q == 0) ? p.stalled + 1 : 0;
@@ -805,7 +806,7 @@ int finish_round(vo_tri* t, int nwin, int nfresh, FILE* trace, float ms_cav, flo
     // (a loser's cavity is wasted work, a round costs little beyond its points), so there
     // the target winning fraction is higher
     const bool big = t->big_window > 0 && nwin >= t->big_window;
-    const double hi = big ? t->big_frac_hi : 0.6, lo = big ? t->big_frac_lo : 0.35;
+    const double hi = big ? t->big_frac_hi : t->small_frac_hi, lo = big ? t->big_frac_lo : t->small_frac_lo;
     if (frac > hi) t->gamma = std::min(t->gamma * 1.5, 4.0);
     else if (frac < lo) t->gamma = std::max(t->gamma * 0.7, 0.002);
     // every round either inserts, drops a duplicate, or resolves queued predicates
@@ -886,6 +887,8 @@ RunPlan plan_from_host(const vo_tri* t) {
     p.gamma = t->gamma;
     p.big_frac_lo = t->big_frac_lo;
     p.big_frac_hi = t->big_frac_hi;
+    p.small_frac_lo = t->small_frac_lo;
+    p.small_frac_hi = t->small_frac_hi;
     p.head = (int)t->head;
     p.n_pts = (int)t->n_pts;
     p.npend = t->npend;
@@ -1590,6 +1593,9 @@ int vo_tri_config(vo_tri* t, const char* key, int64_t value) {
     } else if (k == "round_timing") {
         if (value != 0 && value != 1) return fail(VO_ERR_INVALID, "round_timing must be 0 or 1");
         t->round_timing = (int)value;
+    } else if (k == "small_frac_lo_milli" || k == "small_frac_hi_milli") {
+        if (value < 1 || value > 999) return fail(VO_ERR_INVALID, k + " must be in 1..999");
+        (k == "small_frac_lo_milli" ? t->small_frac_lo : t->small_frac_hi) = (double)value / 1000.0;
     } else if (k == "min_window") {
         if (value < 1 || value > (1 << 16)) return fail(VO_ERR_INVALID, "min_window must be in 1..65536");
         t->min_window = (int)value;
