@@ -117,7 +117,9 @@ int vo_tri_stats(const vo_tri* t, vo_stats* out);
  * cannot win; it retries next round).  Same triangulation either way; with 1 the set of winners of a
  * round depends on timing, so round counts and simplex ids may differ from run to run (0: the same
  * rounds, windows and ids on every run; measured 157 -> 147 ms at 3D n = 4M).  "min_window" (default 64,
- * 1..65536): least points per round (the window is max(min_window, gamma x inserted)). */
+ * 1..65536): least points per round (the window is max(min_window, gamma x inserted)).  "small_run" (default
+ * 64): a vo_tri_run with at most this many points to insert plans its rounds on the host (one point: 0.16 ms
+ * in 2D instead of 0.22 ms through the device planner, which submits at least two chunks of rounds). */
 int vo_tri_config(vo_tri* t, const char* key, int64_t value);
 
 /* The precision tiers one predicate at a time (no counterpart in the reference, whose in_sphere

@@ -164,6 +164,7 @@ struct vo_tri {
     int round_timing = 0;              // events around each round's kernels (cost: ~3% at n = 4M)
     int abort_early = 1;               // losing cavity searches stop at the first reservation they lose
     int min_window = 64;               // least points per round (8 measured ~2% slower at 3D 4M / 2D 10M)
+    int64_t small_run = 64;            // runs of at most this many points use the host loop
     int exact_dev = 0;                 // this run has queued exact predicates: evaluate them in the stream
     int64_t stalled = 0;
     double gamma = 0.25;
@@ -1311,8 +1312,11 @@ int vo_tri_run(vo_tri* t) {
         ~TraceFile() { if (f) std::fclose(f); }
     } tf;
     if (const char* tp = std::getenv("VO_TRACE")) tf.f = std::fopen(tp, "a");
-    // the per-round host decisions of compaction and XCD regions need the host loop
-    const bool host_loop = !t->pipelined || tf.f || t->xcd || t->compact_ratio > 0;
+    // the per-round host decisions of compaction and XCD regions need the host loop; so does a
+    // small run (a few points need a round or two, and the device planner submits at least two
+    // chunks of rounds: one point through the host loop takes about a third of the time)
+    const int64_t todo = t->n_pts - t->head + t->npend;
+    const bool host_loop = !t->pipelined || tf.f || t->xcd || t->compact_ratio > 0 || todo <= t->small_run;
     int rc = host_loop ? run_host_planned(t, tf.f) : run_device_planned(t);
     if (rc) return rc;
     VO_HIP(hipEventRecord(t->ev[1], t->s));
@@ -1596,6 +1600,9 @@ int vo_tri_config(vo_tri* t, const char* key, int64_t value) {
     } else if (k == "small_frac_lo_milli" || k == "small_frac_hi_milli") {
         if (value < 1 || value > 999) return fail(VO_ERR_INVALID, k + " must be in 1..999");
         (k == "small_frac_lo_milli" ? t->small_frac_lo : t->small_frac_hi) = (double)value / 1000.0;
+    } else if (k == "small_run") {
+        if (value < 0) return fail(VO_ERR_INVALID, "small_run must be >= 0");
+        t->small_run = value;
     } else if (k == "min_window") {
         if (value < 1 || value > (1 << 16)) return fail(VO_ERR_INVALID, "min_window must be in 1..65536");
         t->min_window = (int)value;
