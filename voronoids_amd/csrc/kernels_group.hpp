@@ -46,6 +46,9 @@ template <int G> struct Group {
 #ifndef VO_COMMIT_WAVES
 #define VO_COMMIT_WAVES 5
 #endif
+#ifndef VO_CAVITY_WAVES2
+#define VO_CAVITY_WAVES2 3   // 2D (the kernel reaches 4 waves at its natural register count)
+#endif
 #ifndef VO_CAVITY_WAVES
 #define VO_CAVITY_WAVES 3
 #endif
@@ -84,7 +87,7 @@ __device__ __forceinline__ int lds_find(const int* list, int n, int x) {
 }
 
 template <int D, int G>
-__global__ void __launch_bounds__(64, VO_CAVITY_WAVES) k_cavity_g(Ctx c, int nwin, unsigned long long key_hi) {
+__global__ void __launch_bounds__(64, D == 3 ? VO_CAVITY_WAVES : VO_CAVITY_WAVES2) k_cavity_g(Ctx c, int nwin, unsigned long long key_hi) {
     using V = VT<D>;
     using F = typename V::F;
     constexpr int M = D + 1;
