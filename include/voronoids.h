@@ -138,6 +138,11 @@ int vo_tri_vertices(const vo_tri* t, double* coords);
 const char* vo_last_error(void);
 int vo_version(void);
 
+/* geometry::bounding_sphere (src/geometry.rs:99-141), host only: center[dim] and radius of the sphere
+ * DelaunayTree::new builds its super simplex around (per-axis min/max midpoint, then the radius x 1.5
+ * for each point, in input order, not strictly inside).  n >= 1. */
+int vo_bounding_sphere(int dim, const double* pts, int64_t n, double* center, double* radius);
+
 /* ------------------------------------------------------------------------------------------
  * Several independent point sets on one device (batched and served modes).
  *

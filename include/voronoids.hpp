@@ -3,8 +3,9 @@
  *
  * It mirrors the reference's Rust API, voronoids::delaunay_tree (src/delaunay_tree.rs), for C++
  * callers: DelaunayTree<N, M> with new (the constructor), add_points_to_tree, insert_point,
- * check_delaunay and max_simplex_id, TreeUpdate<N, M>, and the Simplex / Vertex records of its
- * DashMaps as snapshot views.  Every call goes through the C ABI; the construction runs on the
+ * insert_points_parallel, check_delaunay and max_simplex_id, TreeUpdate<N, M>, the Simplex / Vertex
+ * records of its DashMaps as snapshot views, and the geometry module's circumsphere, in_sphere and
+ * bounding_sphere.  Every call goes through the C ABI; the construction runs on the
  * GPU.  Link with -lvoronoids_hip.
  *
  * Differences from the reference, all forced by the construction living on the device:
@@ -97,6 +98,41 @@ inline void circumsphere(const double* const* p, double* center, double* radius)
     *radius = std::sqrt(r2);
 }
 }  // namespace detail
+
+/* the reference's geometry module (src/geometry.rs) on the host */
+namespace geometry {
+
+/* circumsphere (src/geometry.rs:58-83): center and radius of N + 1 points (the same sphere as the
+ * reference's slope (2D) and LU (3D) formulas, computed by elimination with pivoting) */
+template <std::size_t N, std::size_t M>
+std::pair<std::array<double, N>, double> circumsphere(const std::array<std::array<double, N>, M>& vertices) {
+    static_assert(M == N + 1, "N + 1 points");
+    const double* p[M];
+    for (std::size_t k = 0; k < M; ++k) p[k] = vertices[k].data();
+    std::pair<std::array<double, N>, double> out;
+    detail::circumsphere<N>(p, out.first.data(), &out.second);
+    return out;
+}
+
+/* in_sphere (src/geometry.rs:91-97): the reference's strict test, squared distance < radius^2 */
+template <std::size_t N>
+bool in_sphere(const std::array<double, N>& vertex, const std::array<double, N>& center, double radius) {
+    double distance = 0.0;
+    for (std::size_t i = 0; i < N; ++i) distance += (center[i] - vertex[i]) * (center[i] - vertex[i]);
+    return distance < radius * radius;
+}
+
+/* bounding_sphere (src/geometry.rs:99-141), bit for bit (vo_bounding_sphere) */
+template <std::size_t N>
+std::pair<std::array<double, N>, double> bounding_sphere(const std::vector<std::array<double, N>>& vertices) {
+    static_assert(N == 2 || N == 3, "2D and 3D only");
+    std::pair<std::array<double, N>, double> out;
+    detail::check(vo_bounding_sphere((int)N, vertices.empty() ? nullptr : vertices[0].data(), (int64_t)vertices.size(),
+                                     out.first.data(), &out.second));
+    return out;
+}
+
+}  // namespace geometry
 
 namespace delaunay_tree {
 
@@ -191,6 +227,26 @@ public:
             throw std::invalid_argument("insert_point: vertex id " + std::to_string(update.id()) +
                                         " is not the next free id " + std::to_string(num_vertices()));
         stage(update.vertex().data(), 1, 0);
+        detail::check(vo_tri_run(h_));
+    }
+
+    /* insert_points_parallel (src/delaunay_tree.rs:213-334): the updates' points inserted together
+     * (ids num_vertices() .. + updates.size() - 1, in order).  The reference expects updates whose
+     * cavities do not overlap (one level of find_placement); here any set is valid. */
+    void insert_points_parallel(const std::vector<TreeUpdate<N, M>>& updates, uint64_t seed = 0) {
+        if (updates.empty()) return;
+        std::vector<double> pts;
+        pts.reserve(updates.size() * N);
+        for (std::size_t i = 0; i < updates.size(); ++i) {
+            if (updates[i].tree() != this)
+                throw std::invalid_argument("insert_points_parallel: an update was made for another tree");
+            if (updates[i].id() != num_vertices() + i)
+                throw std::invalid_argument("insert_points_parallel: update " + std::to_string(i) + " has vertex id " +
+                                            std::to_string(updates[i].id()) + ", not " +
+                                            std::to_string(num_vertices() + i));
+            pts.insert(pts.end(), updates[i].vertex().begin(), updates[i].vertex().end());
+        }
+        stage(pts.data(), updates.size(), seed);
         detail::check(vo_tri_run(h_));
     }
 

@@ -245,6 +245,53 @@ static void test_delaunay_function() {
                 st.ms_run);
 }
 
+// insert_points_parallel (src/delaunay_tree.rs:213-334) with the updates of 1,000 points, then the
+// rest through add_points_to_tree; geometry::bounding_sphere bit for bit against the oracle's
+// restatement of src/geometry.rs:99-141
+static void test_insert_points_parallel_and_geometry() {
+    auto rng = voronoids::rand_compat::StdRng::seed_from_u64(11);
+    const auto pts = reference_points<3>(rng, 3000);
+    const auto bs = voronoids::geometry::bounding_sphere<3>(pts);
+    const std::vector<double> a = flat(pts);
+    double oc[3], orad;
+    or_bounding_sphere(3, a.data(), 3000, oc, &orad);
+    CHECK(bs.first[0] == oc[0] && bs.first[1] == oc[1] && bs.first[2] == oc[2] && bs.second == orad);
+    CHECK(voronoids::geometry::in_sphere<3>(bs.first, bs.first, bs.second));
+    CHECK(!voronoids::geometry::in_sphere<3>({bs.first[0] + bs.second, bs.first[1], bs.first[2]}, bs.first, bs.second));
+
+    DelaunayTree<3, 4> tree(pts);
+    const std::size_t n0 = tree.num_vertices();
+    std::vector<TreeUpdate<3, 4>> ups;
+    for (std::size_t i = 0; i < 1000; ++i) ups.emplace_back(n0 + i, pts[i], tree);
+    bool threw = false;
+    try {
+        std::vector<TreeUpdate<3, 4>> bad = {TreeUpdate<3, 4>(n0 + 1, pts[0], tree)};
+        tree.insert_points_parallel(bad);
+    } catch (const std::invalid_argument&) {
+        threw = true;
+    }
+    CHECK(threw);
+    tree.insert_points_parallel(ups);
+    CHECK(tree.num_vertices() == n0 + 1000);
+    tree.add_points_to_tree(std::vector<std::array<double, 3>>(pts.begin() + 1000, pts.end()));
+    CHECK(tree.check_delaunay());
+    or_tree* ot = or_tree_new(3, a.data(), 3000, OR_MODE_EXACT);
+    CHECK(or_tree_insert_seq(ot, a.data(), 3000, 0) == 0);
+    check_canonical_equal(tree.canonical(), oracle_canonical(ot, 3), "insert_points_parallel");
+    or_tree_free(ot);
+
+    // the Simplex view's spheres are the geometry module's
+    const auto simplices = tree.simplices();
+    const auto x = tree.coordinates();
+    for (std::size_t s = 0; s < simplices.size(); s += 97) {
+        std::array<std::array<double, 3>, 4> v;
+        for (int k = 0; k < 4; ++k) v[k] = x[simplices[s].vertices[k]];
+        const auto cs = voronoids::geometry::circumsphere<3, 4>(v);
+        CHECK(cs.first == simplices[s].center && cs.second == simplices[s].radius);
+    }
+    std::printf("test_insert_points_parallel_and_geometry: ok\n");
+}
+
 // error behaviour: ids, foreign updates, invalid coordinates and coincident points
 static void test_errors() {
     std::mt19937_64 rng(3);
@@ -304,6 +351,7 @@ int main() {
     } tests[] = {{"test_delaunay_tree_3d", test_delaunay_tree_3d},
                  {"test_delaunay_tree_2d", test_delaunay_tree_2d},
                  {"test_delaunay_function", test_delaunay_function},
+                 {"test_insert_points_parallel_and_geometry", test_insert_points_parallel_and_geometry},
                  {"test_errors", test_errors}};
     for (const auto& t : tests) {
         std::printf("running %s\n", t.name);

@@ -52,3 +52,27 @@ def test_result_canonical_is_host_only_and_matches_the_oracle():
         T, E = B.canonical_from_simplices(S, dim)
         ref = O.canonical_from_real(np.where(S >= 2 * M, S - 2 * M, -1), dim)
         assert np.array_equal(T, ref.T) and np.array_equal(E, ref.E)
+
+
+def test_bounding_sphere_is_the_reference_s_bit_for_bit():
+    # host-only entry point: the 10k-point configuration's bounding sphere against the known-answer
+    # bits the oracle is pinned to (tests/golden/kat.json), and the closed-form case
+    import json
+    import struct
+    import numpy as np
+    G = os.path.join(ROOT, "tests", "golden")
+    with open(os.path.join(G, "kat.json")) as f:
+        kat = json.load(f)
+    P = np.ascontiguousarray(np.load(os.path.join(G, "c1_points.npy")))
+    L = _lib.lib()
+    dp = ctypes.POINTER(ctypes.c_double)
+    c = np.zeros(3)
+    r = ctypes.c_double()
+    assert L.vo_bounding_sphere(3, P.ctypes.data_as(dp), P.shape[0], c.ctypes.data_as(dp), ctypes.byref(r)) == 0
+    bits = lambda x: "0x%016x" % struct.unpack("<Q", struct.pack("<d", float(x)))[0]
+    assert [bits(x) for x in c] == kat["c1_bounding_sphere_hex"]["center"]
+    assert bits(r.value) == kat["c1_bounding_sphere_hex"]["radius"]
+    Q = np.array([[0.0, 0.0, 0.5], [1.0, 2.0, 1.5], [0.5, 1.0, 0.0], [0.5, 1.0, 2.0]])
+    assert L.vo_bounding_sphere(3, Q.ctypes.data_as(dp), 4, c.ctypes.data_as(dp), ctypes.byref(r)) == 0
+    assert c.tolist() == [0.5, 1.0, 1.0] and r.value == 1.5
+    assert L.vo_bounding_sphere(4, Q.ctypes.data_as(dp), 4, c.ctypes.data_as(dp), ctypes.byref(r)) == _lib.VO_ERR_INVALID

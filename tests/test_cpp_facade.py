@@ -26,6 +26,11 @@ template voronoids::delaunay_tree::DelaunayTree<3, 4> voronoids::delaunay_tree::
     const std::vector<std::array<double, 3>>&, uint64_t, int);
 template voronoids::delaunay_tree::DelaunayTree<2, 3> voronoids::delaunay_tree::delaunay<2>(
     const std::vector<std::array<double, 2>>&, uint64_t, int);
+template std::pair<std::array<double, 3>, double> voronoids::geometry::circumsphere<3, 4>(
+    const std::array<std::array<double, 3>, 4>&);
+template bool voronoids::geometry::in_sphere<2>(const std::array<double, 2>&, const std::array<double, 2>&, double);
+template std::pair<std::array<double, 3>, double> voronoids::geometry::bounding_sphere<3>(
+    const std::vector<std::array<double, 3>>&);
 """
 
 
@@ -109,5 +114,6 @@ def test_cpp_delaunay_tree_facade():
     r = subprocess.run([BIN], capture_output=True, text=True, timeout=300)
     print(r.stdout, flush=True)
     assert r.returncode == 0, r.stdout + r.stderr
-    for name in ("test_delaunay_tree_3d", "test_delaunay_tree_2d", "test_delaunay_function", "test_errors"):
+    for name in ("test_delaunay_tree_3d", "test_delaunay_tree_2d", "test_delaunay_function",
+                 "test_insert_points_parallel_and_geometry", "test_errors"):
         assert f"{name} passed" in r.stdout

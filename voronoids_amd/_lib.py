@@ -28,7 +28,7 @@ EXPORTED = [
     "vo_engine_finish", "vo_engine_results", "vo_engine_stats_get", "vo_engine_config", "vo_batch_triangulate",
     "vo_result_canonical", "vo_clock",
     "vo_server_create", "vo_server_submit", "vo_server_results", "vo_server_stats_get", "vo_server_destroy",
-    "vo_predicates",
+    "vo_predicates", "vo_bounding_sphere",
 ]
 
 
@@ -97,6 +97,7 @@ def lib() -> ctypes.CDLL:
         i32p = ctypes.POINTER(ctypes.c_int32)
         i64p = ctypes.POINTER(ctypes.c_int64)
         L.vo_tri_create.argtypes = [ctypes.c_int, ctypes.c_int, dp, ctypes.c_int64, ctypes.POINTER(P)]
+        L.vo_bounding_sphere.argtypes = [ctypes.c_int, dp, ctypes.c_int64, dp, dp]
         L.vo_tri_destroy.argtypes = [P]
         L.vo_tri_destroy.restype = None
         for name in ("vo_tri_insert", "vo_tri_stage"):

@@ -1157,6 +1157,13 @@ int vo_predicates(int hip_device, int type, const double* pts, int64_t n, const 
 
 const char* vo_last_error(void) { return vo_host::g_err.c_str(); }
 int vo_version(void) { return 1; }
+
+int vo_bounding_sphere(int dim, const double* pts, int64_t n, double* center, double* radius) {
+    if (dim != 2 && dim != 3) return fail(VO_ERR_INVALID, "dim must be 2 or 3");
+    if (n < 1 || !pts || !center || !radius) return fail(VO_ERR_INVALID, "bad argument");
+    vo_host::bounding_sphere(dim, pts, n, center, radius);
+    return 0;
+}
 void vo_free(void* p) { std::free(p); }
 
 int vo_tri_create(int hip_device, int dim, const double* pts, int64_t n, vo_tri** out) {
