@@ -76,3 +76,27 @@ def test_bounding_sphere_is_the_reference_s_bit_for_bit():
     assert L.vo_bounding_sphere(3, Q.ctypes.data_as(dp), 4, c.ctypes.data_as(dp), ctypes.byref(r)) == 0
     assert c.tolist() == [0.5, 1.0, 1.0] and r.value == 1.5
     assert L.vo_bounding_sphere(4, Q.ctypes.data_as(dp), 4, c.ctypes.data_as(dp), ctypes.byref(r)) == _lib.VO_ERR_INVALID
+
+
+def test_bounding_sphere_equals_the_oracle_bitwise():
+    # the product's host routine and the oracle's restatement of src/geometry.rs:99-141 on random
+    # sets, on the reference test's 2D points (one of them on the first sphere: the strict test
+    # decides by rounding) and on sets where the 1.5x growth fires
+    import numpy as np
+    from oracle import oracle as O
+    from voronoids_amd import datasets
+    L = _lib.lib()
+    dp = ctypes.POINTER(ctypes.c_double)
+    sets = [np.array([[0.3, 0.1], [1.0, 0.2], [0.1, 1.0], [0.5, 0.5]])]
+    for dim in (2, 3):
+        for seed in range(5):
+            sets.append(datasets.uniform(1000 + 37 * seed, dim, seed=500 + seed))
+        sets.append(np.vstack([datasets.uniform(50, dim, seed=9), np.full((1, dim), 7.0), -np.full((1, dim), 9.0)]))
+    for P in sets:
+        P = np.ascontiguousarray(P, dtype=np.float64)
+        dim = P.shape[1]
+        c = np.zeros(dim)
+        r = ctypes.c_double()
+        assert L.vo_bounding_sphere(dim, P.ctypes.data_as(dp), P.shape[0], c.ctypes.data_as(dp), ctypes.byref(r)) == 0
+        oc, orad = O.bounding_sphere(P)
+        assert c.tobytes() == np.asarray(oc, np.float64).tobytes() and r.value == float(orad), P.shape
