@@ -90,6 +90,38 @@ void vo_free(void* p);
  * positively oriented.  Writes the number of violations. */
 int vo_tri_check(vo_tri* t, int64_t* n_violations);
 
+/* locate + get_new_simplices (src/delaunay_tree.rs:33-75, :77-123) for one point p[dim] that is
+ * not yet in the tree, against the tree as it is (nothing changes): the simplices whose
+ * circumsphere strictly contains p -- the cavity, as positions in vo_tri_export's numbering,
+ * sorted ascending as the reference sorts them -- and the cavity's boundary facets, one row of
+ * dim + 2 ints per facet: (cavity simplex, the simplex across the facet or -1 on the super-simplex
+ * hull, then the facet's dim vertex ids), so that inserting p as vertex id creates one simplex
+ * [id, facet vertices] per row (positively oriented in that order).  Exact signs.  n_killed and
+ * n_facets always receive the sizes; the rows are written only when both fit (else
+ * VO_ERR_OVERFLOW).  VO_ERR_DUPLICATE when p coincides with a vertex (the reference panics there,
+ * :47-54), VO_ERR_INVALID when p lies outside the super simplex. */
+int vo_tri_locate(vo_tri* t, const double* p, int32_t* killed, int64_t killed_cap, int64_t* n_killed,
+                  int32_t* facets, int64_t facets_cap, int64_t* n_facets);
+
+/* check_delaunay with the reference's own semantics (src/delaunay_tree.rs:512-541, :642-671):
+ * the number of (simplex made only of input points, vertex not in it) pairs whose vertex lies
+ * strictly inside the simplex's circumsphere, super and ghost vertices included (a ghost counts
+ * as the super vertex it copies); the reference returns true iff this is 0.  Exact signs.  It
+ * is O(simplices x vertices): VO_ERR_INVALID beyond 2^36 pairs (use vo_tri_check there). */
+int vo_tri_check_pairs(vo_tri* t, int64_t* n_pairs);
+
+/* The reference's tree exposes its state as pub fields (src/delaunay_tree.rs:24-30), so a
+ * caller can move a vertex (Vertex::coordinates, :18-22) or reorder a simplex's vertices
+ * (Simplex::vertices, :10-16) and then ask check_delaunay about the result.  These two do the
+ * same to the device state; nothing is re-triangulated.
+ *   vo_tri_set_vertex: input point with reference vertex id `id` (2(dim+1) + input index) moves
+ *     to coords[dim]; records = 1 also rewrites the fp32 copies held in its simplices' records
+ *     (records = 0 leaves them stale, which vo_tri_check reports as record mismatches).
+ *   vo_tri_swap_simplex_vertices: the live simplex at position `simplex` of vo_tri_export's
+ *     order swaps its vertex entries i and j (with their coordinates; its neighbor entries stay). */
+int vo_tri_set_vertex(vo_tri* t, int64_t id, const double* coords, int records);
+int vo_tri_swap_simplex_vertices(vo_tri* t, int64_t simplex, int i, int j);
+
 int vo_tri_stats(const vo_tri* t, vo_stats* out);
 
 /* tuning and test knobs (no counterpart in the reference): "filters" = mask of the FP32 (1)
@@ -142,6 +174,20 @@ int vo_version(void);
  * DelaunayTree::new builds its super simplex around (per-axis min/max midpoint, then the radius x 1.5
  * for each point, in input order, not strictly inside).  n >= 1. */
 int vo_bounding_sphere(int dim, const double* pts, int64_t n, double* center, double* radius);
+
+/* geometry::circumsphere (src/geometry.rs:58-88, with circumsphere_2d :2-22 and circumsphere_3d
+ * :24-56), host only, bit for bit: the 2D perpendicular-bisector slope formula (an axis-parallel
+ * first or second edge gives the formula's IEEE result, e.g. NaN for a horizontal one) and the
+ * 3D LU solve in nalgebra 0.32's order; verts = (dim + 1) x dim, vertex 0 first.  A 3D simplex
+ * with a zero pivot returns VO_ERR_INVALID with NaN outputs (the reference panics there). */
+int vo_circumsphere(int dim, const double* verts, double* center, double* radius);
+/* the same for n simplices given as vertex ids into coords (n_coords x dim); degenerate ones get
+ * NaN (used for PySimplex.center / radius) */
+int vo_circumspheres(int dim, const double* coords, int64_t n_coords, const int32_t* simplices, int64_t n,
+                     double* centers, double* radii);
+/* geometry::in_sphere (src/geometry.rs:91-97), host only: 1 iff sum_k (c_k - p_k)^2 < r * r (the
+ * sum in axis order, no fused multiply-add), else 0 */
+int vo_in_sphere(int dim, const double* p, const double* center, double radius);
 
 /* ------------------------------------------------------------------------------------------
  * Several independent point sets on one device (batched and served modes).

@@ -59,67 +59,36 @@ inline void check(int rc) {
     throw Error(rc, (m && *m) ? std::string(m) : "voronoids: error " + std::to_string(rc));
 }
 
-/* circumcenter and radius of M = N + 1 points: c = p0 + x with 2 (p_i - p0) . x = |p_i - p0|^2,
- * solved by Gaussian elimination with partial pivoting (src/geometry.rs:2-56 computes the same
- * sphere from slopes in 2D and an LU solve in 3D).  Degenerate simplices give non-finite values. */
-template <std::size_t N>
-inline void circumsphere(const double* const* p, double* center, double* radius) {
-    double A[N][N + 1];
-    for (std::size_t i = 0; i < N; ++i) {
-        double s = 0.0;
-        for (std::size_t k = 0; k < N; ++k) {
-            A[i][k] = p[i + 1][k] - p[0][k];
-            s += A[i][k] * A[i][k];
-        }
-        A[i][N] = 0.5 * s;
-    }
-    for (std::size_t c = 0; c < N; ++c) {
-        std::size_t piv = c;
-        for (std::size_t r = c + 1; r < N; ++r)
-            if (std::fabs(A[r][c]) > std::fabs(A[piv][c])) piv = r;
-        if (piv != c)
-            for (std::size_t k = 0; k <= N; ++k) std::swap(A[c][k], A[piv][k]);
-        for (std::size_t r = c + 1; r < N; ++r) {
-            const double f = A[r][c] / A[c][c];
-            for (std::size_t k = c; k <= N; ++k) A[r][k] -= f * A[c][k];
-        }
-    }
-    double x[N];
-    for (std::size_t c = N; c-- > 0;) {
-        double s = A[c][N];
-        for (std::size_t k = c + 1; k < N; ++k) s -= A[c][k] * x[k];
-        x[c] = s / A[c][c];
-    }
-    double r2 = 0.0;
-    for (std::size_t k = 0; k < N; ++k) {
-        center[k] = p[0][k] + x[k];
-        r2 += x[k] * x[k];
-    }
-    *radius = std::sqrt(r2);
-}
 }  // namespace detail
 
 /* the reference's geometry module (src/geometry.rs) on the host */
 namespace geometry {
 
-/* circumsphere (src/geometry.rs:58-83): center and radius of N + 1 points (the same sphere as the
- * reference's slope (2D) and LU (3D) formulas, computed by elimination with pivoting) */
+/* circumsphere (src/geometry.rs:58-88): the reference's own formulas bit for bit, computed by the
+ * library (vo_circumsphere): the slope formula in 2D (:2-22; an axis-parallel first or second edge
+ * gives its IEEE result, NaN for a horizontal one) and nalgebra's LU solve in 3D (:24-56).  A 3D
+ * simplex with a zero pivot throws voronoids::Error (the reference panics on the failed solve). */
 template <std::size_t N, std::size_t M>
 std::pair<std::array<double, N>, double> circumsphere(const std::array<std::array<double, N>, M>& vertices) {
+    static_assert(N == 2 || N == 3, "2D and 3D only");
     static_assert(M == N + 1, "N + 1 points");
-    const double* p[M];
-    for (std::size_t k = 0; k < M; ++k) p[k] = vertices[k].data();
+    double v[M * N];
+    for (std::size_t k = 0; k < M; ++k)
+        for (std::size_t j = 0; j < N; ++j) v[k * N + j] = vertices[k][j];
     std::pair<std::array<double, N>, double> out;
-    detail::circumsphere<N>(p, out.first.data(), &out.second);
+    detail::check(vo_circumsphere((int)N, v, out.first.data(), &out.second));
     return out;
 }
 
-/* in_sphere (src/geometry.rs:91-97): the reference's strict test, squared distance < radius^2 */
+/* in_sphere (src/geometry.rs:91-97): the reference's strict test, squared distance < radius^2,
+ * evaluated by the library (compiled without fused multiply-adds, so that the caller's compiler
+ * flags cannot change a boundary decision) */
 template <std::size_t N>
 bool in_sphere(const std::array<double, N>& vertex, const std::array<double, N>& center, double radius) {
-    double distance = 0.0;
-    for (std::size_t i = 0; i < N; ++i) distance += (center[i] - vertex[i]) * (center[i] - vertex[i]);
-    return distance < radius * radius;
+    static_assert(N == 2 || N == 3, "2D and 3D only");
+    const int r = vo_in_sphere((int)N, vertex.data(), center.data(), radius);
+    detail::check(r < 0 ? r : 0);
+    return r == 1;
 }
 
 /* bounding_sphere (src/geometry.rs:99-141), bit for bit (vo_bounding_sphere) */
@@ -294,14 +263,16 @@ public:
         export_arrays(s, nb);
         const std::vector<Point> x = coordinates();
         std::vector<Simplex<N, M>> out(s.size() / M);
+        std::vector<double> centers(out.size() * N), radii(out.size());
+        detail::check(vo_circumspheres((int)N, x.empty() ? nullptr : x[0].data(), (int64_t)x.size(), s.data(),
+                                       (int64_t)out.size(), centers.data(), radii.data()));
         for (std::size_t i = 0; i < out.size(); ++i) {
-            const double* p[M];
             for (std::size_t k = 0; k < M; ++k) {
                 out[i].vertices[k] = (std::size_t)s[i * M + k];
-                p[k] = x[(std::size_t)s[i * M + k]].data();
                 if (nb[i * M + k] >= 0) out[i].neighbors.push_back((std::size_t)nb[i * M + k]);
             }
-            detail::circumsphere<N>(p, out[i].center.data(), &out[i].radius);
+            for (std::size_t j = 0; j < N; ++j) out[i].center[j] = centers[i * N + j];
+            out[i].radius = radii[i];
         }
         return out;
     }

@@ -90,6 +90,9 @@ int64_t  or_tree_vertex_simplices(const or_tree* t, int64_t v, int64_t* out, int
 /* check_delaunay (src/delaunay_tree.rs:512-541 / 642-671) with the tree's predicate: number of
  * (simplex, vertex) violations over simplices made only of real vertices (O(T*V)). */
 int64_t  or_tree_check_delaunay(const or_tree* t);
+/* vertex v moves to p[dim] without re-triangulation (the reference's pub Vertex::coordinates);
+ * -1 if v is out of range */
+int      or_tree_set_vertex(or_tree* t, int64_t v, const double* p);
 
 /* find_placement (src/scheduler.rs:30-55) on an explicit queue: footprints given as CSR */
 int      or_find_placement(int64_t n, const int64_t* fp_off, const int64_t* fp, int64_t* placement);

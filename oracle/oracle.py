@@ -68,6 +68,7 @@ def lib():
         L.or_tree_vertex_simplices.argtypes = [ctypes.c_void_p, ctypes.c_int64, _i64p, ctypes.c_int64]
         L.or_tree_vertex_simplices.restype = ctypes.c_int64
         L.or_find_placement.argtypes = [ctypes.c_int64, _i64p, _i64p, _i64p]
+        L.or_tree_set_vertex.argtypes = [ctypes.c_void_p, ctypes.c_int64, _dp]
         _lib = L
     return _lib
 
@@ -216,6 +217,12 @@ class Tree:
 
     def check_delaunay(self) -> int:
         return int(lib().or_tree_check_delaunay(self.h))
+
+    def set_vertex(self, v: int, p) -> None:
+        """vertex v moves to p, nothing re-triangulated (the reference's pub Vertex::coordinates)"""
+        q = np.ascontiguousarray(p, dtype=np.float64)
+        if lib().or_tree_set_vertex(self.h, int(v), _ptr(q, _dp)) != 0:
+            raise IndexError("no vertex %d" % v)
 
     def canonical(self) -> "Canonical":
         ex = self.export()

@@ -751,3 +751,21 @@ int64_t or_tree_check_delaunay(const or_tree* t) {
     }
     return bad;
 }
+
+/* Vertex v moves to p: the reference's Vertex::coordinates is a pub field
+ * (src/delaunay_tree.rs:18-22), so a caller can do this before check_delaunay.  Nothing is
+ * re-triangulated; exact mode re-derives the stored orientation of v's simplices (its conflict
+ * test reads coordinates), reference mode keeps its stored spheres, as the reference does. */
+int or_tree_set_vertex(or_tree* t, int64_t v, const double* p) {
+    if (v < 0 || v >= t->nv) return -1;
+    memcpy(t->X + v * t->dim, p, sizeof(double) * (size_t)t->dim);
+    if (t->mode != OR_MODE_EXACT) return 0;
+    for (int64_t s = 0; s <= t->max_id; ++s) {
+        if (!t->alive[s]) continue;
+        const int32_t* vs = t->sv + s * t->M;
+        int has = 0;
+        for (int i = 0; i < t->M; ++i) has |= vs[i] == v;
+        if (has) t->sor[s] = (int8_t)orientation_of(t, vs);
+    }
+    return 0;
+}

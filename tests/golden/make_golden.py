@@ -12,7 +12,11 @@ fixtures pin the oracle from outside it:
   * kat.json : known answers -- the reference test's circumsphere KAT
     (tests/test_geometry.rs:5-15), max_simplex_id after new (tests/test_delaunay_tree.rs:20,44),
     the super-simplex trigonometric constants, and the C1 bounding sphere bits;
-  * c1_points.npy : the 3D n=10,000 seed-0 uniform input (pins the generator too).
+  * c1_points.npy : the 3D n=10,000 seed-0 uniform input (pins the generator too);
+  * oracle_hashes.json : the oracle's exact-mode canonical hashes of inputs too slow for it to
+    run inside a GPU test (a 32^3 block of the 1/128 lattice with 1e-9 jitter: about 30 s;
+    200k clustered points); the oracle itself is pinned by the fixtures above.
+    `python tests/golden/make_golden.py --oracle-only` rewrites just this file.
 """
 from __future__ import annotations
 
@@ -87,7 +91,28 @@ def brute_canonical(P: np.ndarray) -> O.Canonical:
     return O.canonical_from_real(np.array(simplices, np.int64), dim)
 
 
+ORACLE_CASES = {
+    "lattice_block_32_of_128": lambda: datasets.lattice_block(32, side=128, seed=7),
+    "clustered_3d_200k": lambda: datasets.clustered(200_000, seed=3),
+}
+
+
+def oracle_hashes() -> None:
+    out = {}
+    for name, make in ORACLE_CASES.items():
+        P = make()
+        c = O.Tree.delaunay(P, O.MODE_EXACT).canonical()
+        out[name] = {"n": int(P.shape[0]), "dim": int(P.shape[1]), "nT": int(c.T.shape[0]),
+                     "nE": int(c.E.shape[0]), "sha256": c.sha256}
+        print(name, out[name])
+    with open(os.path.join(OUT, "oracle_hashes.json"), "w") as f:
+        json.dump(out, f, indent=1, sort_keys=True)
+
+
 def main() -> None:
+    if "--oracle-only" in sys.argv:
+        oracle_hashes()
+        return
     summary = {}
     c1 = datasets.uniform(10_000, 3, seed=0)
     np.save(os.path.join(OUT, "c1_points.npy"), c1)
@@ -136,6 +161,7 @@ def main() -> None:
     }
     with open(os.path.join(OUT, "kat.json"), "w") as f:
         json.dump(kat, f, indent=1)
+    oracle_hashes()
 
 
 if __name__ == "__main__":

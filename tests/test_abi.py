@@ -100,3 +100,33 @@ def test_bounding_sphere_equals_the_oracle_bitwise():
         assert L.vo_bounding_sphere(dim, P.ctypes.data_as(dp), P.shape[0], c.ctypes.data_as(dp), ctypes.byref(r)) == 0
         oc, orad = O.bounding_sphere(P)
         assert c.tobytes() == np.asarray(oc, np.float64).tobytes() and r.value == float(orad), P.shape
+
+
+_STATIC_MAIN = r"""
+#include <stdio.h>
+#include "voronoids.h"
+int main(void) {
+    const double pts[6] = {-1, 0, 0, 1, 0, 0};
+    double c[3], r;
+    if (vo_version() != 1 || vo_bounding_sphere(3, pts, 2, c, &r) != 0) return 1;
+    printf("static ok %g %g %g %g\n", c[0], c[1], c[2], r);
+    return r == 1.5 ? 0 : 2;
+}
+"""
+
+
+def test_static_archive_links_and_runs(tmp_path):
+    # libvoronoids_hip.a (built beside the shared library): a C program linked against the archive
+    import shutil
+    import subprocess
+    gcc = shutil.which("gcc")
+    archive = os.path.join(ROOT, "voronoids_amd", "libvoronoids_hip.a")
+    assert os.path.exists(archive), "run __graft_entry__.build()"
+    src, exe = tmp_path / "m.c", tmp_path / "m"
+    src.write_text(_STATIC_MAIN)
+    r = subprocess.run([gcc, "-I", os.path.join(ROOT, "include"), "-o", str(exe), str(src), archive,
+                        "-L/opt/rocm/lib", "-lamdhip64", "-lstdc++", "-lm", "-lpthread", "-Wl,-rpath,/opt/rocm/lib"],
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    r = subprocess.run([str(exe)], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0 and "static ok 0 0 0 1.5" in r.stdout, r.stdout + r.stderr

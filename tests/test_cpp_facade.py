@@ -64,29 +64,35 @@ _SPHERES = r"""
 #include <cstdio>
 #include "voronoids.hpp"
 int main() {
-    // the unit corner tetrahedron: center (1/2, 1/2, 1/2), radius sqrt(3)/2
-    const double t[4][3] = {{0, 0, 0}, {1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
-    const double* pt[4] = {t[0], t[1], t[2], t[3]};
-    double c3[3], r3;
-    voronoids::detail::circumsphere<3>(pt, c3, &r3);
-    // a right triangle: center at the hypotenuse's midpoint (2, 1.5), radius 2.5
-    const double u[3][2] = {{0, 0}, {4, 0}, {0, 3}};
-    const double* pu[3] = {u[0], u[1], u[2]};
-    double c2[2], r2;
-    voronoids::detail::circumsphere<2>(pu, c2, &r2);
-    // a translated, rotated regular tetrahedron: every vertex at the radius
-    const double s[4][3] = {{1001, 1001, 1001}, {1001, 999, 999}, {999, 1001, 999}, {999, 999, 1001}};
-    const double* ps[4] = {s[0], s[1], s[2], s[3]};
-    double cs[3], rs;
-    voronoids::detail::circumsphere<3>(ps, cs, &rs);
-    bool ok = std::fabs(c3[0] - 0.5) < 1e-15 && std::fabs(c3[1] - 0.5) < 1e-15 && std::fabs(c3[2] - 0.5) < 1e-15 &&
-              std::fabs(r3 - std::sqrt(3.0) / 2) < 1e-15 && std::fabs(c2[0] - 2) < 1e-15 && std::fabs(c2[1] - 1.5) < 1e-15 &&
-              std::fabs(r2 - 2.5) < 1e-15;
+    using voronoids::geometry::circumsphere;
+    using voronoids::geometry::in_sphere;
+    // tests/test_geometry.rs:5-15, bit for bit: the unit corner tetrahedron
+    const auto k = circumsphere<3, 4>({{{0, 0, 0}, {1, 0, 0}, {0, 1, 0}, {0, 0, 1}}});
+    bool ok = k.first[0] == 0.5 && k.first[1] == 0.5 && k.first[2] == 0.5 && k.second == 0.8660254037844386;
+    // circumsphere_2d's slopes exact in binary: center (3, 1), radius fl(sqrt 10)
+    const auto e = circumsphere<2, 3>({{{0, 0}, {2, 4}, {6, 2}}});
+    ok = ok && e.first[0] == 3.0 && e.first[1] == 1.0 && e.second == std::sqrt(10.0);
+    // a horizontal first edge: the reference's slope formula gives NaN
+    const auto h = circumsphere<2, 3>({{{0, 0}, {4, 0}, {0, 3}}});
+    ok = ok && std::isnan(h.first[0]) && std::isnan(h.second);
+    // a translated regular tetrahedron: every vertex at the radius
+    const std::array<std::array<double, 3>, 4> s = {{{1001, 1001, 1001}, {1001, 999, 999}, {999, 1001, 999}, {999, 999, 1001}}};
+    const auto r = circumsphere<3, 4>(s);
     for (int i = 0; i < 4; ++i) {
-        const double d = std::sqrt((s[i][0] - cs[0]) * (s[i][0] - cs[0]) + (s[i][1] - cs[1]) * (s[i][1] - cs[1]) +
-                                   (s[i][2] - cs[2]) * (s[i][2] - cs[2]));
-        ok = ok && std::fabs(d - rs) < 1e-12 && std::fabs(rs - std::sqrt(3.0)) < 1e-12;
+        const double d = std::sqrt((s[i][0] - r.first[0]) * (s[i][0] - r.first[0]) + (s[i][1] - r.first[1]) * (s[i][1] - r.first[1]) +
+                                   (s[i][2] - r.first[2]) * (s[i][2] - r.first[2]));
+        ok = ok && std::fabs(d - r.second) < 1e-12 && std::fabs(r.second - std::sqrt(3.0)) < 1e-12;
     }
+    // coplanar points: the reference's LU solve fails (it panics); the facade throws
+    bool threw = false;
+    try {
+        circumsphere<3, 4>({{{0, 0, 0}, {1, 0, 0}, {0, 1, 0}, {1, 1, 0}}});
+    } catch (const voronoids::Error& err) {
+        threw = err.code() == VO_ERR_INVALID;
+    }
+    ok = ok && threw;
+    // in_sphere is strict: a point on the sphere is outside
+    ok = ok && !in_sphere<3>({3, 4, 0}, {0, 0, 0}, 5.0) && in_sphere<3>({3, 4 - 0x1p-50, 0}, {0, 0, 0}, 5.0);
     std::printf(ok ? "spheres ok\n" : "spheres FAILED\n");
     return ok ? 0 : 1;
 }
@@ -94,14 +100,16 @@ int main() {
 
 
 def test_facade_circumsphere_known_answers(tmp_path):
-    # the Simplex view's center and radius (include/voronoids.hpp detail::circumsphere)
+    # the geometry module of the facade, which calls the library (vo_circumsphere / vo_in_sphere)
     cxx = shutil.which("g++")
     if cxx is None:
         pytest.skip("g++ not found")
+    libdir = os.path.join(ROOT, "voronoids_amd")
     src, exe = tmp_path / "spheres.cpp", tmp_path / "spheres"
     src.write_text(_SPHERES)
-    r = subprocess.run([cxx, "-std=c++17", "-O2", "-Wall", "-Wextra", "-Werror", "-I", os.path.join(ROOT, "include"),
-                        "-o", str(exe), str(src)], capture_output=True, text=True)
+    r = subprocess.run([cxx, "-std=c++17", "-O2", "-mfma", "-Wall", "-Wextra", "-Werror", "-I",
+                        os.path.join(ROOT, "include"), "-o", str(exe), str(src), "-L", libdir, "-lvoronoids_hip",
+                        "-Wl,-rpath," + libdir], capture_output=True, text=True)
     assert r.returncode == 0, r.stderr
     r = subprocess.run([str(exe)], capture_output=True, text=True, timeout=60)
     assert r.returncode == 0 and "spheres ok" in r.stdout, r.stdout + r.stderr

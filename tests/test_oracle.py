@@ -93,6 +93,18 @@ def test_oracle_matches_qhull_on_p_union_s(name):
     assert c.sha256 == ref["sha256"]
 
 
+@pytest.mark.parametrize("name", ["c1_3d_10k", "uniform_2d_10k"])
+def test_reference_arithmetic_matches_qhull(name):
+    # the reference's own arithmetic (stored slope / LU spheres, strict distance test) reproduces
+    # DT(P u S) on the CPU-runnable configuration and on 2D 10k
+    with open(os.path.join(G, "qhull_hashes.json")) as f:
+        ref = json.load(f)[name]
+    P = np.load(os.path.join(G, "c1_points.npy")) if name == "c1_3d_10k" else datasets.uniform(10_000, 2, seed=0)
+    c = O.Tree.delaunay(P, O.MODE_REF).canonical()
+    assert (c.T.shape[0], c.E.shape[0]) == (ref["nT"], ref["nE"])
+    assert c.sha256 == ref["sha256"]
+
+
 def test_oracle_matches_qhull_small_arrays():
     z = np.load(os.path.join(G, "qhull_small.npz"))
     keys = sorted({k.rsplit("_", 1)[0] for k in z.files})

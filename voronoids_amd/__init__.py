@@ -1,6 +1,7 @@
 """voronoids_amd: MI355X-native parallel randomized incremental Delaunay construction (2D/3D).
 
-Drop-in for the reference's surface:
+Drop-in for the reference's surface (the ``voronoids`` package re-exports it under the
+reference's module name):
   * ``DelaunayTree(points)`` == ``DelaunayTree::new`` (src/delaunay_tree.rs:389-510, :544-640):
     bounding sphere, radius x10, super simplex; no point inserted;
   * ``DelaunayTree.add_points_to_tree(points)`` == ``add_points_to_tree`` (:336-386);
@@ -9,9 +10,14 @@ Drop-in for the reference's surface:
     ``PyDelauanyTree`` with ``max_simplex_id``, ``vertices`` and ``simplices`` (src/lib.rs:62-102).
 
 All construction runs on the GPU through libvoronoids_hip.so (include/voronoids.h); there is
-no CPU fallback.  Differences from the reference, by design: simplex ids are export
-positions, not creation-history ids; hull faces have no ghost simplices (no neighbor);
-``center``/``radius`` are computed on export.
+no CPU fallback.  The views hold what the reference's maps hold: the super vertices 0..d, the
+ghost vertices d+1..2d+1, the ghost simplices 1..d+1 (src/delaunay_tree.rs:467-502, :606-632)
+with their incidence and adjacency, and every live simplex, super-incident ones included.
+Differences from the reference, by design: a live simplex's id is d + 2 + its position in the
+export (not its creation id; the super simplex before the first insertion keeps id 0), its
+neighbor list is positional (entry k across the face opposite vertex k), and ``center`` /
+``radius`` are computed when the view is built, with the reference's own formulas
+(src/geometry.rs:2-56, ``vo_circumspheres``), instead of being stored.
 """
 from __future__ import annotations
 
@@ -55,18 +61,12 @@ class PySimplex:
     neighbors: list
 
 
-def _circumspheres(X: np.ndarray, S: np.ndarray):
-    """circumcenters and radii of simplices S (vertex rows) with coordinates X"""
-    P = X[S]                                    # (n, M, d)
-    A = P[:, 1:, :] - P[:, :1, :]
-    b = 0.5 * np.sum(A * A, axis=2)
-    try:
-        c = np.linalg.solve(A, b[..., None])[..., 0]
-    except np.linalg.LinAlgError:
-        c = np.full(b.shape, np.nan)
-    center = P[:, 0, :] + c
-    radius = np.sqrt(np.sum(c * c, axis=1))
-    return center, radius
+def _ghosts(dim: int) -> list:
+    """ghost simplices 1..d+1 of DelaunayTree::new (src/delaunay_tree.rs:467-502 (3D), :606-632 (2D)):
+    ghost vertex first, then the super-hull face it guards"""
+    if dim == 3:
+        return [[4, 0, 1, 2], [5, 0, 2, 3], [6, 0, 3, 1], [7, 1, 2, 3]]
+    return [[3, 0, 1], [4, 0, 2], [5, 1, 2]]
 
 
 def _tuned(dim: int) -> dict:
@@ -89,6 +89,14 @@ class DelaunayTree:
         _lib.check(_lib.lib().vo_tri_create(device, self.dim, p.ctypes.data_as(_dp), p.shape[0], ctypes.byref(h)))
         self._h = h
         self._n_input = 0
+        # the sphere DelaunayTree::new stores on simplex 0: (bounding-sphere center, 10 x radius)
+        # (an empty or single-point input, which has no super simplex in the reference, uses radius 1)
+        c = np.zeros(3)
+        r = ctypes.c_double(0.0)
+        if p.shape[0]:
+            _lib.check(_lib.lib().vo_bounding_sphere(self.dim, p.ctypes.data_as(_dp), p.shape[0],
+                                                     c.ctypes.data_as(_dp), ctypes.byref(r)))
+        self._root_sphere = (c[:self.dim].tolist(), 10.0 * (r.value if r.value > 0 else 1.0))
         for key, value in _tuned(self.dim).items():
             self.config(key, value)
 
@@ -170,21 +178,93 @@ class DelaunayTree:
         _lib.check(_lib.lib().vo_tri_stats(self._h, ctypes.byref(s)))
         return s.as_dict()
 
-    @property
-    def simplices(self) -> dict:
+    def locate(self, point) -> tuple[np.ndarray, np.ndarray]:
+        """locate + get_new_simplices (src/delaunay_tree.rs:33-123) against the current tree,
+        nothing changed (vo_tri_locate): the cavity as sorted export positions, and one row per
+        boundary facet: (cavity simplex, simplex across or -1, the facet's vertex ids)"""
+        q = np.ascontiguousarray(point, dtype=np.float64)
+        if q.shape != (self.dim,):
+            raise ValueError(f"point must have {self.dim} coordinates")
+        L = _lib.lib()
+        nk, nf = ctypes.c_int64(), ctypes.c_int64()
+        cap_k, cap_f = 64, 128
+        while True:
+            k = np.zeros(cap_k, np.int32)
+            f = np.zeros((cap_f, self.M + 1), np.int32)
+            rc = L.vo_tri_locate(self._h, q.ctypes.data_as(_dp), k.ctypes.data_as(_i32p), cap_k, ctypes.byref(nk),
+                                 f.ctypes.data_as(_i32p), cap_f, ctypes.byref(nf))
+            if rc == _lib.VO_ERR_OVERFLOW and (nk.value > cap_k or nf.value > cap_f):
+                cap_k, cap_f = max(cap_k, nk.value), max(cap_f, nf.value)
+                continue
+            _lib.check(rc)
+            return k[: nk.value], f[: nf.value]
+
+    def check_pairs(self) -> int:
+        """the reference's own check_delaunay count (vo_tri_check_pairs): (all-real simplex, vertex)
+        pairs with the vertex strictly inside the sphere; O(simplices x vertices)"""
+        n = ctypes.c_int64()
+        _lib.check(_lib.lib().vo_tri_check_pairs(self._h, ctypes.byref(n)))
+        return n.value
+
+    def set_vertex(self, vertex_id: int, point, records: bool = True) -> None:
+        """move an inserted vertex without re-triangulating (the reference's pub
+        Vertex::coordinates); records=False leaves the simplex records' copies stale"""
+        q = np.ascontiguousarray(point, dtype=np.float64)
+        if q.shape != (self.dim,):
+            raise ValueError(f"point must have {self.dim} coordinates")
+        _lib.check(_lib.lib().vo_tri_set_vertex(self._h, int(vertex_id), q.ctypes.data_as(_dp), int(records)))
+
+    def swap_simplex_vertices(self, simplex: int, i: int, j: int) -> None:
+        """swap vertex entries i and j of the live simplex at export position `simplex` (the
+        reference's pub Simplex::vertices); its neighbor entries stay"""
+        _lib.check(_lib.lib().vo_tri_swap_simplex_vertices(self._h, int(simplex), int(i), int(j)))
+
+    def _views(self):
+        """ids, vertex rows, neighbor-id rows of live simplices, the ghost rows, coordinates"""
+        d, M = self.dim, self.M
         s, nb = self.export()
         X = self.coordinates()
-        center, radius = _circumspheres(X, s.astype(np.int64))
-        return {i: PySimplex(s[i].tolist(), center[i].tolist(), float(radius[i]), [int(x) for x in nb[i] if x >= 0])
-                for i in range(s.shape[0])}
+        n = s.shape[0]
+        ids = (d + 2) + np.arange(n, dtype=np.int64)
+        root = np.all(s < M, axis=1)               # the super simplex, live until the first insertion
+        ids[root] = 0
+        ghosts = _ghosts(d)
+        face_ghost = {frozenset(g[1:]): k + 1 for k, g in enumerate(ghosts)}
+        nbr_ids = np.where(nb >= 0, ids[np.maximum(nb, 0)], -1)
+        ghost_nbr = [[] for _ in ghosts]
+        for i, f in zip(*np.nonzero(nb < 0)):
+            g = face_ghost[frozenset(int(x) for k, x in enumerate(s[i]) if k != f)]
+            nbr_ids[i, f] = g
+            ghost_nbr[g - 1].append(int(ids[i]))
+        return ids, s, nbr_ids, ghosts, ghost_nbr, X
+
+    @property
+    def simplices(self) -> dict:
+        ids, s, nbr_ids, ghosts, ghost_nbr, X = self._views()
+        n, d = s.shape[0], self.dim
+        center = np.zeros((n, d))
+        radius = np.zeros(n)
+        if n:
+            _lib.check(_lib.lib().vo_circumspheres(d, X.ctypes.data_as(_dp), X.shape[0],
+                                                   np.ascontiguousarray(s).ctypes.data_as(_i32p), n,
+                                                   center.ctypes.data_as(_dp), radius.ctypes.data_as(_dp)))
+        out = {}
+        for k, g in enumerate(ghosts):              # center 0, radius 0: never in conflict
+            out[k + 1] = PySimplex(list(g), [0.0] * d, 0.0, ghost_nbr[k])
+        for i in range(n):
+            c, r = center[i].tolist(), float(radius[i])
+            if ids[i] == 0:                          # simplex 0 keeps the sphere new() gave it
+                c, r = list(self._root_sphere[0]), float(self._root_sphere[1])
+            out[int(ids[i])] = PySimplex(s[i].tolist(), c, r, nbr_ids[i].tolist())
+        return out
 
     @property
     def vertices(self) -> dict:
-        s, _ = self.export()
-        X = self.coordinates()
-        flat = s.ravel().astype(np.int64)
-        owner = np.repeat(np.arange(s.shape[0]), self.M)
-        order = np.argsort(flat, kind="stable")
+        ids, s, _, ghosts, _, X = self._views()
+        M = self.M
+        flat = np.concatenate([s.ravel().astype(np.int64), np.asarray(ghosts, np.int64).ravel()])
+        owner = np.concatenate([np.repeat(ids, M), np.repeat(np.arange(1, M + 1, dtype=np.int64), M)])
+        order = np.lexsort((owner, flat))
         flat, owner = flat[order], owner[order]
         bounds = np.searchsorted(flat, np.arange(X.shape[0] + 1))
         return {v: PyVertex(X[v].tolist(), owner[bounds[v]:bounds[v + 1]].tolist()) for v in range(X.shape[0])}

@@ -29,6 +29,8 @@ EXPORTED = [
     "vo_result_canonical", "vo_clock",
     "vo_server_create", "vo_server_submit", "vo_server_results", "vo_server_stats_get", "vo_server_destroy",
     "vo_predicates", "vo_bounding_sphere",
+    "vo_tri_check_pairs", "vo_tri_set_vertex", "vo_tri_swap_simplex_vertices",
+    "vo_circumsphere", "vo_circumspheres", "vo_in_sphere", "vo_tri_locate",
 ]
 
 
@@ -109,6 +111,13 @@ def lib() -> ctypes.CDLL:
         L.vo_free.argtypes = [ctypes.c_void_p]
         L.vo_free.restype = None
         L.vo_tri_check.argtypes = [P, i64p]
+        L.vo_tri_check_pairs.argtypes = [P, i64p]
+        L.vo_tri_locate.argtypes = [P, dp, i32p, ctypes.c_int64, i64p, i32p, ctypes.c_int64, i64p]
+        L.vo_tri_set_vertex.argtypes = [P, ctypes.c_int64, dp, ctypes.c_int]
+        L.vo_tri_swap_simplex_vertices.argtypes = [P, ctypes.c_int64, ctypes.c_int, ctypes.c_int]
+        L.vo_circumsphere.argtypes = [ctypes.c_int, dp, dp, dp]
+        L.vo_circumspheres.argtypes = [ctypes.c_int, dp, ctypes.c_int64, i32p, ctypes.c_int64, dp, dp]
+        L.vo_in_sphere.argtypes = [ctypes.c_int, dp, dp, ctypes.c_double]
         L.vo_tri_stats.argtypes = [P, ctypes.POINTER(VoStats)]
         L.vo_tri_vertices.argtypes = [P, dp]
         L.vo_tri_config.argtypes = [P, ctypes.c_char_p, ctypes.c_int64]

@@ -56,6 +56,94 @@ inline void bounding_sphere(int D, const double* pts, int64_t n, double* c, doub
     *r = rad;
 }
 
+// circumsphere, src/geometry.rs:2-56, in the reference's evaluation order.  Returns false when
+// the 3D solve meets a zero pivot (the reference unwraps a failed LU solve there, :49).
+//  2D (:2-22): perpendicular bisectors through the edge midpoints by their slopes; an edge
+//     parallel to an axis makes an infinite or zero slope, and the IEEE results of the formula
+//     are kept (NaN when the first or second edge is horizontal).
+//  3D (:24-56): rows D_i = v_i - v_0, Mid_i = (v_i + v_0) / 2, right-hand side
+//     planes_i = ((0 + D_i0 Mid_i0) + D_i1 Mid_i1) + D_i2 Mid_i2 (nalgebra's column_sum starts
+//     from zeros), center = the LU solve of D c = planes as nalgebra 0.32 computes it: for each
+//     column the first row of largest |entry| is the pivot, the rows are swapped, the entries
+//     below are scaled by 1 / pivot and the rest updated by (-u) * l + a (a product, then a
+//     sum); the solve applies the swaps in order, then unit-lower forward and upper backward
+//     substitution, each column subtracted as (-coeff) * entry + b.  The radius is measured
+//     from v_0, summing x, y, z in order.
+inline bool ref_circumsphere(int D, const double* v, double* center, double* radius) {
+    if (D == 2) {
+        const double x1 = v[0], y1 = v[1], x2 = v[2], y2 = v[3], x3 = v[4], y3 = v[5];
+        const double d0 = (x1 + x2) / 2.0, d1 = (y1 + y2) / 2.0;
+        const double e0 = (x2 + x3) / 2.0, e1 = (y2 + y3) / 2.0;
+        const double m_ab = (y2 - y1) / (x2 - x1);
+        const double m_bc = (y3 - y2) / (x3 - x2);
+        const double m_d = -1. / m_ab;
+        const double m_e = -1. / m_bc;
+        const double x = (m_d * d0 - m_e * e0 + e1 - d1) / (m_d - m_e);
+        const double y = m_d * (x - d0) + d1;
+        center[0] = x;
+        center[1] = y;
+        *radius = std::sqrt((x - x1) * (x - x1) + (y - y1) * (y - y1));
+        return true;
+    }
+    double a[3][3], b[3];
+    for (int i = 0; i < 3; ++i) {
+        double s = 0.0;
+        for (int k = 0; k < 3; ++k) {
+            a[i][k] = v[(i + 1) * 3 + k] - v[k];
+            const double mid = (v[(i + 1) * 3 + k] + v[k]) / 2.0;
+            s += a[i][k] * mid;
+        }
+        b[i] = s;
+    }
+    int swap_i[3], swap_p[3], n_swaps = 0;
+    for (int i = 0; i < 3; ++i) {
+        int piv = i;
+        double best = std::fabs(a[i][i]);
+        for (int r = i + 1; r < 3; ++r)
+            if (std::fabs(a[r][i]) > best) {
+                best = std::fabs(a[r][i]);
+                piv = r;
+            }
+        const double diag = a[piv][i];
+        if (diag == 0.0) continue;
+        if (piv != i) {
+            swap_i[n_swaps] = i;
+            swap_p[n_swaps++] = piv;
+            for (int k = 0; k < 3; ++k) std::swap(a[i][k], a[piv][k]);
+        }
+        const double inv = 1.0 / diag;
+        for (int r = i + 1; r < 3; ++r) a[r][i] = a[r][i] * inv;
+        for (int k = i + 1; k < 3; ++k)
+            for (int r = i + 1; r < 3; ++r) a[r][k] = (-a[i][k]) * a[r][i] + a[r][k];
+    }
+    double x[3] = {b[0], b[1], b[2]};
+    for (int s = 0; s < n_swaps; ++s) std::swap(x[swap_i[s]], x[swap_p[s]]);
+    for (int i = 0; i < 2; ++i) {
+        const double coeff = x[i] / 1.0;
+        for (int r = i + 1; r < 3; ++r) x[r] = (-coeff) * a[r][i] + x[r];
+    }
+    for (int i = 2; i >= 0; --i) {
+        if (a[i][i] == 0.0) {
+            center[0] = center[1] = center[2] = *radius = NAN;
+            return false;
+        }
+        const double coeff = x[i] / a[i][i];
+        x[i] = coeff;
+        for (int r = 0; r < i; ++r) x[r] = (-coeff) * a[r][i] + x[r];
+    }
+    for (int k = 0; k < 3; ++k) center[k] = x[k];
+    *radius = std::sqrt((v[0] - x[0]) * (v[0] - x[0]) + (v[1] - x[1]) * (v[1] - x[1]) + (v[2] - x[2]) * (v[2] - x[2]));
+    return true;
+}
+
+// in_sphere, src/geometry.rs:91-97: squared distance summed over the axes in order, strictly
+// below radius * radius
+inline bool ref_in_sphere(int D, const double* p, const double* c, double radius) {
+    double distance = 0.0;
+    for (int i = 0; i < D; ++i) distance += (c[i] - p[i]) * (c[i] - p[i]);
+    return distance < radius * radius;
+}
+
 // center and radius the super simplex is built from: the reference's bounding sphere, except
 // that a zero radius (one point, or all points equal), for which the reference has no
 // super simplex (all its vertices coincide), is replaced by 1

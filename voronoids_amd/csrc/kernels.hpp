@@ -1096,4 +1096,245 @@ __global__ void __launch_bounds__(256) k_check(Ctx c, int n, unsigned long long*
     flush_counters(c.cnt, lc);
 }
 
+// check_delaunay with the reference's own semantics (src/delaunay_tree.rs:512-541 / 642-671):
+// for every live simplex made only of inserted points, the vertices strictly inside its
+// circumsphere that are not its own, counted with exact signs (inside = in-sphere sign times
+// orientation sign > 0, so a simplex whose orientation a moved vertex flipped still counts
+// what lies inside its sphere).  Vertex w < M is a super vertex; w_weight[w] is how many
+// reference vertices share its coordinates (itself plus its ghost copies).  One block per
+// simplex at a time, the block's threads over the vertices: O(T * V), for small trees.
+template <int D>
+__global__ void __launch_bounds__(256) k_check_pairs(Ctx c, int n, int nv, int4 w_weight, unsigned long long* bad,
+                                                     int* unresolved) {
+    using V = VT<D>;
+    constexpr int M = D + 1;
+    __shared__ int s_o;
+    LocalCnt lc = {};
+    unsigned long long nbad = 0;
+    unsigned nun = 0;
+    for (int t = blockIdx.x; t < n; t += gridDim.x) {
+        const int4 v = ld_v<D>(c, t);
+        if (v.x < 0) continue;
+        bool real = true;
+        for (int k = 0; k < M; ++k) real &= comp(v, k) >= c.M;
+        if (!real) continue;
+        typename V::F x[4];
+        load_tx<D>(c, t, x);
+        // the orientation once per simplex (one queue entry when it needs the exact tier)
+        if (threadIdx.x == 0) s_o = orient_at<D>(c, x, t, 0, v.x, x[0], lc);
+        __syncthreads();
+        const int o = s_o;
+        __syncthreads();
+        if (o == SIGN_UNDECIDED) {
+            if (threadIdx.x == 0) nun++;
+            continue;
+        }
+        if (o == 0) continue;                 // flat: no sphere, nothing inside
+        for (int w = threadIdx.x; w < nv; w += blockDim.x) {
+            bool own = false;
+            for (int k = 0; k < M; ++k) own |= comp(v, k) == w;
+            if (own) continue;
+            const int s = conflict_at<D>(c, x, t, w, V::ld(c.X32, w), lc);
+            if (s == SIGN_UNDECIDED) nun++;
+            else if (s * o > 0) nbad += w < M ? (unsigned long long)comp(w_weight, w) : 1ull;
+        }
+    }
+    const unsigned long long sb = wave_sum(nbad), su = wave_sum(nun);
+    if (__lane_id() == 0) {
+        if (sb) atomicAdd(bad, sb);
+        if (su) atomicAdd(unresolved, (int)su);
+    }
+    flush_counters(c.cnt, lc);
+}
+
+// ---------------------------------------------------------------- locate query
+// locate + get_new_simplices (src/delaunay_tree.rs:33-123) for one point q that is not a
+// vertex of the tree, against the current tree, changing nothing: walk from the hint grid (or
+// from record `start` when it is >= 0) to
+// the simplex containing q, then search the conflict region breadth-first.  Signs come from the
+// same cascade as the construction, the exact tier evaluated inline on q's f64 coordinates
+// (no queue: q has no vertex id), in the workspace ws.  One thread.
+//   out_k[0..nk): killed records, out_f[0..nf): boundary facets (t << 2 | face), res[0..2]:
+//   nk, nf, status (0 ok, 1 q on the sphere of its own simplex: a vertex, 2 lists full,
+//   3 outside the super simplex or a walk bound, 4 an exact sign left the f64 range)
+template <int D> struct QueryPoint {
+    typename VT<D>::F f;   // fl32(q - center)
+    double x[3];           // q
+};
+template <int D>
+__device__ int q_sign(const Ctx& c, const typename VT<D>::F* a, const int* ids, int k, const QueryPoint<D>& q, bool sphere,
+                      double* ws, int ws_cap, bool& ovf) {
+    constexpr int S = VT<D>::S;
+    if (c.filters & 1) {
+        int s;
+        if (sphere) {
+            if constexpr (D == 3) s = t1_insphere(a[0], a[1], a[2], a[3], q.f);
+            else s = t1_incircle(a[0], a[1], a[2], q.f);
+        } else {
+            typename VT<D>::F b[4];
+            for (int i = 0; i < D + 1; ++i) b[i] = i == k ? q.f : a[i];
+            if constexpr (D == 3) s = t1_orient3d(b[0], b[1], b[2], b[3]);
+            else s = t1_orient2d(b[0], b[1], b[2]);
+        }
+        if (s != SIGN_UNDECIDED) return s;
+    }
+    const double* P[5];
+    for (int i = 0; i < D + 1; ++i) P[i] = c.X64 + (size_t)S * ids[i];
+    if (!sphere) P[k] = q.x;
+    int s = SIGN_UNDECIDED;
+    if (c.filters & 2) {
+        if (sphere) s = D == 3 ? t2_insphere(P[0], P[1], P[2], P[3], q.x) : t2_incircle(P[0], P[1], P[2], q.x);
+        else s = D == 3 ? t2_orient3d(P[0], P[1], P[2], P[3]) : t2_orient2d(P[0], P[1], P[2]);
+        if (s != SIGN_UNDECIDED) return s;
+    }
+    Ws w{ws, ws_cap, 0, 0};
+    if (sphere) s = D == 3 ? ex_insphere(w, P[0], P[1], P[2], P[3], q.x) : ex_incircle(w, P[0], P[1], P[2], q.x);
+    else s = D == 3 ? ex_orient3d(w, P[0], P[1], P[2], P[3]) : ex_orient2d(w, P[0], P[1], P[2]);
+    if (w.ovf) ovf = true;
+    return s;
+}
+
+template <int D>
+__global__ void k_locate_query(Ctx c, QueryPoint<D> q, int start, double* ws, int ws_cap, int* out_k, int* out_f,
+                               int cap, int* res) {
+    using V = VT<D>;
+    constexpr int M = D + 1;
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    bool ovf = false;
+    int status = 0, nk = 0, nf = 0;
+    int t = start >= 0 ? start : hint_lookup<D>(hint_view(c, 0), q.f);
+    int4 tt = ld_v<D>(c, t);
+    for (int hops = 0; tt.x < 0 && status == 0; ++hops) {
+        if (hops > kChainLimit) status = 3;
+        t = tt.y;
+        tt = ld_v<D>(c, t);
+    }
+    typename V::F a[4];
+    int came = -1;
+    for (int step = 0; status == 0; ++step) {
+        if (step > kWalkLimit) { status = 3; break; }
+        load_tx<D>(c, t, a);
+        const int4 v = ld_v<D>(c, t), aa = ld_a<D>(c, t);
+        const int ids[4] = {v.x, v.y, v.z, v.w};
+        int next = -1;
+        for (int k = 0; k < M && next < 0 && status == 0; ++k) {
+            const int nb = comp(aa, k);
+            if (nb >= 0 && (nb >> 2) == came) continue;
+            const int s = q_sign<D>(c, a, ids, k, q, false, ws, ws_cap, ovf);
+            if (ovf) status = 4;
+            else if (s < 0) {
+                if (nb < 0) status = 3;
+                else next = nb >> 2;
+            }
+        }
+        if (next < 0) break;
+        came = t;
+        t = next;
+    }
+    if (status == 0) {
+        const int4 v = ld_v<D>(c, t);
+        const int ids[4] = {v.x, v.y, v.z, v.w};
+        const int s0 = q_sign<D>(c, a, ids, 0, q, true, ws, ws_cap, ovf);
+        if (ovf) status = 4;
+        else if (s0 <= 0) status = 1;
+    }
+    if (status == 0) {
+        out_k[nk++] = t;
+        for (int i = 0; i < nk && status == 0; ++i) {
+            const int ct = out_k[i];
+            const int4 aa = ld_a<D>(c, ct);
+            for (int f = 0; f < M && status == 0; ++f) {
+                const int nb = comp(aa, f);
+                bool fac = nb < 0;
+                if (!fac) {
+                    const int n = nb >> 2;
+                    bool seen = false;
+                    for (int j = 0; j < nk && !seen; ++j) seen = out_k[j] == n;
+                    if (seen) continue;
+                    typename V::F an[4];
+                    load_tx<D>(c, n, an);
+                    const int4 vn = ld_v<D>(c, n);
+                    const int idn[4] = {vn.x, vn.y, vn.z, vn.w};
+                    const int s = q_sign<D>(c, an, idn, 0, q, true, ws, ws_cap, ovf);
+                    if (ovf) { status = 4; break; }
+                    if (s > 0) {
+                        if (nk == cap) { status = 2; break; }
+                        out_k[nk++] = n;
+                        continue;
+                    }
+                    fac = true;
+                }
+                if (nf == cap) { status = 2; break; }
+                out_f[nf++] = (ct << 2) | f;
+            }
+        }
+    }
+    res[0] = nk;
+    res[1] = nf;
+    res[2] = status;
+}
+
+// rows of a located cavity in the export's numbering: killed records -> export positions;
+// facet j -> (killed position, outside position or -1, the facet's D vertex ids in the
+// reference id space, in the order replace_first gives the new simplex after q)
+template <int D>
+__global__ void k_locate_rows(Ctx c, const int* pos, const int* vmap, const int* in_k, int nk, const int* in_f, int nf,
+                              int32_t* out_k, int32_t* out_f) {
+    constexpr int M = D + 1;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < nk) out_k[i] = pos[in_k[i]];
+    if (i < nf) {
+        const int fe = in_f[i];
+        const int t = fe >> 2, face = fe & 3;
+        const int nb = *a_ptr<D>(c, t, face);
+        const int4 nv = replace_first<D>(ld_v<D>(c, t), face, -1);
+        int32_t* row = out_f + (size_t)i * (M + 1);
+        row[0] = pos[t];
+        row[1] = nb < 0 ? -1 : pos[nb >> 2];
+        for (int k = 1; k < M; ++k) row[1 + k] = vmap[comp(nv, k)];
+    }
+}
+
+// Vertex vi moved: rewrite its fp32 coordinates f in every live record that holds it.
+template <int D>
+__global__ void __launch_bounds__(256) k_patch_vertex(Ctx c, int n, int vi, typename VT<D>::F f) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const int4 v = ld_v<D>(c, t);
+    if (v.x < 0) return;
+    const int k = index_of<D>(v, vi);
+    if (k < 0) return;
+    typename VT<D>::F x[4];
+    load_tx<D>(c, t, x);
+#pragma unroll
+    for (int j = 0; j < D + 1; ++j)
+        if (j == k) x[j] = f;
+    store_tx<D>(c, t, x);
+}
+
+// The live record at export position `target` (pos: exclusive scan of the live flags) swaps
+// its vertex entries i and j and their coordinates; its adjacency is left as it was.
+template <int D>
+__global__ void __launch_bounds__(256) k_swap_in_record(Ctx c, int n, const int* pos, int target, int i, int j) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n || pos[t] != target) return;
+    int4 v = ld_v<D>(c, t);
+    if (v.x < 0) return;
+    typename VT<D>::F x[4], y[4];
+    load_tx<D>(c, t, x);
+    const int vi = comp(v, i), vj = comp(v, j);
+    set_comp(v, i, vj);
+    set_comp(v, j, vi);
+    typename VT<D>::F xi = x[0], xj = x[0];
+#pragma unroll
+    for (int k = 0; k < D + 1; ++k) {
+        if (k == i) xi = x[k];
+        if (k == j) xj = x[k];
+    }
+#pragma unroll
+    for (int k = 0; k < D + 1; ++k) y[k] = k == i ? xj : (k == j ? xi : x[k]);
+    st_v<D>(c, t, v);
+    store_tx<D>(c, t, y);
+}
+
 }  // namespace vo

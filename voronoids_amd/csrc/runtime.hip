@@ -1183,6 +1183,37 @@ int vo_bounding_sphere(int dim, const double* pts, int64_t n, double* center, do
 }
 void vo_free(void* p) { std::free(p); }
 
+int vo_circumsphere(int dim, const double* verts, double* center, double* radius) {
+    if (dim != 2 && dim != 3) return fail(VO_ERR_INVALID, "dim must be 2 or 3");
+    if (!verts || !center || !radius) return fail(VO_ERR_INVALID, "null argument");
+    if (!vo_host::ref_circumsphere(dim, verts, center, radius))
+        return fail(VO_ERR_INVALID, "degenerate simplex: the LU solve has a zero pivot");
+    return 0;
+}
+
+int vo_in_sphere(int dim, const double* p, const double* center, double radius) {
+    if (dim != 2 && dim != 3) return fail(VO_ERR_INVALID, "dim must be 2 or 3");
+    if (!p || !center) return fail(VO_ERR_INVALID, "null argument");
+    return vo_host::ref_in_sphere(dim, p, center, radius) ? 1 : 0;
+}
+
+int vo_circumspheres(int dim, const double* coords, int64_t n_coords, const int32_t* simplices, int64_t n,
+                     double* centers, double* radii) {
+    if (dim != 2 && dim != 3) return fail(VO_ERR_INVALID, "dim must be 2 or 3");
+    if (n < 0 || (n > 0 && (!coords || !simplices || !centers || !radii))) return fail(VO_ERR_INVALID, "null argument");
+    const int M = dim + 1;
+    for (int64_t i = 0; i < n; ++i) {
+        double v[12];
+        for (int k = 0; k < M; ++k) {
+            const int32_t id = simplices[i * M + k];
+            if (id < 0 || id >= n_coords) return fail(VO_ERR_INVALID, "vertex id out of range");
+            for (int j = 0; j < dim; ++j) v[k * dim + j] = coords[(int64_t)id * dim + j];
+        }
+        vo_host::ref_circumsphere(dim, v, centers + i * dim, radii + i);   // degenerate: NaN
+    }
+    return 0;
+}
+
 int vo_tri_create(int hip_device, int dim, const double* pts, int64_t n, vo_tri** out) {
     if (!out) return fail(VO_ERR_INVALID, "out is null");
     *out = nullptr;
@@ -1366,36 +1397,47 @@ int vo_tri_counts(const vo_tri* t, int64_t* n_vertices, int64_t* n_simplices, in
     return 0;
 }
 
-static int export_device(const vo_tri* tc, int32_t** d_simp, int32_t** d_nbr, int64_t* n_live_out) {
-    vo_tri* t = const_cast<vo_tri*>(tc);
+// export position of every record (exclusive scan of the live flags) and the live count
+static int live_positions(vo_tri* t, DevBuf<int>& pos, int64_t* n_live) {
     const int n = t->n_tets;
     const int nb = (n + 255) / 256;
-    int *flag, *pos, *bsum;
+    DevBuf<int> flag, bsum;
     int rc;
-    if ((rc = dev_alloc(&flag, n)) || (rc = dev_alloc(&pos, n)) || (rc = dev_alloc(&bsum, nb))) return rc;
+    if ((rc = dev_alloc(&flag.p, n)) || (rc = dev_alloc(&pos.p, n)) || (rc = dev_alloc(&bsum.p, nb))) return rc;
     const Ctx c = make_ctx(t);
-    if (t->D == 3) hipLaunchKernelGGL(k_live_flags<3>, dim3(nb), dim3(256), 0, t->s, c, n, flag);
-    else hipLaunchKernelGGL(k_live_flags<2>, dim3(nb), dim3(256), 0, t->s, c, n, flag);
-    hipLaunchKernelGGL(k_block_scan, dim3(nb), dim3(256), 0, t->s, flag, pos, bsum, n);
+    if (t->D == 3) hipLaunchKernelGGL(k_live_flags<3>, dim3(nb), dim3(256), 0, t->s, c, n, flag.p);
+    else hipLaunchKernelGGL(k_live_flags<2>, dim3(nb), dim3(256), 0, t->s, c, n, flag.p);
+    hipLaunchKernelGGL(k_block_scan, dim3(nb), dim3(256), 0, t->s, flag.p, pos.p, bsum.p, n);
     std::vector<int> hb(nb);
-    VO_HIP(hipMemcpyAsync(hb.data(), bsum, sizeof(int) * nb, hipMemcpyDeviceToHost, t->s));
+    VO_HIP(hipMemcpyAsync(hb.data(), bsum.p, sizeof(int) * nb, hipMemcpyDeviceToHost, t->s));
     VO_HIP(hipStreamSynchronize(t->s));
     int64_t acc = 0;
     for (int i = 0; i < nb; ++i) { int v = hb[i]; hb[i] = (int)acc; acc += v; }
-    VO_HIP(hipMemcpyAsync(bsum, hb.data(), sizeof(int) * nb, hipMemcpyHostToDevice, t->s));
-    hipLaunchKernelGGL(k_add_offsets, dim3(nb), dim3(256), 0, t->s, pos, bsum, n);
-    int32_t *simp = nullptr, *nbr = nullptr;
+    VO_HIP(hipMemcpyAsync(bsum.p, hb.data(), sizeof(int) * nb, hipMemcpyHostToDevice, t->s));
+    hipLaunchKernelGGL(k_add_offsets, dim3(nb), dim3(256), 0, t->s, pos.p, bsum.p, n);
+    VO_HIP(hipGetLastError());
+    VO_HIP(hipStreamSynchronize(t->s));     // before bsum and flag are freed
+    *n_live = acc;
+    return 0;
+}
+
+static int export_device(const vo_tri* tc, DevBuf<int32_t>& simp, DevBuf<int32_t>& nbr, int64_t* n_live_out) {
+    vo_tri* t = const_cast<vo_tri*>(tc);
+    const int n = t->n_tets;
+    const int nb = (n + 255) / 256;
+    DevBuf<int> pos;
+    int64_t acc = 0;
+    int rc;
+    if ((rc = live_positions(t, pos, &acc))) return rc;
     const int M = t->M;
-    if ((rc = dev_alloc(&simp, acc * M)) || (rc = dev_alloc(&nbr, acc * M))) return rc;
+    if ((rc = dev_alloc(&simp.p, acc * M)) || (rc = dev_alloc(&nbr.p, acc * M))) return rc;
+    const Ctx c = make_ctx(t);
     if (t->D == 3)
-        hipLaunchKernelGGL(k_export<3>, dim3(nb), dim3(256), 0, t->s, c, pos, n, t->vmap, simp, nbr);
+        hipLaunchKernelGGL(k_export<3>, dim3(nb), dim3(256), 0, t->s, c, pos.p, n, t->vmap, simp.p, nbr.p);
     else
-        hipLaunchKernelGGL(k_export<2>, dim3(nb), dim3(256), 0, t->s, c, pos, n, t->vmap, simp, nbr);
+        hipLaunchKernelGGL(k_export<2>, dim3(nb), dim3(256), 0, t->s, c, pos.p, n, t->vmap, simp.p, nbr.p);
     VO_HIP(hipGetLastError());
     VO_HIP(hipStreamSynchronize(t->s));
-    hipFree(flag); hipFree(pos); hipFree(bsum);
-    *d_simp = simp;
-    *d_nbr = nbr;
     *n_live_out = acc;
     return 0;
 }
@@ -1403,15 +1445,13 @@ static int export_device(const vo_tri* tc, int32_t** d_simp, int32_t** d_nbr, in
 int vo_tri_export(const vo_tri* t, int32_t* simplices, int32_t* neighbors) {
     if (!t) return fail(VO_ERR_INVALID, "null handle");
     VO_HIP(hipSetDevice(t->dev));
-    int32_t *ds, *dn;
+    DevBuf<int32_t> ds, dn;
     int64_t nl;
-    int rc = export_device(t, &ds, &dn, &nl);
+    int rc = export_device(t, ds, dn, &nl);
     if (rc) return rc;
     const size_t bytes = sizeof(int32_t) * (size_t)nl * t->M;
-    if (simplices) VO_HIP(hipMemcpy(simplices, ds, bytes, hipMemcpyDeviceToHost));
-    if (neighbors) VO_HIP(hipMemcpy(neighbors, dn, bytes, hipMemcpyDeviceToHost));
-    hipFree(ds);
-    hipFree(dn);
+    if (simplices && nl) VO_HIP(hipMemcpy(simplices, ds.p, bytes, hipMemcpyDeviceToHost));
+    if (neighbors && nl) VO_HIP(hipMemcpy(neighbors, dn.p, bytes, hipMemcpyDeviceToHost));
     return 0;
 }
 
@@ -1423,7 +1463,7 @@ static int canonical_device(const vo_tri* t, int32_t** T, int64_t* n_T, int32_t*
     const hipStream_t s = t->s;
     DevBuf<int32_t> ds, dn;
     int64_t nl = 0;
-    int rc = export_device(t, &ds.p, &dn.p, &nl);
+    int rc = export_device(t, ds, dn, &nl);
     if (rc) return rc;
     hipFree(dn.p);
     dn.p = nullptr;
@@ -1511,38 +1551,195 @@ int vo_tri_canonical(const vo_tri* t, int32_t** T, int64_t* n_T, int32_t** E, in
     return canonical_device(t, T, n_T, E, n_E);
 }
 
-int vo_tri_check(vo_tri* t, int64_t* n_violations) {
-    if (!t || !n_violations) return fail(VO_ERR_INVALID, "null argument");
-    VO_HIP(hipSetDevice(t->dev));
-    unsigned long long* bad;
-    int* unres;
+}  // extern "C"
+
+namespace {
+
+// Violation counts of the two checks, with the exact tier's retry: a pass whose predicates
+// needed the expansions queues them; they are evaluated into the cache and the pass is run
+// again (each pass resolves what the last one queued).  pairs = 0: the local check (k_check),
+// 1: the reference's all-pairs check (k_check_pairs).
+int run_check(vo_tri* t, int pairs, int64_t* out) {
+    DevBuf<unsigned long long> bad;
+    DevBuf<int> unres;
     int rc;
-    if ((rc = dev_alloc(&bad, 1)) || (rc = dev_alloc(&unres, 1))) return rc;
+    if ((rc = dev_alloc(&bad.p, 1)) || (rc = dev_alloc(&unres.p, 1))) return rc;
     const int n = t->n_tets;
+    const int nv = (int)(t->M + t->n_pts);
+    // super vertex w stands for itself and its ghost copies (3D: vertex 0 has ghosts 4, 5, 6
+    // and vertex 1 has ghost 7; 2D: vertices 0, 1, 2 have ghosts 3, 4, 5)
+    const int4 weight = t->D == 3 ? make_int4(4, 2, 1, 1) : make_int4(2, 2, 2, 0);
     for (int attempt = 0; attempt < 16; ++attempt) {
-        VO_HIP(hipMemsetAsync(bad, 0, sizeof(unsigned long long), t->s));
-        VO_HIP(hipMemsetAsync(unres, 0, sizeof(int), t->s));
+        VO_HIP(hipMemsetAsync(bad.p, 0, sizeof(unsigned long long), t->s));
+        VO_HIP(hipMemsetAsync(unres.p, 0, sizeof(int), t->s));
         hipLaunchKernelGGL(k_reset_round, dim3(1), dim3(1), 0, t->s, t->st);
         Ctx c = make_ctx(t);
-        if (t->D == 3) hipLaunchKernelGGL(k_check<3>, dim3((n + 255) / 256), dim3(256), 0, t->s, c, n, bad, unres);
-        else hipLaunchKernelGGL(k_check<2>, dim3((n + 255) / 256), dim3(256), 0, t->s, c, n, bad, unres);
+        const unsigned grid = (unsigned)std::max(1, std::min(n, 8192));
+        if (pairs && t->D == 3)
+            hipLaunchKernelGGL(k_check_pairs<3>, dim3(grid), dim3(256), 0, t->s, c, n, nv, weight, bad.p, unres.p);
+        else if (pairs)
+            hipLaunchKernelGGL(k_check_pairs<2>, dim3(grid), dim3(256), 0, t->s, c, n, nv, weight, bad.p, unres.p);
+        else if (t->D == 3)
+            hipLaunchKernelGGL(k_check<3>, dim3((n + 255) / 256), dim3(256), 0, t->s, c, n, bad.p, unres.p);
+        else
+            hipLaunchKernelGGL(k_check<2>, dim3((n + 255) / 256), dim3(256), 0, t->s, c, n, bad.p, unres.p);
         VO_HIP(hipGetLastError());
         unsigned long long hb = 0;
         int hu = 0;
-        VO_HIP(hipMemcpyAsync(&hb, bad, sizeof(hb), hipMemcpyDeviceToHost, t->s));
-        VO_HIP(hipMemcpyAsync(&hu, unres, sizeof(hu), hipMemcpyDeviceToHost, t->s));
+        VO_HIP(hipMemcpyAsync(&hb, bad.p, sizeof(hb), hipMemcpyDeviceToHost, t->s));
+        VO_HIP(hipMemcpyAsync(&hu, unres.p, sizeof(hu), hipMemcpyDeviceToHost, t->s));
         if ((rc = sync_state(t))) return rc;
         if (hu == 0) {
-            hipFree(bad);
-            hipFree(unres);
-            *n_violations = (int64_t)hb;
+            *out = (int64_t)hb;
             return check_err(t);
         }
         if ((rc = run_exact(t, t->st_host->n_q))) return rc;
     }
-    hipFree(bad);
-    hipFree(unres);
     return fail(VO_ERR_INTERNAL, "check: predicates stayed unresolved");
+}
+
+// internal vertex index of reference vertex id `id` (an inserted or staged input point), -1
+int vertex_of_id(const vo_tri* t, int64_t id) {
+    const int64_t in = id - 2 * t->M;
+    if (in < 0) return -1;
+    for (size_t i = 0; i < t->input_of.size(); ++i)
+        if (t->input_of[i] == in) return (int)(t->M + i);
+    return -1;
+}
+
+}  // namespace
+
+extern "C" {
+
+int vo_tri_check(vo_tri* t, int64_t* n_violations) {
+    if (!t || !n_violations) return fail(VO_ERR_INVALID, "null argument");
+    VO_HIP(hipSetDevice(t->dev));
+    return run_check(t, 0, n_violations);
+}
+
+int vo_tri_check_pairs(vo_tri* t, int64_t* n_pairs) {
+    if (!t || !n_pairs) return fail(VO_ERR_INVALID, "null argument");
+    const double work = (double)t->n_live * (double)(t->M + t->n_pts);
+    if (work > (double)((int64_t)1 << 36))
+        return fail(VO_ERR_INVALID, "vo_tri_check_pairs is O(simplices x vertices), limited to 2^36 pairs; "
+                                    "use vo_tri_check for large trees");
+    VO_HIP(hipSetDevice(t->dev));
+    return run_check(t, 1, n_pairs);
+}
+
+int vo_tri_locate(vo_tri* t, const double* p, int32_t* killed, int64_t killed_cap, int64_t* n_killed,
+                  int32_t* facets, int64_t facets_cap, int64_t* n_facets) {
+    if (!t || !p || !n_killed || !n_facets) return fail(VO_ERR_INVALID, "null argument");
+    if (killed_cap < 0 || facets_cap < 0 || (killed_cap > 0 && !killed) || (facets_cap > 0 && !facets))
+        return fail(VO_ERR_INVALID, "bad output buffer");
+    const int D = t->D, M = t->M;
+    for (int k = 0; k < D; ++k)
+        if (!std::isfinite(p[k])) return fail(VO_ERR_INVALID, "non-finite coordinate");
+    VO_HIP(hipSetDevice(t->dev));
+    const Ctx c = make_ctx(t);
+    const int start = t->hint ? -1 : 0;          // no hint grid before the first stage: the first record
+    float f[3] = {0, 0, 0};
+    double x[3] = {0, 0, 0};
+    for (int k = 0; k < D; ++k) {
+        x[k] = p[k];
+        f[k] = (float)(p[k] - t->c[k]);
+    }
+    if (D == 3) {
+        QueryPoint<3> q{make_float4(f[0], f[1], f[2], 0.f), {x[0], x[1], x[2]}};
+        hipLaunchKernelGGL(k_locate_query<3>, dim3(1), dim3(64), 0, t->s, c, q, start, t->ws, t->ws_cap, t->cav1, t->fac1,
+                           t->big_cap, t->misc1);
+    } else {
+        QueryPoint<2> q{make_float2(f[0], f[1]), {x[0], x[1], 0.0}};
+        hipLaunchKernelGGL(k_locate_query<2>, dim3(1), dim3(64), 0, t->s, c, q, start, t->ws, t->ws_cap, t->cav1, t->fac1,
+                           t->big_cap, t->misc1);
+    }
+    VO_HIP(hipGetLastError());
+    int res[3] = {0, 0, 0};
+    VO_HIP(hipMemcpyAsync(res, t->misc1, sizeof(res), hipMemcpyDeviceToHost, t->s));
+    VO_HIP(hipStreamSynchronize(t->s));
+    if (res[2] == 1)
+        return fail(VO_ERR_DUPLICATE, "the point coincides with a vertex: no simplex has it strictly inside its sphere");
+    if (res[2] == 2) return fail(VO_ERR_OVERFLOW, "the conflict region exceeds the query lists");
+    if (res[2] == 3) return fail(VO_ERR_INVALID, "the point lies outside the super simplex");
+    if (res[2] == 4) return fail(VO_ERR_PRECISION, "an exact sign left the f64 range or the workspace");
+    const int nk = res[0], nf = res[1];
+    *n_killed = nk;
+    *n_facets = nf;
+    if (nk > killed_cap || nf > facets_cap)
+        return fail(VO_ERR_OVERFLOW, "output buffers too small: n_killed / n_facets hold the sizes needed");
+    DevBuf<int> pos;
+    DevBuf<int32_t> dk, df;
+    int64_t nl = 0;
+    int rc;
+    if ((rc = live_positions(t, pos, &nl)) || (rc = dev_alloc(&dk.p, (size_t)nk)) ||
+        (rc = dev_alloc(&df.p, (size_t)nf * (M + 1))))
+        return rc;
+    const unsigned grid = (unsigned)((std::max(nk, nf) + 255) / 256);
+    if (D == 3)
+        hipLaunchKernelGGL(k_locate_rows<3>, dim3(grid), dim3(256), 0, t->s, c, pos.p, t->vmap, t->cav1, nk, t->fac1, nf,
+                           dk.p, df.p);
+    else
+        hipLaunchKernelGGL(k_locate_rows<2>, dim3(grid), dim3(256), 0, t->s, c, pos.p, t->vmap, t->cav1, nk, t->fac1, nf,
+                           dk.p, df.p);
+    VO_HIP(hipGetLastError());
+    VO_HIP(hipMemcpyAsync(killed, dk.p, sizeof(int32_t) * (size_t)nk, hipMemcpyDeviceToHost, t->s));
+    VO_HIP(hipMemcpyAsync(facets, df.p, sizeof(int32_t) * (size_t)nf * (M + 1), hipMemcpyDeviceToHost, t->s));
+    VO_HIP(hipStreamSynchronize(t->s));
+    std::sort(killed, killed + nk);           // the reference sorts and dedups (src/delaunay_tree.rs:55-56)
+    return 0;
+}
+
+int vo_tri_set_vertex(vo_tri* t, int64_t id, const double* coords, int records) {
+    if (!t || !coords) return fail(VO_ERR_INVALID, "null argument");
+    const int D = t->D, S = t->S;
+    for (int k = 0; k < D; ++k)
+        if (!std::isfinite(coords[k])) return fail(VO_ERR_INVALID, "non-finite coordinate");
+    const int vi = vertex_of_id(t, id);
+    if (vi < 0) return fail(VO_ERR_INVALID, "no input point has vertex id " + std::to_string(id));
+    VO_HIP(hipSetDevice(t->dev));
+    double x64[4] = {0, 0, 0, 0};
+    float x32[4] = {0, 0, 0, 0};
+    for (int k = 0; k < D; ++k) {
+        x64[k] = coords[k];
+        x32[k] = (float)(coords[k] - t->c[k]);     // as staged: the f64 difference rounded once
+    }
+    VO_HIP(hipMemcpyAsync(t->X64 + (size_t)vi * S, x64, sizeof(double) * S, hipMemcpyHostToDevice, t->s));
+    VO_HIP(hipMemcpyAsync(t->X32 + (size_t)vi * S, x32, sizeof(float) * S, hipMemcpyHostToDevice, t->s));
+    if (records && t->n_tets > 0) {
+        const Ctx c = make_ctx(t);
+        const unsigned grid = (unsigned)((t->n_tets + 255) / 256);
+        if (D == 3)
+            hipLaunchKernelGGL(k_patch_vertex<3>, dim3(grid), dim3(256), 0, t->s, c, t->n_tets, vi,
+                               make_float4(x32[0], x32[1], x32[2], 0.f));
+        else
+            hipLaunchKernelGGL(k_patch_vertex<2>, dim3(grid), dim3(256), 0, t->s, c, t->n_tets, vi,
+                               make_float2(x32[0], x32[1]));
+        VO_HIP(hipGetLastError());
+    }
+    // exact signs cached by vertex ids no longer hold for the moved vertex
+    VO_HIP(hipMemsetAsync(t->cache, 0xFF, sizeof(CEntry) * ((size_t)t->cmask + 1), t->s));
+    VO_HIP(hipStreamSynchronize(t->s));
+    return 0;
+}
+
+int vo_tri_swap_simplex_vertices(vo_tri* t, int64_t simplex, int i, int j) {
+    if (!t) return fail(VO_ERR_INVALID, "null handle");
+    if (i < 0 || j < 0 || i > t->D || j > t->D || i == j) return fail(VO_ERR_INVALID, "i, j must be distinct in 0..dim");
+    if (simplex < 0 || simplex >= t->n_live) return fail(VO_ERR_INVALID, "no live simplex at that export index");
+    VO_HIP(hipSetDevice(t->dev));
+    DevBuf<int> pos;
+    int64_t nl = 0;
+    int rc;
+    if ((rc = live_positions(t, pos, &nl))) return rc;
+    const Ctx c = make_ctx(t);
+    const unsigned grid = (unsigned)((t->n_tets + 255) / 256);
+    if (t->D == 3)
+        hipLaunchKernelGGL(k_swap_in_record<3>, dim3(grid), dim3(256), 0, t->s, c, t->n_tets, pos.p, (int)simplex, i, j);
+    else
+        hipLaunchKernelGGL(k_swap_in_record<2>, dim3(grid), dim3(256), 0, t->s, c, t->n_tets, pos.p, (int)simplex, i, j);
+    VO_HIP(hipGetLastError());
+    VO_HIP(hipStreamSynchronize(t->s));
+    return 0;
 }
 
 int vo_tri_stats(const vo_tri* t, vo_stats* out) {

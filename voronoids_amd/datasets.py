@@ -68,6 +68,16 @@ def lattice_jitter(side: int = 128, seed: int = 0, jitter: float = 1e-9) -> np.n
     return np.ascontiguousarray(base + j.reshape(base.shape))
 
 
+def lattice_block(block: int, side: int = 128, seed: int = 0, jitter: float = 1e-9) -> np.ndarray:
+    """3D: the block^3 corner of the side^3 lattice, (i/side, j/side, k/side) for i, j, k < block,
+    + i.i.d. uniform jitter in [-jitter, jitter) per coordinate (the lattice's own spacing)"""
+    g = np.arange(block, dtype=np.float64) / side
+    x, y, z = np.meshgrid(g, g, g, indexing="ij")
+    base = np.stack([x.ravel(), y.ravel(), z.ravel()], axis=1)
+    j = (2.0 * unit(seed, 16, 0, base.size) - 1.0) * jitter
+    return np.ascontiguousarray(base + j.reshape(base.shape))
+
+
 def batch_sizes(n_sets: int = 4096, lo: int = 64, hi: int = 65536, seed: int = 0) -> np.ndarray:
     """set sizes floor(exp(U[ln lo, ln hi]))"""
     u = unit(seed, 17, 0, n_sets)
