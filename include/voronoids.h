@@ -164,6 +164,15 @@ int vo_tri_config(vo_tri* t, const char* key, int64_t value);
 int vo_predicates(int hip_device, int type, const double* pts, int64_t n, const double* center, int32_t* signs,
                   int32_t* tiers);
 
+/* vo_predicates without any device allocation: no counterpart in the reference; for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  Same arguments, signs and tiers as vo_predicates, but n is at most 65,536 per
+ * call (VO_ERR_INVALID beyond): the instances, signs, tiers and expansion workspaces are staged in
+ * one static device array of about 16.5 MiB that each device holds from the first call until the
+ * process ends.  Calls that overlap on one device share that array and race: serialize them. */
+int vo_predicates_noalloc(int hip_device, int type, const double* pts, int64_t n, const double* center,
+                          int32_t* signs, int32_t* tiers);
+
 /* input vertex coordinates in the reference id space (for PyVertex.point): (2(dim+1) + n) x dim */
 int vo_tri_vertices(const vo_tri* t, double* coords);
 

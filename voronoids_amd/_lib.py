@@ -30,7 +30,7 @@ EXPORTED = [
     "vo_server_create", "vo_server_submit", "vo_server_results", "vo_server_stats_get", "vo_server_destroy",
     "vo_predicates", "vo_bounding_sphere",
     "vo_tri_check_pairs", "vo_tri_set_vertex", "vo_tri_swap_simplex_vertices",
-    "vo_circumsphere", "vo_circumspheres", "vo_in_sphere", "vo_tri_locate",
+    "vo_circumsphere", "vo_circumspheres", "vo_in_sphere", "vo_predicates_noalloc", "vo_tri_locate",
 ]
 
 
@@ -142,6 +142,7 @@ def lib() -> ctypes.CDLL:
         L.vo_server_destroy.argtypes = [P]
         L.vo_server_destroy.restype = None
         L.vo_predicates.argtypes = [ctypes.c_int, ctypes.c_int, dp, ctypes.c_int64, dp, i32p, i32p]
+        L.vo_predicates_noalloc.argtypes = [ctypes.c_int, ctypes.c_int, dp, ctypes.c_int64, dp, i32p, i32p]
         L.vo_last_error.restype = ctypes.c_char_p
         L.vo_version.restype = ctypes.c_int
         _lib = L

@@ -15,6 +15,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -1128,6 +1129,46 @@ int bit_length(int64_t x) {
 
 }  // namespace
 
+// ------------------------------------------------------------------ allocation-free predicates
+// vo_predicates_noalloc stages its instances, signs, tiers and expansion workspaces in one
+// file-scope device array instead of hipMalloc'd buffers.  Each device the library runs on holds
+// its own copy (about 16.5 MiB) until the process ends.  Layout (offsets 256 B aligned):
+//   [0, kNaPts)                 instance coordinates, up to kNaMaxN x 5 x 3 f64
+//   [kNaPts, + kNaWs)           expansion workspaces, kNaThreads x kNaWsCap f64 (as vo_predicates)
+//   [.., + kNaMaxN x 4 B)       signs, then tiers
+constexpr int64_t kNaMaxN = 1 << 16;
+constexpr int kNaBlocks = 4, kNaThreadsPerBlock = 64, kNaThreads = kNaBlocks * kNaThreadsPerBlock;
+constexpr int kNaWsCap = 1 << 12;
+constexpr size_t kNaPts = (size_t)kNaMaxN * 5 * 3 * sizeof(double);
+constexpr size_t kNaWs = (size_t)kNaThreads * kNaWsCap * sizeof(double);
+constexpr size_t kNaOut = (size_t)kNaMaxN * sizeof(int32_t);
+constexpr size_t kNaBytes = kNaPts + kNaWs + 2 * kNaOut;
+static_assert(kNaPts % 256 == 0 && kNaWs % 256 == 0 && kNaOut % 256 == 0, "256 B aligned parts");
+
+static __device__ __align__(256) unsigned char scratch[kNaBytes];
+
+namespace {
+// the device address of `scratch` on each device, looked up once with that device current (in
+// host code the array's name is not a device address)
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+unsigned char* g_scratch[kMaxDevices] = {};
+
+int scratch_on(int dev, unsigned char** out) {
+    if (dev < 0 || dev >= kMaxDevices) return fail(VO_ERR_INVALID, "device index beyond the scratch table");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch[dev]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return fail(VO_ERR_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+        g_scratch[dev] = static_cast<unsigned char*>(p);
+    }
+    *out = g_scratch[dev];
+    return 0;
+}
+}  // namespace
+
 // =================================================================== C ABI
 extern "C" {
 
@@ -1169,6 +1210,44 @@ int vo_predicates(int hip_device, int type, const double* pts, int64_t n, const 
     if (e == hipSuccess) e = hipMemcpy(tiers, dt, sizeof(int32_t) * (size_t)n, hipMemcpyDeviceToHost);
     release();
     if (e != hipSuccess) return fail(VO_ERR_HIP, std::string("vo_predicates: ") + hipGetErrorString(e));
+    return 0;
+}
+
+int vo_predicates_noalloc(int hip_device, int type, const double* pts, int64_t n, const double* center,
+                          int32_t* signs, int32_t* tiers) {
+    if (type < 1 || type > 4) return fail(VO_ERR_INVALID, "type must be 1 (orient2d), 2 (incircle), 3 (orient3d) or 4 (insphere)");
+    if (n < 0 || (n > 0 && (!pts || !center || !signs || !tiers))) return fail(VO_ERR_INVALID, "null argument");
+    if (n > kNaMaxN)
+        return fail(VO_ERR_INVALID, "vo_predicates_noalloc: " + std::to_string(n) + " instances exceed the maximum of " +
+                                        std::to_string(kNaMaxN) + " per call (split the batch, or use vo_predicates)");
+    if (n == 0) return 0;
+    const int D = type <= 2 ? 2 : 3;
+    const int A = pred_arity((uint32_t)type);
+    for (int64_t i = 0; i < n * A * D; ++i)
+        if (!std::isfinite(pts[i])) return fail(VO_ERR_INVALID, "non-finite coordinate");
+    int ndev = 0;
+    VO_HIP(hipGetDeviceCount(&ndev));
+    if (hip_device < 0 || hip_device >= ndev) return fail(VO_ERR_INVALID, "no such HIP device");
+    VO_HIP(hipSetDevice(hip_device));
+    unsigned char* base = nullptr;
+    int rc;
+    if ((rc = scratch_on(hip_device, &base))) return rc;
+    double* dp = reinterpret_cast<double*>(base);
+    double* ws = reinterpret_cast<double*>(base + kNaPts);
+    int32_t* ds = reinterpret_cast<int32_t*>(base + kNaPts + kNaWs);
+    int32_t* dt = reinterpret_cast<int32_t*>(base + kNaPts + kNaWs + kNaOut);
+    // every sign, tier and workspace entry a call reads is written by that call first: nothing
+    // left by an earlier call needs clearing
+    VO_HIP(hipMemcpy(dp, pts, sizeof(double) * (size_t)n * A * D, hipMemcpyHostToDevice));
+    if (D == 3)
+        hipLaunchKernelGGL(k_eval_predicates<3>, dim3(kNaBlocks), dim3(kNaThreadsPerBlock), 0, 0, type, dp, (int)n,
+                           center[0], center[1], center[2], ws, kNaWsCap, ds, dt);
+    else
+        hipLaunchKernelGGL(k_eval_predicates<2>, dim3(kNaBlocks), dim3(kNaThreadsPerBlock), 0, 0, type, dp, (int)n,
+                           center[0], center[1], 0.0, ws, kNaWsCap, ds, dt);
+    VO_HIP(hipGetLastError());
+    VO_HIP(hipMemcpy(signs, ds, sizeof(int32_t) * (size_t)n, hipMemcpyDeviceToHost));
+    VO_HIP(hipMemcpy(tiers, dt, sizeof(int32_t) * (size_t)n, hipMemcpyDeviceToHost));
     return 0;
 }
 
