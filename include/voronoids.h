@@ -52,6 +52,9 @@ typedef struct {
      * and their device time, ms (included in ms_run) */
     int64_t compactions;
     double ms_compact;
+    /* simplex records in use, live and dead: with reuse (the default) about the live count, since
+     * new simplices take their winner's killed records (vo_tri_config "reuse") */
+    int64_t records;
 } vo_stats;
 
 /* DelaunayTree::<3,4>::new / ::<2,3>::new (src/delaunay_tree.rs:389-510, :544-640):

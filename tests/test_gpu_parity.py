@@ -186,6 +186,7 @@ def test_compaction_keeps_the_triangulation(dim):
     P = datasets.uniform(30000, dim, seed=110 + dim)
     Q = datasets.uniform(5000, dim, seed=120 + dim) * 0.5 + 0.25
     t = vo.DelaunayTree(np.vstack([P, Q]))
+    t.config("reuse", 0)                      # append-only records: compaction has dead ones to drop
     t.config("compact_min", 256)
     t.config("compact_ratio_milli", 1500)
     t.add_points_to_tree(P, seed=5)
@@ -201,12 +202,35 @@ def test_compaction_keeps_the_triangulation(dim):
 
 
 @pytest.mark.parametrize("dim", [2, 3])
+def test_record_reuse_keeps_records_near_the_live_count(dim):
+    # new simplices take their winner's killed records: records stay close to the live simplices
+    # (append-only: about 4x (3D) / 3x (2D) as many), with the same triangulation either way
+    P = datasets.uniform(100_000, dim, seed=130 + dim)
+    out = {}
+    for reuse in (1, 0):
+        t = vo.DelaunayTree(P)
+        t.config("reuse", reuse)
+        t.add_points_to_tree(P, seed=7)
+        st = t.stats()
+        out[reuse] = (t.canonical(), st["records"], st["created"], t.counts())
+        assert t.violations() == 0
+    (T1, E1), rec1, cr1, (nv1, ns1, mx1) = out[1]
+    (T0, E0), rec0, cr0, (nv0, ns0, mx0) = out[0]
+    assert np.array_equal(T1, T0) and np.array_equal(E1, E0) and ns1 == ns0
+    assert rec1 <= 1.25 * ns1 + 64 and rec0 >= 2.5 * ns0
+    assert mx1 == cr1 + dim + 1 and mx0 == cr0 + dim + 1
+    ref = O.Tree.delaunay(P, O.MODE_EXACT).canonical()
+    assert np.array_equal(T1, ref.T) and np.array_equal(E1, ref.E)
+
+
+@pytest.mark.parametrize("dim", [2, 3])
 def test_xcd_region_windows_keep_the_triangulation(dim):
     # windows sorted into 8 spatial regions, block b serving region b % 8, with frequent
     # compactions: same triangulation as the oracle's
     P = datasets.uniform(60000, dim, seed=130 + dim)
     t = vo.DelaunayTree(P)
     t.config("xcd_regions", 1)
+    t.config("reuse", 0)                      # append-only records: compaction has dead ones to drop
     t.config("compact_ratio_milli", 1500)
     t.config("compact_min", 256)
     t.add_points_to_tree(P, seed=3)

@@ -68,7 +68,7 @@ class VoStats(ctypes.Structure):
         ("ms_exact", ctypes.c_double), ("ms_other", ctypes.c_double),
         ("launches_locate_cavity", ctypes.c_int64), ("launches_commit", ctypes.c_int64),
         ("solo_runs", ctypes.c_int64), ("duplicates_detected", ctypes.c_int64),
-        ("compactions", ctypes.c_int64), ("ms_compact", ctypes.c_double),
+        ("compactions", ctypes.c_int64), ("ms_compact", ctypes.c_double), ("records", ctypes.c_int64),
     ]
 
     def as_dict(self) -> dict:

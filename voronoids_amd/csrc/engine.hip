@@ -133,6 +133,7 @@ __global__ void __launch_bounds__(kPlanThreads) k_plan(BatchCtl* ctl, SetRec* se
         st->n_next_pending = npend - take;
         st->n_winners = 0;
         st->n_q = 0;
+        st->n_fix = 0;
         st->n_live_delta = 0;
         ctl->tail = npend - take;
         s_npend = take;
@@ -474,7 +475,7 @@ struct vo_engine {
     int ws_threads = 1024, ws_cap = 1 << 14;
     // device arrays
     Arena a{};
-    int *win = nullptr, *pend[2] = {nullptr, nullptr}, *cav = nullptr, *fac = nullptr, *nk = nullptr, *nf = nullptr,
+    int *win = nullptr, *pend[2] = {nullptr, nullptr}, *cav = nullptr, *fac = nullptr, *fnb = nullptr, *nk = nullptr, *nf = nullptr,
         *status = nullptr, *big = nullptr, *cav1 = nullptr, *fac1 = nullptr, *misc1 = nullptr;
     RoundState* st = nullptr;
     RoundState* st_host = nullptr;
@@ -511,6 +512,9 @@ struct vo_engine {
     int tpp_milli = 0;
     int filters = 3;
     int abort_early = 1;               // as vo_tri_config "abort_early"
+    int reuse = 0;                     // as vo_tri_config "reuse" (0 or 1 here)
+    int2* fix = nullptr;               // shared facets of a round's winners (k_fix_shared)
+    int fix_cap = 0;
     float gamma0 = 0.2f;
     bool launched = false;
     int last_nwin = 0;
@@ -617,8 +621,9 @@ void parallel_for(vo_engine* e, int n, const std::function<void(int)>& fn) {
 void engine_free(vo_engine* e) {
     stop_workers(e);
     void* ptrs[] = {e->a.X64, e->a.X32, e->a.cp, e->a.vmap, e->a.pset, e->a.rec, e->a.hint, e->a.sets,
-                    e->win, e->pend[0], e->pend[1], e->cav, e->fac, e->nk, e->nf, e->status, e->big, e->cav1,
-                    e->fac1, e->misc1, e->st, e->ctl, e->cnt, e->q, e->cache, e->ws, e->done, e->stage_d, e->out_d};
+                    e->win, e->pend[0], e->pend[1], e->cav, e->fac, e->fnb, e->nk, e->nf, e->status, e->big, e->cav1,
+                    e->fac1, e->misc1, e->st, e->ctl, e->cnt, e->q, e->cache, e->ws, e->done, e->stage_d, e->out_d,
+                    e->fix};
     for (void* p : ptrs)
         if (p) hipFree(p);
     if (e->st_host) hipHostFree(e->st_host);
@@ -645,6 +650,7 @@ Ctx engine_ctx(vo_engine* e) {
     c.big = e->big;
     c.cav = e->cav;
     c.fac = e->fac;
+    c.fnb = e->fnb;
     c.nk = e->nk;
     c.nf = e->nf;
     c.status = e->status;
@@ -660,6 +666,9 @@ Ctx engine_ctx(vo_engine* e) {
     c.filters = e->filters;
     c.solo = 0;
     c.abort_early = e->abort_early;
+    c.reuse = e->reuse;
+    c.fix = e->fix;
+    c.fix_cap = e->fix_cap;
     c.M = e->M;
     c.win_cap = e->win_cap;
     c.kcap = e->kcap;
@@ -711,6 +720,17 @@ int gindex(int G) { return G == 1 ? 0 : G == 4 ? 1 : G == 8 ? 2 : G == 16 ? 3 : 
 
 void launch_round_kernels(vo_engine* e, const Ctx& c, int G, int which) {
     const int tblocks = std::max(1, std::min(2048, (e->win_cap + 255) / 256));
+    // after a lane-group commit without reuse: links across the facets two winners share
+    struct FixAfter {
+        vo_engine* e;
+        const Ctx& c;
+        int G, which, blocks;
+        ~FixAfter() {
+            if (!which || G == 1 || c.reuse || !c.fix) return;
+            if (e->D == 3) hipLaunchKernelGGL(k_fix_shared<3>, dim3(blocks), dim3(256), 0, e->s, c);
+            else hipLaunchKernelGGL(k_fix_shared<2>, dim3(blocks), dim3(256), 0, e->s, c);
+        }
+    } fix_after{e, c, G, which, std::max(1, std::min(1024, e->win_cap / 64 + 1))};
     const int gblocks = std::max(1, std::min(8192, (e->win_cap * G + 63) / 64));
     if (G == 1) {
         if (e->D == 3) {
@@ -792,7 +812,7 @@ void drop_graphs(vo_engine* e) {
 }
 
 int64_t first_range_len(const vo_engine* e, int64_t n) {
-    const int64_t milli = e->tpp_milli ? e->tpp_milli : (e->D == 3 ? 30000 : 7000);
+    const int64_t milli = e->tpp_milli ? e->tpp_milli : e->reuse ? (e->D == 3 ? 12000 : 4000) : (e->D == 3 ? 30000 : 7000);
     return n * milli / 1000 + 64;
 }
 
@@ -1213,14 +1233,16 @@ int vo_engine_create(int hip_device, int dim, int64_t max_points, int64_t max_si
         (rc = dev_alloc(&E->win, E->win_cap)) || (rc = dev_alloc(&E->pend[0], E->win_cap + E->big_cap)) ||
         (rc = dev_alloc(&E->pend[1], E->win_cap + E->big_cap)) ||
         (rc = dev_alloc(&E->cav, (size_t)E->win_cap * E->kcap)) || (rc = dev_alloc(&E->fac, (size_t)E->win_cap * E->fcap)) ||
+        (rc = dev_alloc(&E->fnb, (size_t)E->win_cap * E->fcap)) ||
         (rc = dev_alloc(&E->nk, E->win_cap)) || (rc = dev_alloc(&E->nf, E->win_cap)) ||
         (rc = dev_alloc(&E->status, E->win_cap)) || (rc = dev_alloc(&E->big, E->big_cap)) ||
         (rc = dev_alloc(&E->cav1, E->solo_cap)) || (rc = dev_alloc(&E->fac1, E->solo_cap)) ||
         (rc = dev_alloc(&E->misc1, 4)) || (rc = dev_alloc(&E->st, 1)) || (rc = dev_alloc(&E->ctl, 1)) ||
         (rc = dev_alloc(&E->cnt, kCntStripes)) || (rc = dev_alloc(&E->q, E->qcap)) ||
         (rc = dev_alloc(&E->cache, (size_t)E->cmask + 1)) || (rc = dev_alloc(&E->ws, (size_t)E->ws_threads * E->ws_cap)) ||
-        (rc = dev_alloc(&E->done, E->nslots)))
+        (rc = dev_alloc(&E->done, E->nslots)) || (rc = dev_alloc(&E->fix, (size_t)2 * E->win_cap)))
         return bail(rc);
+    E->fix_cap = 2 * E->win_cap;
     E->out_cap = (size_t)1 << 26;
     if ((rc = dev_alloc(&E->out_d, E->out_cap))) return bail(rc);
     if (hipHostMalloc((void**)&E->out_h, sizeof(int32_t) * E->out_cap) != hipSuccess)
@@ -1307,6 +1329,14 @@ int vo_engine_config(vo_engine* e, const char* key, int64_t value) {
     } else if (k == "abort_early") {
         if (value != 0 && value != 1) return fail(VO_ERR_INVALID, "abort_early must be 0 or 1");
         e->abort_early = (int)value;
+    } else if (k == "reuse") {
+        if (value != 0 && value != 1) return fail(VO_ERR_INVALID, "reuse must be 0 or 1");
+        e->reuse = (int)value;
+        for (auto& g : e->graph)              // the captured rounds depend on it
+            if (g) {
+                hipGraphExecDestroy(g);
+                g = nullptr;
+            }
         drop_graphs(e);
     } else if (k == "tets_per_point_milli") {
         if (value < 1000 || value > 1000000) return fail(VO_ERR_INVALID, "tets_per_point_milli must be in 1000..1e6");

@@ -57,6 +57,7 @@ struct RoundState {
     alignas(256) int n_tets;          // simplices allocated (append-only high-water mark; one set only)
     alignas(256) int n_next_pending;  // losers and deferred points appended for the next window
     alignas(256) int n_q;             // exact-queue fill (may exceed capacity: excess dropped)
+    alignas(256) int n_fix;           // shared-facet links to repair after the commit (see Ctx::fix)
     unsigned int err;                 // sticky ERR_* bits
     int n_big;                        // points whose cavity outgrew the group kernel's LDS lists
     int n_nospace;                    // several sets: winners whose set had no simplex space left
@@ -117,6 +118,10 @@ struct Ctx {
     int* big;              // points to re-run alone with large cavity buffers
     int* cav;              // killed lists, one run per window slot: cav[slot * kcap + k]
     int* fac;              // boundary facets (t << 2 | face), likewise: fac[slot * fcap + k]
+    int* fnb;              // lane-group kernels: the adjacency entry across each facet as the cavity
+                           // search saw it (the simplex outside << 2 | its face, -1 on the hull),
+                           // parallel to fac; the commit reads it instead of the record, which
+                           // winners of the same commit may be patching
     int* nk;
     int* nf;
     int* status;
@@ -132,6 +137,10 @@ struct Ctx {
     uint32_t cmask;
     int filters;           // bit 0: FP32 tier on, bit 1: FP64 filter on (both on in production)
     int solo;              // the window holds one point: cavity membership is stamped in kres
+    int reuse;             // lane-group commit: new simplices take the winner's killed records first
+    int2* fix;             // lane-group commit without reuse: a winner whose cavity touches a
+    int fix_cap;           // higher-priority one lists (new simplex, killed record << 2 | face) for
+                           // each shared facet; k_fix_shared links the two sides after the commit
     int abort_early;       // lane-group cavity search: reserve each member as it is found and stop
                            // once a reservation shows a higher-priority point of the round
     int M;                 // vertex id of point p is M + p
@@ -1145,6 +1154,33 @@ __global__ void __launch_bounds__(256) k_check_pairs(Ctx c, int n, int nv, int4 
         if (su) atomicAdd(unresolved, (int)su);
     }
     flush_counters(c.cnt, lc);
+}
+
+// Round rule of the lane-group kernels.  Winners hold every kill; then either (shared facets)
+// a winner's cavity may touch a higher-priority winner's when the sphere test of the commit
+// admits it, the links across such facets being set by k_fix_shared, or (stricter, with record
+// reuse, or with no shared-facet list) the cavities of winners are also pairwise non-adjacent,
+// which the commit checks through outside reservations (nres).
+__device__ __forceinline__ bool shared_facets(const Ctx& c) { return !c.reuse && c.fix != nullptr; }
+
+// ---------------------------------------------------------------- k_fix_shared
+// Links across the facets shared by two winners of a round (a lower-priority winner W next to a
+// higher-priority cavity, admitted by the commit's sphere test).  W built its new simplex N on
+// the facet without linking it, and listed (N, A << 2 | face) with A its killed simplex on the
+// facet.  Whatever is across that face of A now is N's neighbor: the higher-priority winner's
+// new simplex on the facet (its outside patch landed in A, which only it writes there), or, when
+// that point did not win, the old simplex across the facet, which nobody patched.  One thread per
+// entry; the two ends are set here, after every commit of the round has finished.
+template <int D>
+__global__ void __launch_bounds__(256) k_fix_shared(Ctx c) {
+    const int n = min(c.st->n_fix, c.fix_cap);
+    for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < n; e += gridDim.x * blockDim.x) {
+        const int2 f = c.fix[e];
+        if (f.x < 0) continue;                  // reserved by a point that then did not win
+        const int X = *a_ptr<D>(c, f.y >> 2, f.y & 3);
+        *a_ptr<D>(c, f.x, 0) = X;
+        if (X >= 0) *a_ptr<D>(c, X >> 2, X & 3) = f.x << 2;
+    }
 }
 
 // ---------------------------------------------------------------- locate query

@@ -44,7 +44,7 @@ template <int G> struct Group {
 #define VO_GROUP_FC3 256
 #endif
 #ifndef VO_COMMIT_WAVES
-#define VO_COMMIT_WAVES 5
+#define VO_COMMIT_WAVES 4
 #endif
 #ifndef VO_CAVITY_WAVES2
 #define VO_CAVITY_WAVES2 3   // 2D (the kernel reaches 4 waves at its natural register count)
@@ -208,13 +208,13 @@ __global__ void __launch_bounds__(64, D == 3 ? VO_CAVITY_WAVES : VO_CAVITY_WAVES
         const int items = bfs ? (hi - lo) * M : 0;
         const int i = ci + g.gl;
         const bool valid = bfs && i < items;
-        int tet = 0, face = 0, n = -1, s = 0;
+        int tet = 0, face = 0, n = -1, s = 0, nb = -1;
         bool hull = false, test = false;
         if (valid) {
             const int mi = lo + i / M;
             tet = s_cav[g.gid][mi];
             face = i % M;
-            const int nb = mi < AC ? reinterpret_cast<const int*>(s_adj[g.gid])[mi * 4 + face] : *a_ptr<D>(c, tet, face);
+            nb = mi < AC ? reinterpret_cast<const int*>(s_adj[g.gid])[mi * 4 + face] : *a_ptr<D>(c, tet, face);
             hull = nb < 0;
             if (!hull) {
                 n = nb >> 2;
@@ -257,8 +257,11 @@ __global__ void __launch_bounds__(64, D == 3 ? VO_CAVITY_WAVES : VO_CAVITY_WAVES
                 if (isfac) {
                     // facets go straight to the point's global list: only the killed list,
                     // which every candidate is checked against, occupies LDS
-                    c.fac[(size_t)slot * c.fcap + nf + g.below(facm)] = (tet << 2) | face;
-                    if (!hull) atomicMin(nres_p<D>(c, n), key);
+                    const size_t fi = (size_t)slot * c.fcap + nf + g.below(facm);
+                    c.fac[fi] = (tet << 2) | face;
+                    c.fnb[fi] = hull ? -1 : nb;
+                    // (only the stricter commit rule reads outside reservations)
+                    if (!hull && !shared_facets(c)) atomicMin(nres_p<D>(c, n), key);
                 }
                 nk += cnew;
                 nf += cfac;
@@ -356,6 +359,85 @@ template <int D, int G> constexpr int kGroupMC =
     : ((D == 3 ? VO_MATCH_BYTES3 / (16 * (64 / G)) : 2048 / (8 * (64 / G))) < kGroupFC<D>
            ? (D == 3 ? VO_MATCH_BYTES3 / (16 * (64 / G)) : 2048 / (8 * (64 / G))) : kGroupFC<D>);
 
+// the new simplex (p first, then the facet) from its LDS tuple
+__device__ __forceinline__ int4 tuple_simplex(int4 fv, int pv) { return make_int4(pv, fv.y, fv.z, fv.w); }
+__device__ __forceinline__ int4 tuple_simplex(int2 fv, int pv) { return make_int4(pv, fv.x, fv.y, -1); }
+
+// Coordinates of new simplex N = (p, nv_1 .. nv_D): p's are pf; each facet vertex's come from
+// the record of the simplex across the facet, which holds the same vertex and lies outside the
+// cavity (no winner rewrites its vertices or coordinates), or, on the super-simplex hull, from
+// the vertex table.  Selects only, no dynamically indexed private array.
+template <int D>
+__device__ __forceinline__ void write_coords_from_outside(const Ctx& c, int N, int4 nv, typename VT<D>::F pf,
+                                                          int outside) {
+    using F = typename VT<D>::F;
+    if (outside < 0) {
+        F x[4];
+        x[0] = pf;
+        x[1] = VT<D>::ld(c.X32, nv.y);
+        x[2] = VT<D>::ld(c.X32, nv.z);
+        x[3] = D == 3 ? VT<D>::ld(c.X32, nv.w) : pf;
+        store_tx<D>(c, N, x);
+        return;
+    }
+    const int O = outside >> 2;
+    const int4 ov = ld_v<D>(c, O);
+    // position in O of each facet vertex
+    int m[3];
+#pragma unroll
+    for (int k = 0; k < D; ++k) {
+        const int vk = comp(nv, k + 1);
+        m[k] = ov.x == vk ? 0 : (ov.y == vk ? 1 : (ov.z == vk ? 2 : 3));
+    }
+    if constexpr (D == 3) {
+        // the packed floats of O, vertex m at X[3m .. 3m+2]
+        const float4* q = reinterpret_cast<const float4*>(rp<D>(c, O) + Rec<D>::X);
+        const float4 f0 = q[0], f1 = q[1], f2 = q[2];
+        const float X[12] = {f0.x, f0.y, f0.z, f0.w, f1.x, f1.y, f1.z, f1.w, f2.x, f2.y, f2.z, f2.w};
+        float o[9];
+#pragma unroll
+        for (int k = 0; k < 3; ++k)
+#pragma unroll
+            for (int j = 0; j < 3; ++j)
+                o[3 * k + j] = m[k] == 0 ? X[j] : (m[k] == 1 ? X[3 + j] : (m[k] == 2 ? X[6 + j] : X[9 + j]));
+        float4* w = reinterpret_cast<float4*>(rp<D>(c, N) + Rec<D>::X);
+        w[0] = make_float4(pf.x, pf.y, pf.z, o[0]);
+        w[1] = make_float4(o[1], o[2], o[3], o[4]);
+        w[2] = make_float4(o[5], o[6], o[7], o[8]);
+    } else {
+        F ox[4], x[4];
+        load_tx<D>(c, O, ox);
+        x[0] = pf;
+        x[1] = m[0] == 0 ? ox[0] : (m[0] == 1 ? ox[1] : ox[2]);
+        x[2] = m[1] == 0 ? ox[0] : (m[1] == 1 ? ox[1] : ox[2]);
+        x[3] = pf;
+        store_tx<D>(c, N, x);
+    }
+}
+
+// Two winners of a round may have adjacent cavities (sharing a boundary facet) when the later
+// one's point lies outside the sphere of the simplex the earlier one builds on that facet: then
+// inserting the earlier point first leaves the later one's cavity unchanged, and committing both
+// at once equals inserting them in priority order.  True iff vertex pv lies outside or on the
+// sphere of (p0, facet i of simplex t); t is pv's killed simplex on the facet, so (pv, facet as
+// replace_first orders it) is positively oriented and p0, across the facet, is positive with two
+// facet vertices swapped.  Only the FP32 tier is tried here (the commit kernel's registers are
+// few): undecided counts as inside, and the point waits for the next round.
+template <int D>
+__device__ __forceinline__ bool outside_next_star(const Ctx& c, int t, int i, int p0, int pv, typename VT<D>::F pf,
+                                                  LocalCnt& lc) {
+    using V = VT<D>;
+    if (!(c.filters & 1)) return false;
+    const int4 nv = replace_first<D>(ld_v<D>(c, t), i, pv);
+    const typename V::F a = V::ld(c.X32, c.M + p0), b = V::ld(c.X32, nv.z), d = V::ld(c.X32, nv.y);
+    int s;
+    if constexpr (D == 3) s = t1_insphere(a, b, d, V::ld(c.X32, nv.w), pf);
+    else s = t1_incircle(a, b, d, pf);
+    if (s == SIGN_UNDECIDED) return false;
+    lc.t1[1]++;
+    return s <= 0;
+}
+
 template <int D, int G>
 __global__ void __launch_bounds__(64, VO_COMMIT_WAVES) k_commit_g(Ctx c, int nwin, unsigned long long key_hi) {
     constexpr int M = D + 1;
@@ -369,6 +451,8 @@ __global__ void __launch_bounds__(64, VO_COMMIT_WAVES) k_commit_g(Ctx c, int nwi
     __shared__ __align__(16) typename FacetTuple<D>::T s_fv[NG][MC > 0 ? MC : 1];
     const Group<G> g;
     const bool lead = g.gl == 0;
+    // winners may share facets (see outside_next_star; linked by k_fix_shared)
+    const bool refined = shared_facets(c);
     LocalCnt lc = {};
     round_params(c, nwin, key_hi);
     const WinView wv = win_view(c);
@@ -378,7 +462,7 @@ __global__ void __launch_bounds__(64, VO_COMMIT_WAVES) k_commit_g(Ctx c, int nwi
     const bool valid = kb + g.gid < it.lim;
     int st = ST_IDLE, p = 0, nk = 0, nf = 0;
     unsigned long long key = 0;
-    bool ok = true;
+    bool ok = true, adj = false;
     if (valid) {
         st = c.status[slot];
         p = win_point(wv, slot);
@@ -386,10 +470,11 @@ __global__ void __launch_bounds__(64, VO_COMMIT_WAVES) k_commit_g(Ctx c, int nwi
         if (st == ST_OK) {
             nk = c.nk[slot];
             nf = c.nf[slot];
-            // members hold their kill reservation and no lower outside reservation; outside
-            // neighbors are killed by no lower key.  Member and facet checks share one pass with
-            // unconditional loads (indices clamped to the lists, never empty for an OK point):
-            // three dependent loads per pass
+            // members hold their kill reservation; outside neighbors are killed by no lower key
+            // (with reuse, also: members are nobody's outside neighbor with a lower key; without,
+            // an outside neighbor killed by a lower key marks its facet (bit 31) for the sphere
+            // test below).  Member and facet checks share one pass with unconditional loads
+            // (indices clamped to the lists, never empty for an OK point): three dependent loads
             const int n_it = nk > nf ? nk : nf;
             // two entries per lane and pass, their loads interleaved (with few lanes per point,
             // as in large windows, the passes would otherwise run one after another)
@@ -399,39 +484,133 @@ __global__ void __launch_bounds__(64, VO_COMMIT_WAVES) k_commit_g(Ctx c, int nwi
                 const int jk1 = j1 < nk ? j1 : nk - 1, jf1 = j1 < nf ? j1 : nf - 1;
                 const int x0 = c.cav[(size_t)slot * c.kcap + jk0], x1 = c.cav[(size_t)slot * c.kcap + jk1];
                 const int fe0 = c.fac[(size_t)slot * c.fcap + jf0], fe1 = c.fac[(size_t)slot * c.fcap + jf1];
+                const int nb0 = c.fnb[(size_t)slot * c.fcap + jf0], nb1 = c.fnb[(size_t)slot * c.fcap + jf1];
                 const ulonglong2 r0 = ld_res<D>(c, x0), r1 = ld_res<D>(c, x1);
-                const int nb0 = *a_ptr<D>(c, fe0 >> 2, fe0 & 3), nb1 = *a_ptr<D>(c, fe1 >> 2, fe1 & 3);
                 const unsigned long long ko0 = *kres_p<D>(c, nb0 >= 0 ? nb0 >> 2 : x0);
                 const unsigned long long ko1 = *kres_p<D>(c, nb1 >= 0 ? nb1 >> 2 : x1);
-                if (j0 < nf) s_fac[g.gid][j0] = fe0;
-                if (j1 < nf) s_fac[g.gid][j1] = fe1;
-                ok = ok & (j0 >= nk || ((r0.x == key) & (r0.y > key))) & (j0 >= nf || nb0 < 0 || ko0 > key);
-                ok = ok & (j1 >= nk || ((r1.x == key) & (r1.y > key))) & (j1 >= nf || nb1 < 0 || ko1 > key);
+                if (refined) {
+                    const bool a0 = j0 < nf && nb0 >= 0 && ko0 < key, a1 = j1 < nf && nb1 >= 0 && ko1 < key;
+                    if (j0 < nf) s_fac[g.gid][j0] = a0 ? (fe0 | (int)0x80000000) : fe0;
+                    if (j1 < nf) s_fac[g.gid][j1] = a1 ? (fe1 | (int)0x80000000) : fe1;
+                    ok = ok & (j0 >= nk || r0.x == key) & (j1 >= nk || r1.x == key);
+                    adj = adj | a0 | a1;
+                } else {
+                    if (j0 < nf) s_fac[g.gid][j0] = fe0;
+                    if (j1 < nf) s_fac[g.gid][j1] = fe1;
+                    ok = ok & (j0 >= nk || ((r0.x == key) & (r0.y > key))) & (j0 >= nf || nb0 < 0 || ko0 > key);
+                    ok = ok & (j1 >= nk || ((r1.x == key) & (r1.y > key))) & (j1 >= nf || nb1 < 0 || ko1 > key);
+                }
             }
         }
     }
-    const bool win = st == ST_OK && g.ballot(!ok) == 0;
-    // one slot range per winning group (the group's lead lane asks)
-    const int base = g.bcast(alloc_slots(c, win && lead, nf, p), 0);
+    // a point that holds every kill but touches a higher-priority cavity: the sphere test on
+    // each shared facet (the higher point is the least killer of the simplex across it, the only
+    // one of its killers that can win); nsh counts the facets to link after the commit
+    int nsh = 0;
+    const bool need = refined && g.ballot(adj) != 0 && g.ballot(!ok) == 0 && st == ST_OK;
+    __builtin_amdgcn_wave_barrier();
+    if (need) {
+        const typename VT<D>::F pf = VT<D>::ld(c.X32, c.M + p);
+        bool pass = true;
+        for (int j0 = 0; j0 < nf; j0 += G) {
+            const int j = j0 + g.gl;
+            bool sh = false;
+            if (j < nf) {
+                const int fe = s_fac[g.gid][j];
+                if (fe < 0) {
+                    sh = true;
+                    const int ff = fe & 0x7fffffff;
+                    const int nb = c.fnb[(size_t)slot * c.fcap + j];
+                    const unsigned long long ko = *kres_p<D>(c, nb >> 2);
+                    pass = pass && outside_next_star<D>(c, ff >> 2, ff & 3, (int)(uint32_t)ko, c.M + p, pf, lc);
+                }
+            }
+            nsh += __popcll(g.ballot(sh));
+        }
+        ok = ok && pass;
+    }
+    bool win = st == ST_OK && g.ballot(!ok) == 0;
+    // entries of the shared-facet list, reserved by the lead; a full list makes the point wait
+    int fix_base = 0;
+    if (refined) {
+        int fb = 0;
+        if (win && lead && nsh > 0) {
+            fb = atomicAdd(&c.st->n_fix, nsh);
+            if (fb + nsh > c.fix_cap) {
+                for (int e2 = fb; e2 < c.fix_cap; ++e2) c.fix[e2] = make_int2(-1, -1);
+                fb = -1;
+            }
+        }
+        fb = g.bcast(fb, 0);
+        win = win && fb >= 0;
+        fix_base = fb;
+    }
+    // Slots.  With reuse (and the facet tuples within the LDS match list), facet j's new simplex
+    // takes the winner's killed record j for j < nr = min(nk, nf) and a fresh slot past that, so
+    // that records stay about as many as live simplices and new ones land on lines the round has
+    // just read; only the fresh ones are allocated (one range per winning group, the lead asks).
+    const bool reuse = c.reuse && nf <= MC;
+    const int nr = reuse ? (nk < nf ? nk : nf) : 0;
+    const int base = g.bcast(alloc_slots(c, win && lead, nf - nr, p), 0);
     const bool build = win && base >= 0;
+    if (refined && win && !build && lead)
+        for (int e2 = 0; e2 < nsh; ++e2) c.fix[fix_base + e2] = make_int2(-1, -1);
     // a winner's check loops ran to completion, so s_fac holds its whole facet list
     __builtin_amdgcn_wave_barrier();
     const int pv = c.M + p;
     if (build) {
         const typename VT<D>::F pf = VT<D>::ld(c.X32, pv);
-        // phase 1: the new simplex of facet j, its outside link and the outside neighbor's patch
-        for (int j = g.gl; j < nf; j += G) {
-            const int fe = s_fac[g.gid][j];
-            const int t = fe >> 2, i = fe & 3;
-            const int4 nv = replace_first<D>(ld_v<D>(c, t), i, pv);
-            const int outside = *a_ptr<D>(c, t, i);
-            const int N = base + j;
-            st_v<D>(c, N, nv);
-            write_star_coords<D>(c, t, i, pf, N);
-            *a_ptr<D>(c, N, 0) = outside;
-            if (outside >= 0) *a_ptr<D>(c, outside >> 2, outside & 3) = N << 2;
-            if (nf <= MC) facet_tuple(s_fv[g.gid][j], nv);
+        if (reuse) {
+            // 1a: every read of the killed records before any of them is rewritten: the new
+            // simplex's vertices (its tuple, kept for the ridge match) and the simplex across
+            // facet j, which replaces facet j in s_fac
+            for (int j = g.gl; j < nf; j += G) {
+                const int fe = s_fac[g.gid][j];
+                const int t = fe >> 2, i = fe & 3;
+                facet_tuple(s_fv[g.gid][j], replace_first<D>(ld_v<D>(c, t), i, pv));
+                s_fac[g.gid][j] = *a_ptr<D>(c, t, i);
+            }
+            __builtin_amdgcn_wave_barrier();
+            // 1b: the new simplices, their outside links and the outside neighbors' patches;
+            // s_fac[j] becomes facet j's slot
+            for (int j = g.gl; j < nf; j += G) {
+                const int N = j < nr ? c.cav[(size_t)slot * c.kcap + j] : base + (j - nr);
+                const int outside = s_fac[g.gid][j];
+                const int4 nv = tuple_simplex(s_fv[g.gid][j], pv);
+                st_v<D>(c, N, nv);
+                write_coords_from_outside<D>(c, N, nv, pf, outside);
+                *a_ptr<D>(c, N, 0) = outside;
+                if (outside >= 0) *a_ptr<D>(c, outside >> 2, outside & 3) = N << 2;
+                s_fac[g.gid][j] = N;
+            }
+            __builtin_amdgcn_wave_barrier();
         }
+        // phase 1: the new simplex of facet j, its outside link and the outside neighbor's patch;
+        // a shared facet (marked) is left unlinked and listed for k_fix_shared instead, and its
+        // mark is cleared for the lookups that follow
+        int run = 0;
+        for (int j0 = 0; !reuse && j0 < nf; j0 += G) {
+            const int j = j0 + g.gl;
+            const int fm = j < nf ? s_fac[g.gid][j] : 0;
+            const bool sh = fm < 0;
+            const unsigned long long shm = g.ballot(sh);
+            if (j < nf) {
+                const int fe = fm & 0x7fffffff;
+                const int t = fe >> 2, i = fe & 3;
+                const int4 nv = replace_first<D>(ld_v<D>(c, t), i, pv);
+                const int outside = *a_ptr<D>(c, t, i);
+                const int N = base + j;
+                st_v<D>(c, N, nv);
+                write_star_coords<D>(c, t, i, pf, N);
+                *a_ptr<D>(c, N, 0) = outside;
+                if (sh) c.fix[fix_base + run + g.below(shm)] = make_int2(N, fe);
+                else if (outside >= 0) *a_ptr<D>(c, outside >> 2, outside & 3) = N << 2;
+                if (nf <= MC) facet_tuple(s_fv[g.gid][j], nv);
+                if (sh) s_fac[g.gid][j] = fe;
+            }
+            run += __popcll(shm);
+        }
+        __builtin_amdgcn_wave_barrier();
         // phase 2: internal faces, one lane per (facet, ridge); each link is written from both
         // sides by the lane that finds it, and the other side's lane skips it when it comes later
         for (int w = g.gl; w < DW; w += G) s_done[g.gid][w] = 0u;
@@ -442,13 +621,15 @@ __global__ void __launch_bounds__(64, VO_COMMIT_WAVES) k_commit_g(Ctx c, int nwi
             const int j = q / D, k = 1 + q % D;
             int kk = 1;
             const int jj = match_ridge(s_fv[g.gid], nf, j, k, kk);
+            const int Nj = reuse ? s_fac[g.gid][j] : base + j;
             if (jj < 0) {
                 atomicOr(&c.st->err, ERR_WALK);
-                *a_ptr<D>(c, base + j, k) = -1;
+                *a_ptr<D>(c, Nj, k) = -1;
                 continue;
             }
-            *a_ptr<D>(c, base + j, k) = ((base + jj) << 2) | kk;
-            *a_ptr<D>(c, base + jj, kk) = ((base + j) << 2) | k;
+            const int Njj = reuse ? s_fac[g.gid][jj] : base + jj;
+            *a_ptr<D>(c, Nj, k) = (Njj << 2) | kk;
+            *a_ptr<D>(c, Njj, kk) = (Nj << 2) | k;
             const int qq = jj * D + kk - 1;
             atomicOr(&s_done[g.gid][qq >> 5], 1u << (qq & 31));
         }
@@ -504,13 +685,15 @@ __global__ void __launch_bounds__(64, VO_COMMIT_WAVES) k_commit_g(Ctx c, int nwi
             const int qq = jj * D + kk - 1;
             atomicOr(&s_done[g.gid][qq >> 5], 1u << (qq & 31));
         }
-        // phase 3: killed simplices die, forwarding to a new simplex on one of their facets
-        for (int j = g.gl; j < nk; j += G) {
+        // phase 3: killed simplices die, forwarding to a new simplex on one of their facets; with
+        // reuse only the records past the reused ones die (nk > nf), forwarding to facet 0's
+        for (int j = g.gl; !reuse && j < nk; j += G) {
             const int t = c.cav[(size_t)slot * c.kcap + (j)];
             const int r = lds_find<2>(s_fac[g.gid], nf, t);
             const int fwd = base + (r < 0 ? 0 : r);
             kill<D>(c, t, fwd);
         }
+        for (int j = nr + g.gl; reuse && j < nk; j += G) kill<D>(c, c.cav[(size_t)slot * c.kcap + (j)], s_fac[g.gid][0]);
         if (lead) {
             lc.killed += nk;
             lc.created += nf;

@@ -129,6 +129,7 @@ struct vo_tri {
     int cur = 0, npend = 0;
     int* cav = nullptr;
     int* fac = nullptr;
+    int* fnb = nullptr;                // the outside link of each facet, as the cavity search saw it
     int* nk = nullptr;
     int* nf = nullptr;
     int* status = nullptr;
@@ -164,6 +165,12 @@ struct vo_tri {
     int chunk_rounds = 4;              // rounds submitted per chunk
     int round_timing = 0;              // events around each round's kernels (cost: ~3% at n = 4M)
     int abort_early = 1;               // losing cavity searches stop at the first reservation they lose
+    int reuse = -1;                    // new simplices take their winner's killed records first: 1 on,
+                                       // 0 off, -1 only when append-only records would pass 2^29
+    int reuse_on = 0;                  // this run's choice
+    int2* fix = nullptr;               // shared facets of the round's winners (k_fix_shared)
+    int fix_cap = 0;
+    int shared_facets = 1;             // 0: the stricter rule (winners' cavities never adjacent)
     int min_window = 64;               // least points per round (8 measured ~2% slower at 3D 4M / 2D 10M)
     int64_t small_run = 64;            // runs of at most this many points use the host loop
     int exact_dev = 0;                 // this run has queued exact predicates: evaluate them in the stream
@@ -195,6 +202,7 @@ Ctx make_ctx(vo_tri* t) {
     c.big = t->big;
     c.cav = t->cav;
     c.fac = t->fac;
+    c.fnb = t->fnb;
     c.nk = t->nk;
     c.nf = t->nf;
     c.status = t->status;
@@ -212,6 +220,9 @@ Ctx make_ctx(vo_tri* t) {
     c.filters = t->filters;
     c.solo = 0;
     c.abort_early = t->abort_early;
+    c.reuse = t->reuse_on;
+    c.fix = t->shared_facets ? t->fix : nullptr;
+    c.fix_cap = t->fix_cap;
     c.win_cap = t->win_cap;
     c.kcap = t->kcap;
     c.fcap = t->fcap;
@@ -324,6 +335,7 @@ __global__ void k_reset_round(RoundState* st) {
     st->n_next_pending = 0;
     st->n_winners = 0;
     st->n_q = 0;
+    st->n_fix = 0;
     st->n_live_delta = 0;
     st->n_big = 0;
 }
@@ -405,6 +417,7 @@ __device__ void plan_step(RunPlan& p, RoundState* st, bool s_absorb) {
     st->n_next_pending = 0;
     st->n_winners = 0;
     st->n_q = 0;
+    st->n_fix = 0;
     st->n_live_delta = 0;
     st->n_big = 0;
     p.launched = 1;
@@ -488,8 +501,8 @@ int upload_super(vo_tri* t) {
 void free_all(vo_tri* t) {
     void* ptrs[] = {t->X64, t->X32, t->cp, t->vmap, t->rec, t->rec_spare, t->cmap, t->chist, t->reg_meta, t->win2,
                     t->regs, t->win, t->pend[0], t->pend[1],
-                    t->cav, t->fac, t->nk, t->nf, t->status, t->hint, t->st, t->cnt, t->q, t->cache, t->ws,
-                    t->big, t->cav1, t->fac1, t->misc1, t->plan};
+                    t->cav, t->fac, t->fnb, t->nk, t->nf, t->status, t->hint, t->st, t->cnt, t->q, t->cache, t->ws,
+                    t->big, t->cav1, t->fac1, t->misc1, t->plan, t->fix};
     for (void* p : ptrs)
         if (p) hipFree(p);
     if (t->st_host) hipHostFree(t->st_host);
@@ -719,6 +732,14 @@ int launch_round(vo_tri* t, const Ctx& c, int nwin, unsigned long long key_hi, i
         }
     }
     VO_HIP(hipGetLastError());
+    // links across the facets two winners share (only without reuse; the thread commit keeps to
+    // disjoint, non-adjacent cavities)
+    if (which && G != 1 && !c.reuse && c.fix) {
+        const int fb = std::max(1, std::min(1024, nwin / 64 + 1));
+        if (t->D == 3) hipLaunchKernelGGL(k_fix_shared<3>, dim3(fb), dim3(256), 0, t->s, c);
+        else hipLaunchKernelGGL(k_fix_shared<2>, dim3(fb), dim3(256), 0, t->s, c);
+        VO_HIP(hipGetLastError());
+    }
     return 0;
 }
 #undef VO_LAUNCH_G
@@ -789,6 +810,22 @@ int ensure_records(vo_tri* t, int64_t need) {
 
 // records a window point may create at most, as the planners reserve them
 int64_t window_records(int D) { return D == 3 ? 64 : 16; }
+
+// Record reuse for this run: on request, or (auto) when append-only records for the points still
+// to insert would not fit the 2^29 the adjacency packing allows.  Reuse costs time (a winner's new
+// simplices scatter over its killed records' lines instead of filling fresh consecutive ones:
+// measured 3D n = 4M 147 -> 154 ms, 2D n = 10M 99 -> 114 ms) and needs the stricter round rule.
+void choose_reuse(vo_tri* t, int64_t to_insert) {
+    const int64_t append_only = (int64_t)t->n_tets + (t->D == 3 ? 30 : 8) * to_insert + 4096;
+    t->reuse_on = t->reuse == 1 || (t->reuse < 0 && append_only > ((int64_t)1 << 29));
+}
+
+// records reserved per point to insert: a winner creates about 27 (3D) / 6 (2D) simplices; with
+// reuse all but about 7 (3D) / 2 (2D) of them take the winner's killed records
+int64_t records_per_point(const vo_tri* t) {
+    if (t->reuse_on) return t->D == 3 ? 12 : 4;
+    return t->D == 3 ? 30 : 8;
+}
 
 // The host's side of a finished round whose state is in st_host: errors, points re-run alone,
 // the exact queue, then the counters and the window factor (k_plan_one applies the same rule
@@ -971,7 +1008,8 @@ int run_device_planned(vo_tri* t) {
     // a planner halt grows it when a distribution needs more
     {
         const int64_t left = t->n_pts - t->head + t->npend;
-        const int64_t want = std::min<int64_t>((int64_t)t->n_tets + left * (D == 3 ? 30 : 8) + 4096, (int64_t)1 << 29);
+        const int64_t want =
+            std::min<int64_t>((int64_t)t->n_tets + left * records_per_point(t) + 4096, (int64_t)1 << 29);
         if ((rc = ensure_records(t, want))) return rc;
     }
     while (t->head < t->n_pts || t->npend > 0) {
@@ -1339,13 +1377,16 @@ int vo_tri_create(int hip_device, int dim, const double* pts, int64_t n, vo_tri*
     t->big_window = dim == 3 ? 16384 : 65536;
     if ((rc = dev_alloc(&t->win, t->win_cap)) || (rc = dev_alloc(&t->pend[0], t->win_cap)) ||
         (rc = dev_alloc(&t->pend[1], t->win_cap)) || (rc = dev_alloc(&t->cav, (size_t)t->win_cap * t->kcap)) ||
-        (rc = dev_alloc(&t->fac, (size_t)t->win_cap * t->fcap)) || (rc = dev_alloc(&t->nk, t->win_cap)) ||
+        (rc = dev_alloc(&t->fac, (size_t)t->win_cap * t->fcap)) || (rc = dev_alloc(&t->fnb, (size_t)t->win_cap * t->fcap)) ||
+        (rc = dev_alloc(&t->nk, t->win_cap)) ||
         (rc = dev_alloc(&t->nf, t->win_cap)) || (rc = dev_alloc(&t->status, t->win_cap)) ||
         (rc = dev_alloc(&t->st, 1)) || (rc = dev_alloc(&t->cnt, kCntStripes)) || (rc = dev_alloc(&t->q, t->qcap)) ||
         (rc = dev_alloc(&t->cache, (size_t)t->cmask + 1)) || (rc = dev_alloc(&t->ws, (size_t)t->ws_threads * t->ws_cap)) ||
         (rc = dev_alloc(&t->big, t->win_cap)) || (rc = dev_alloc(&t->cav1, t->big_cap)) ||
-        (rc = dev_alloc(&t->fac1, t->big_cap)) || (rc = dev_alloc(&t->misc1, 4)))
+        (rc = dev_alloc(&t->fac1, t->big_cap)) || (rc = dev_alloc(&t->misc1, 4)) ||
+        (rc = dev_alloc(&t->fix, (size_t)2 * t->win_cap)))
         return bail(rc);
+    t->fix_cap = 2 * t->win_cap;
     if (hipHostMalloc((void**)&t->st_host, sizeof(RoundState)) != hipSuccess) return bail(fail(VO_ERR_OOM, "pinned"));
     hipMemsetAsync(t->st, 0, sizeof(RoundState), t->s);
     hipMemsetAsync(t->cnt, 0, sizeof(Counters) * kCntStripes, t->s);
@@ -1423,7 +1464,8 @@ int vo_tri_stage(vo_tri* t, const double* pts, int64_t n, uint64_t seed) {
         VO_HIP(hipMemsetAsync(t->hint, 0, sizeof(int), t->s));
     }
     // simplex capacity for everything staged (uniform data creates ~27 (3D) / 6 (2D) per point)
-    const int64_t want = (int64_t)t->n_tets + (D == 3 ? 34 : 8) * (n_after - t->inserted - t->dups) + 4096;
+    choose_reuse(t, n_after - t->inserted - t->dups);
+    const int64_t want = (int64_t)t->n_tets + records_per_point(t) * (n_after - t->inserted - t->dups) + 4096;
     if ((rc = grow_tets(t, want))) return rc;
     VO_HIP(hipStreamSynchronize(t->s));
     t->n_pts = n_after;
@@ -1434,6 +1476,7 @@ int vo_tri_stage(vo_tri* t, const double* pts, int64_t n, uint64_t seed) {
 int vo_tri_run(vo_tri* t) {
     if (!t) return fail(VO_ERR_INVALID, "null handle");
     VO_HIP(hipSetDevice(t->dev));
+    choose_reuse(t, t->n_pts - t->head + t->npend);
     t->ms_cav = t->ms_com = t->ms_ex = t->ms_other = 0;
     t->n_cav = t->n_com = 0;
     t->stalled = 0;
@@ -1472,7 +1515,13 @@ int vo_tri_counts(const vo_tri* t, int64_t* n_vertices, int64_t* n_simplices, in
     if (!t) return fail(VO_ERR_INVALID, "null handle");
     if (n_vertices) *n_vertices = t->M + t->inserted;
     if (n_simplices) *n_simplices = t->n_live;
-    if (max_simplex_id) *max_simplex_id = (int64_t)t->n_tets + t->retired - 1 + t->M;
+    if (max_simplex_id) {
+        // simplices ever created (records are reused, so they are counted, not the records)
+        VO_HIP(hipSetDevice(t->dev));
+        std::vector<Counters> hs(kCntStripes);
+        VO_HIP(hipMemcpy(hs.data(), t->cnt, sizeof(Counters) * kCntStripes, hipMemcpyDeviceToHost));
+        *max_simplex_id = (int64_t)sum_counters(hs.data()).created + t->M;
+    }
     return 0;
 }
 
@@ -1857,6 +1906,7 @@ int vo_tri_stats(const vo_tri* t, vo_stats* out) {
     out->compactions = t->n_compact;
     out->ms_compact = t->ms_compact;
     out->duplicates_detected = (int64_t)h.dups;
+    out->records = t->n_tets;
     return 0;
 }
 
@@ -1909,6 +1959,12 @@ int vo_tri_config(vo_tri* t, const char* key, int64_t value) {
     } else if (k == "abort_early") {
         if (value != 0 && value != 1) return fail(VO_ERR_INVALID, "abort_early must be 0 or 1");
         t->abort_early = (int)value;
+    } else if (k == "shared_facets") {
+        if (value != 0 && value != 1) return fail(VO_ERR_INVALID, "shared_facets must be 0 or 1");
+        t->shared_facets = (int)value;
+    } else if (k == "reuse") {
+        if (value < -1 || value > 1) return fail(VO_ERR_INVALID, "reuse must be -1 (auto), 0 or 1");
+        t->reuse = (int)value;
     } else if (k == "chunk_rounds") {
         if (value < 1 || value > 64) return fail(VO_ERR_INVALID, "chunk_rounds must be in 1..64");
         t->chunk_rounds = (int)value;
