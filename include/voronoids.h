@@ -62,6 +62,10 @@ typedef struct {
  * inserted.  pts is row-major n x dim f64. */
 int vo_tri_create(int hip_device, int dim, const double* pts, int64_t n, vo_tri** out);
 void vo_tri_destroy(vo_tri* t);
+/* the same handle made into DelaunayTree::new(pts) again (a new point set of the same dimension;
+ * no counterpart in the reference, which builds a new tree): its device buffers are kept, so a
+ * caller triangulating many sets one after another allocates nothing per set */
+int vo_tri_reset(vo_tri* t, const double* pts, int64_t n);
 
 /* add_points_to_tree (src/delaunay_tree.rs:336-386) / the insert_point loop of
  * voronoids.delaunay (src/lib.rs:104-125): inserts n points in a random priority order drawn
@@ -135,7 +139,7 @@ int vo_tri_stats(const vo_tri* t, vo_stats* out);
  * size per inserted point x 1000.  "local_order" = 0 (default: the seeded random priority order) or k > 0
  * (the same order with consecutive chunks of position/k points sorted along the Morton curve; set before
  * vo_tri_stage).  "big_window" (default 16384 in 3D, 65536 in 2D; 0 = off), "big_frac_lo_milli" /
- * "big_frac_hi_milli" (default 550 / 800): rounds whose window is at least big_window steer the window
+ * "big_frac_hi_milli" (default 650 / 800 in 3D, 550 / 800 in 2D): rounds whose window is at least big_window steer the window
  * size so that this fraction of it wins (smaller windows aim at "small_frac_lo_milli" .. "small_frac_hi_milli", default 350 / 600).  "compact_ratio_milli" (default
  * 0 = never; measured no faster at n = 4M: every visit misses the caches either way): between rounds, when the record array holds more than this many records (x 1000) per live
  * simplex, the live ones move to a dense array in spatial order; "compact_min" (default 2^20): not before this
@@ -154,7 +158,17 @@ int vo_tri_stats(const vo_tri* t, vo_stats* out);
  * rounds, windows and ids on every run; measured 157 -> 147 ms at 3D n = 4M).  "min_window" (default 64,
  * 1..65536): least points per round (the window is max(min_window, gamma x inserted)).  "small_run" (default
  * 64): a vo_tri_run with at most this many points to insert plans its rounds on the host (one point: 0.16 ms
- * in 2D instead of 0.22 ms through the device planner, which submits at least two chunks of rounds). */
+ * in 2D instead of 0.22 ms through the device planner, which submits at least two chunks of rounds).
+ * "shared_facets" (1 default / 0): a window point whose cavity touches (shares a facet with) a
+ * higher-priority point's may still win when it lies outside the sphere of the simplex that point
+ * builds on the shared facet (inserting that point first would leave its cavity unchanged); the links
+ * across such facets are set after the commit.  0: winners' cavities are also pairwise non-adjacent
+ * (same triangulation; measured 3D n = 4M 149 -> 117 ms and 2D n = 10M 99 -> 74 ms with 1).
+ * "reuse" (-1 default = auto / 0 / 1): new simplices take their winner's killed records first, so that
+ * records stay about as many as live simplices (append-only: about 4x in 3D); auto turns it on only when
+ * the append-only records of a run would pass 2^29, since it is slower (3D n = 4M +5%, 2D n = 10M +15%:
+ * a winner's new simplices then scatter over its killed records' lines); with reuse the rounds keep
+ * winners' cavities non-adjacent. */
 int vo_tri_config(vo_tri* t, const char* key, int64_t value);
 
 /* The precision tiers one predicate at a time (no counterpart in the reference, whose in_sphere

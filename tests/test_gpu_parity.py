@@ -419,3 +419,32 @@ def test_translated_and_scaled_inputs_match_oracle(dim, name):
     assert np.array_equal(T, ref.T) and np.array_equal(E, ref.E), (dim, name)
     assert t.violations() == 0
     t.close()
+
+
+@pytest.mark.parametrize("dim", [3, 2])
+def test_reset_handle_equals_a_fresh_tree(dim):
+    # vo_tri_reset: one handle, many sets in a row (larger, smaller, tiny, shifted), each equal to
+    # a fresh tree's triangulation and the oracle's; the round keys go on decreasing across sets
+    from voronoids_amd import batch as B
+    plain = B.PlainPath()
+    try:
+        for k, (n, shift) in enumerate(((20000, 0.0), (3000, 5.0), (50, -2.0), (40000, 0.5))):
+            P = datasets.uniform(n, dim, seed=140 + k) + shift
+            s, rc = plain.simplices(P, seed=k)
+            s0, rc0 = B.plain_simplices(P, seed=k)
+            assert rc == 0 and rc0 == 0
+            assert B.simplex_set_hash(s) == B.simplex_set_hash(s0), (k, n)
+            T, E = B.canonical_from_simplices(s, dim)
+            ref = O.Tree.delaunay(P, O.MODE_EXACT).canonical()
+            assert np.array_equal(T, ref.T) and np.array_equal(E, ref.E), (k, n)
+    finally:
+        plain.close()
+    t = vo.DelaunayTree(datasets.uniform(1000, dim, seed=1))
+    t.add_points_to_tree(datasets.uniform(1000, dim, seed=1))
+    r1 = t.stats()["rounds"]
+    P2 = datasets.uniform(500, dim, seed=2)
+    t.reset(P2)
+    assert t.counts()[:2] == (dim + 1, 1) and t.max_simplex_id == dim + 1
+    t.add_points_to_tree(P2)
+    assert 0 < t.stats()["rounds"] <= r1 and t.check_delaunay()
+    assert t.simplices[dim + 2].vertices is not None and len(t.vertices) == 2 * (dim + 1) + 500

@@ -100,6 +100,18 @@ class DelaunayTree:
         for key, value in _tuned(self.dim).items():
             self.config(key, value)
 
+    def reset(self, vertices) -> None:
+        """the same handle as DelaunayTree(vertices) again, its device buffers kept (vo_tri_reset)"""
+        p = _as_points(vertices, self.dim)
+        _lib.check(_lib.lib().vo_tri_reset(self._h, p.ctypes.data_as(_dp), p.shape[0]))
+        self._n_input = 0
+        c = np.zeros(3)
+        r = ctypes.c_double(0.0)
+        if p.shape[0]:
+            _lib.check(_lib.lib().vo_bounding_sphere(self.dim, p.ctypes.data_as(_dp), p.shape[0],
+                                                     c.ctypes.data_as(_dp), ctypes.byref(r)))
+        self._root_sphere = (c[:self.dim].tolist(), 10.0 * (r.value if r.value > 0 else 1.0))
+
     def close(self) -> None:
         if getattr(self, "_h", None):
             _lib.lib().vo_tri_destroy(self._h)

@@ -31,6 +31,7 @@ EXPORTED = [
     "vo_predicates", "vo_bounding_sphere",
     "vo_tri_check_pairs", "vo_tri_set_vertex", "vo_tri_swap_simplex_vertices",
     "vo_circumsphere", "vo_circumspheres", "vo_in_sphere", "vo_predicates_noalloc", "vo_tri_locate",
+    "vo_tri_reset",
 ]
 
 
@@ -101,6 +102,7 @@ def lib() -> ctypes.CDLL:
         L.vo_tri_create.argtypes = [ctypes.c_int, ctypes.c_int, dp, ctypes.c_int64, ctypes.POINTER(P)]
         L.vo_bounding_sphere.argtypes = [ctypes.c_int, dp, ctypes.c_int64, dp, dp]
         L.vo_tri_destroy.argtypes = [P]
+        L.vo_tri_reset.argtypes = [P, dp, ctypes.c_int64]
         L.vo_tri_destroy.restype = None
         for name in ("vo_tri_insert", "vo_tri_stage"):
             getattr(L, name).argtypes = [P, dp, ctypes.c_int64, ctypes.c_uint64]

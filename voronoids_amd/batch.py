@@ -77,13 +77,48 @@ def plain_simplices(points, seed: int = 0, device: int = 0) -> tuple[np.ndarray,
         rc = L.vo_tri_insert(h, p.ctypes.data_as(_dp), p.shape[0], seed)
         if rc not in (_lib.VO_OK, _lib.VO_ERR_DUPLICATE):
             _lib.check(rc)
-        nv, ns, mx = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64()
-        _lib.check(L.vo_tri_counts(h, ctypes.byref(nv), ctypes.byref(ns), ctypes.byref(mx)))
-        s = np.zeros((ns.value, p.shape[1] + 1), np.int32)
-        _lib.check(L.vo_tri_export(h, s.ctypes.data_as(_i32p), None))
-        return s, rc
+        return _export_simplices(h, p.shape[1]), rc
     finally:
         L.vo_tri_destroy(h)
+
+
+def _export_simplices(h, dim: int) -> np.ndarray:
+    L = _lib.lib()
+    ns = ctypes.c_int64()
+    _lib.check(L.vo_tri_counts(h, None, ctypes.byref(ns), None))
+    s = np.zeros((ns.value, dim + 1), np.int32)
+    _lib.check(L.vo_tri_export(h, s.ctypes.data_as(_i32p), None))
+    return s
+
+
+class PlainPath:
+    """The one-set path with one handle per dimension, made into each new set's tree with
+    vo_tri_reset (no device allocation per set): insert, export the simplices."""
+
+    def __init__(self, device: int = 0):
+        self.device = device
+        self.h = {}
+
+    def simplices(self, points, seed: int = 0) -> tuple[np.ndarray, int]:
+        p = np.ascontiguousarray(points, dtype=np.float64)
+        dim = p.shape[1]
+        L = _lib.lib()
+        h = self.h.get(dim)
+        if h is None:
+            h = ctypes.c_void_p()
+            _lib.check(L.vo_tri_create(self.device, dim, p.ctypes.data_as(_dp), p.shape[0], ctypes.byref(h)))
+            self.h[dim] = h
+        else:
+            _lib.check(L.vo_tri_reset(h, p.ctypes.data_as(_dp), p.shape[0]))
+        rc = L.vo_tri_insert(h, p.ctypes.data_as(_dp), p.shape[0], seed)
+        if rc not in (_lib.VO_OK, _lib.VO_ERR_DUPLICATE):
+            _lib.check(rc)
+        return _export_simplices(h, dim), rc
+
+    def close(self) -> None:
+        for h in self.h.values():
+            _lib.lib().vo_tri_destroy(h)
+        self.h = {}
 
 
 def simplex_set_hash(simplices) -> int:
