@@ -104,8 +104,10 @@ def cpu_baseline(P: np.ndarray, threads: int) -> dict:
 
 def measured_traffic(workload: str, kernel: str):
     """HBM bytes per launch of `kernel` from the committed rocprofv3 counter summary of this
-    workload (FETCH_SIZE and WRITE_SIZE passes, each scaled by what the counter reads for a
-    device-to-device copy of known size), or None when no summary exists."""
+    workload (FETCH_SIZE scaled by what the counter reads for random 128 B-line gathers of known
+    size -- the round kernels' access pattern; a streaming copy reads twice what FETCH_SIZE
+    reports -- and WRITE_SIZE by what it reads for a device-to-device copy), or None when no
+    summary exists."""
     path = os.path.join(ROOT, "profiles", TRAFFIC_PROFILES.get(workload, "-"))
     if not os.path.exists(path):
         return None
@@ -504,6 +506,27 @@ def measure_workload(wname: str, steps: int, warmup: int, kts: int, e2e_steps: i
     return out
 
 
+def gather_calibration(n_lines: int, reads: int, reps: int = 3) -> None:
+    """Random gathers of a known footprint (torch's index_select on the device), so that
+    rocprofv3's FETCH_SIZE can be scaled for the round kernels' access pattern: `reads` random
+    128 B lines read whole (rows of 32 floats), then `reads` random 16 B reads, one per random
+    line (rows of 4 floats at a line's start), out of a table of n_lines lines."""
+    import torch
+    dev = torch.device("cuda:0")
+    x = torch.ones(n_lines * 32, dtype=torch.float32, device=dev)
+    g = torch.Generator(device=dev)
+    g.manual_seed(1)
+    idx = torch.randint(0, n_lines, (reads,), device=dev, generator=g)
+    lines, quads = x.view(n_lines, 32), x.view(n_lines * 8, 4)
+    for _ in range(reps):
+        a = torch.index_select(lines, 0, idx)
+        b = torch.index_select(quads, 0, idx * 8)
+    torch.cuda.synchronize()
+    print(json.dumps({"gather_calibration": {"table_bytes": n_lines * 128, "reads": reads, "reps": reps,
+                                             "line_bytes_read": reads * 128, "quad_bytes_read": reads * 16,
+                                             "checksum": float(a.sum() + b.sum())}}), flush=True)
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -523,9 +546,15 @@ def main() -> int:
     ap.add_argument("--e2e-steps", type=int, default=3, help="end-to-end steps (host points to host simplices)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run device-to-device copies of BYTES (for rocprofv3 counter scaling)")
+    ap.add_argument("--gather-calibration", type=int, default=0, metavar="READS",
+                    help="only run READS random 128 B-line and 16 B gathers over a 16 GiB table "
+                         "(for rocprofv3 counter scaling of random access)")
     args = ap.parse_args()
     if args.copy_calibration:
         copy_calibration(args.copy_calibration)
+        return 0
+    if args.gather_calibration:
+        gather_calibration(1 << 27, args.gather_calibration)
         return 0
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (asked for {args.gpus}); "
