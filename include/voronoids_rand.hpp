@@ -16,8 +16,11 @@
  *   - Uniform<f64>::from(low..high) (rand 0.8 UniformFloat): a draw takes next_u64() >> 12 as the
  *     mantissa of a double in [1, 2), subtracts 1 and returns value0_1 * scale + low, where scale
  *     is high - low reduced until low + scale * (1 - 2^-52) < high.
- * The ChaCha core and the buffering are checked against RFC 8439 / RFC 7539 known answers; the
- * PCG32 seed expansion and the float mapping have no published vectors to check against.
+ * The ChaCha core and the buffering are checked against RFC 8439 / RFC 7539 known answers, and
+ * the PCG32 step and output function against the published output of O'Neill's pcg32 demo
+ * (pcg32_srandom_r(42, 54)).  What no published vector fixes: rand_core's increment constant
+ * and its advance-before-output order, and the float mapping (checked for range, determinism
+ * and the exact values its bit construction gives).
  */
 #ifndef VORONOIDS_RAND_HPP
 #define VORONOIDS_RAND_HPP
@@ -32,6 +35,15 @@ namespace voronoids {
 namespace rand_compat {
 
 inline uint32_t rotl32(uint32_t x, int r) { return (x << r) | (x >> (32 - r)); }
+
+/* PCG32 (O'Neill 2014, pcg-c-basic): the 64-bit LCG step and the XSH-RR output function */
+constexpr uint64_t kPcgMul = 6364136223846793005ull;
+inline uint64_t pcg32_advance(uint64_t state, uint64_t inc) { return state * kPcgMul + inc; }
+inline uint32_t pcg32_output(uint64_t state) {
+    const uint32_t xorshifted = (uint32_t)(((state >> 18) ^ state) >> 27);
+    const int rot = (int)(state >> 59);
+    return (xorshifted >> rot) | (xorshifted << ((32 - rot) & 31));
+}
 
 /* the ChaCha quarter round (RFC 8439 section 2.1) on four words of the state */
 inline void quarter_round(uint32_t& a, uint32_t& b, uint32_t& c, uint32_t& d) {
@@ -80,10 +92,8 @@ public:
     static ChaChaRng seed_from_u64(uint64_t state) {
         Seed seed{};
         for (int i = 0; i < 8; ++i) {
-            state = state * 6364136223846793005ull + 11634580027462260723ull;
-            const uint32_t xorshifted = (uint32_t)(((state >> 18) ^ state) >> 27);
-            const int rot = (int)(state >> 59);
-            const uint32_t x = (xorshifted >> rot) | (xorshifted << ((32 - rot) & 31));
+            state = pcg32_advance(state, 11634580027462260723ull);   // advanced before the output
+            const uint32_t x = pcg32_output(state);
             for (int b = 0; b < 4; ++b) seed[4 * i + b] = (uint8_t)(x >> (8 * b));
         }
         return from_seed(seed);

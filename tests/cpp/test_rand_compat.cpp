@@ -4,8 +4,12 @@
 //     vectors 1 and 2: counter 0 and 1), which pins the key / counter / stream layout and the
 //     block order of the buffered output;
 //   - the buffer's refill boundary for next_u32 / next_u64 against direct block calls.
-// ChaCha12 (StdRng), the PCG32 seed expansion and the float mapping have no published vectors:
-// StdRng and Uniform are exercised for determinism and range only.
+//   - the PCG32 LCG step and XSH-RR output function against the published output of O'Neill's
+//     pcg32 demo (pcg-c-basic, pcg32_srandom_r(42, 54)), which pins the multiplier, the shifts
+//     and the rotation direction used by the seed expansion;
+//   - the float mapping on chosen next_u64 values, where the bit construction fixes the result.
+// ChaCha12 (StdRng) and rand_core's increment constant have no published vectors: StdRng is
+// exercised for determinism and range.
 #include <cstdio>
 #include <cstdlib>
 #include <vector>
@@ -76,6 +80,30 @@ int main() {
         for (int i = 0; i < 63; ++i) c.next_u32();
         CHECK(c.next_u64() == ((uint64_t)words[63] | (uint64_t)words[64] << 32));   // low word from the old buffer
         CHECK(c.next_u32() == words[65]);
+    }
+    {   // O'Neill's pcg32 demo: srandom(initstate 42, initseq 54) = {state 0, inc 2*54+1; step;
+        // state += 42; step}; each draw outputs from the state before its step
+        const uint64_t inc = (54ull << 1) | 1u;
+        uint64_t st = pcg32_advance(0, inc);
+        st = pcg32_advance(st + 42u, inc);
+        const uint32_t want[6] = {0xa15c02b7u, 0x7b47f409u, 0xba1d3330u, 0x83d2f293u, 0xbfa4784bu, 0xcbed606eu};
+        for (int i = 0; i < 6; ++i) {
+            CHECK(pcg32_output(st) == want[i]);
+            st = pcg32_advance(st, inc);
+        }
+    }
+    {   // the float mapping: next_u64() >> 12 is the mantissa of value1_2 in [1, 2)
+        struct Fixed {
+            uint64_t v;
+            uint64_t next_u64() { return v; }
+        };
+        const UniformF64 unit(0.0, 1.0), wide(-2.0, 3.0);
+        Fixed zero{0}, half{1ull << 63}, top{~0ull}, low_bits{0xfffull};
+        CHECK(unit.sample(zero) == 0.0 && wide.sample(zero) == -2.0);
+        CHECK(unit.sample(half) == 0.5 && wide.sample(half) == 0.5);
+        CHECK(unit.sample(top) == 1.0 - 0x1p-52);
+        CHECK(wide.sample(top) == 3.0 - 0x1p-50);   // 5 (1 - 2^-52) rounds to 5 - 2^-50
+        CHECK(unit.sample(low_bits) == 0.0);         // the low 12 bits are dropped
     }
     {   // Uniform(0, 1): in range, deterministic, mean near 1/2
         StdRng r1 = StdRng::seed_from_u64(0), r2 = StdRng::seed_from_u64(0);
