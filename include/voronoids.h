@@ -52,8 +52,9 @@ typedef struct {
      * and their device time, ms (included in ms_run) */
     int64_t compactions;
     double ms_compact;
-    /* simplex records in use, live and dead: with reuse (the default) about the live count, since
-     * new simplices take their winner's killed records (vo_tri_config "reuse") */
+    /* simplex records in use, live and dead: with record reuse on (vo_tri_config "reuse") about
+     * the live count, since new simplices take their winner's killed records; append-only (the
+     * default below 2^29 records) about 4x that in 3D */
     int64_t records;
 } vo_stats;
 
