@@ -102,6 +102,17 @@ def cpu_baseline(P: np.ndarray, threads: int) -> dict:
                       f"add_points_to_tree, the reference's arithmetic), one run, {dt:.1f} s wall"}
 
 
+def tuning_applied() -> dict:
+    """the autotuner's stored choice per dimension, applied to every run here: its build constants
+    are compiled into the library and DelaunayTree applies its runtime setting"""
+    path = os.path.join(ROOT, "voronoids_amd", "tuning.json")
+    if not os.path.exists(path):
+        return {}
+    with open(path) as f:
+        table = json.load(f)
+    return {d: {k: table[d].get(k) for k in ("build", "setting", "ms", "default_ms", "workload")} for d in sorted(table)}
+
+
 def measured_traffic(workload: str, kernel: str):
     """HBM bytes per launch of `kernel` from the committed rocprofv3 counter summary of this
     workload (FETCH_SIZE scaled by what the counter reads for random 128 B-line gathers of known
@@ -584,6 +595,7 @@ def main() -> int:
     }
     if "violations" in m:
         out["violations"] = m["violations"]
+    out["tuning"] = tuning_applied()
     if args.workload == "3d4m" and not args.no_second:
         s2 = measure_workload("2d10m", args.steps, args.warmup, kts, args.e2e_steps, args.check)
         out["second_config"] = {"metric": "2D Delaunay inserted pts/sec @10M", "value": s2["value"], "unit": "pts/s",

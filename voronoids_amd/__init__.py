@@ -81,7 +81,9 @@ def _tuned(dim: int) -> dict:
 
 
 class DelaunayTree:
-    def __init__(self, vertices, device: int = 0):
+    def __init__(self, vertices, device: int = 0, tuned: bool = True):
+        """DelaunayTree::new (src/delaunay_tree.rs:389-510, :544-640) on HIP device `device`;
+        tuned: apply the autotuner's stored runtime setting for this dimension (tuning.json)"""
         p = _as_points(vertices)
         self.dim = p.shape[1]
         self.M = self.dim + 1
@@ -97,8 +99,9 @@ class DelaunayTree:
             _lib.check(_lib.lib().vo_bounding_sphere(self.dim, p.ctypes.data_as(_dp), p.shape[0],
                                                      c.ctypes.data_as(_dp), ctypes.byref(r)))
         self._root_sphere = (c[:self.dim].tolist(), 10.0 * (r.value if r.value > 0 else 1.0))
-        for key, value in _tuned(self.dim).items():
-            self.config(key, value)
+        if tuned:
+            for key, value in _tuned(self.dim).items():
+                self.config(key, value)
 
     def reset(self, vertices) -> None:
         """the same handle as DelaunayTree(vertices) again, its device buffers kept (vo_tri_reset)"""

@@ -1,72 +1,246 @@
-"""Autotuner for the round kernels: sweeps lanes per window point (fixed sizes and the
-per-round rule at several lane targets) and the initial window ratio on a workload, times
-each setting with the library's hipEvent clock, and writes the best setting per dimension to
-voronoids_amd/tuning.json, which DelaunayTree applies when it exists.
+"""Autotuner of the round kernels (k_cavity_g / k_commit_g): build constants and runtime knobs.
 
-    python -m voronoids_amd.autotune --dim 3 --n 1000000
+Two kinds of choice, per dimension:
+  build    compile-time constants of voronoids_amd/csrc/kernels_group.hpp: the least waves per
+           SIMD each round kernel is compiled for, which sets its register budget
+           (VO_CAVITY_WAVES / VO_COMMIT_WAVES in 3D, VO_CAVITY_WAVES2 / VO_COMMIT_WAVES2 in 2D),
+           and the LDS list capacities per window point -- cavity entries and boundary facets
+           (VO_GROUP_KC3 / VO_GROUP_FC3, VO_GROUP_KC2 / VO_GROUP_FC2; a point whose cavity
+           outgrows them is inserted alone) -- with the bytes of facet tuples kept for ridge
+           matching (VO_MATCH_BYTES3).  Each candidate is its own build of the library
+           (voronoids_amd/variants/), timed in its own process.
+  runtime  vo_tri_config knobs of that build: lanes per window point (a fixed size, or the
+           per-round rule at another lane target), the window's initial ratio, the winner
+           fractions the large windows aim at, and rounds per submitted chunk.
+
+The search: every build candidate at the default runtime setting, then every runtime candidate on
+the fastest build.  A candidate counts only if its triangulation has no violation and as many
+simplices as the default's.  Times are medians of vo_tri_run's device time; the winner replaces
+the defaults only when it is more than 1% faster (run-to-run noise is about that).
+
+    python -m voronoids_amd.autotune build                      # compile the build candidates (hipcc)
+    python -m voronoids_amd.autotune sweep --dim 3 --n 4000000 --write
+    python -m voronoids_amd.autotune sweep --dim 2 --n 10000000 --write
+
+--write stores the winner per dimension in voronoids_amd/tuning.json: voronoids_amd/build.py
+compiles its build constants into the shipped library, and every DelaunayTree applies its runtime
+setting.
 """
 from __future__ import annotations
 
 import argparse
 import json
 import os
+import subprocess
 import sys
-
-import numpy as np
+import time
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 TUNING = os.path.join(HERE, "tuning.json")
+VARIANTS = os.path.join(HERE, "variants")
+
+# build constants and their defaults (kernels_group.hpp); a candidate changes one group of them
+BUILD_DEFAULTS = {
+    3: {"VO_CAVITY_WAVES": 3, "VO_COMMIT_WAVES": 4, "VO_GROUP_KC3": 192, "VO_GROUP_FC3": 256, "VO_MATCH_BYTES3": 4096},
+    2: {"VO_CAVITY_WAVES2": 3, "VO_COMMIT_WAVES2": 4, "VO_GROUP_KC2": 64, "VO_GROUP_FC2": 96},
+}
+BUILD_CANDIDATES = {
+    3: [{}, {"VO_CAVITY_WAVES": 2}, {"VO_CAVITY_WAVES": 4}, {"VO_COMMIT_WAVES": 3}, {"VO_COMMIT_WAVES": 5},
+        {"VO_GROUP_KC3": 160, "VO_GROUP_FC3": 224}, {"VO_GROUP_KC3": 256, "VO_GROUP_FC3": 320},
+        {"VO_MATCH_BYTES3": 2048}, {"VO_MATCH_BYTES3": 8192}],
+    2: [{}, {"VO_CAVITY_WAVES2": 2}, {"VO_CAVITY_WAVES2": 4}, {"VO_COMMIT_WAVES2": 3}, {"VO_COMMIT_WAVES2": 5},
+        {"VO_GROUP_KC2": 48, "VO_GROUP_FC2": 80}, {"VO_GROUP_KC2": 96, "VO_GROUP_FC2": 128}],
+}
 
 
-def candidates(dim: int) -> list[dict]:
-    out = [{"group_lanes": g} for g in (1, 4, 8, 16, 32)]
-    for lt in (1 << 16, 1 << 17, 1 << 18, 1 << 19, 1 << 20, 1 << 21):
-        out.append({"group_lanes": 0, "lanes_target": lt})
-    for gm in (100, 400):
-        out.append({"group_lanes": 0, "window_gamma_milli": gm})
+def runtime_candidates(dim: int) -> list[dict]:
+    out: list[dict] = [{}]
+    base_lt = 1 << 21 if dim == 3 else 1 << 16
+    for lt in (base_lt >> 2, base_lt >> 1, base_lt << 1, base_lt << 2):
+        out.append({"lanes_target": lt})
+    out += [{"group_lanes": g} for g in ((4, 8, 16) if dim == 3 else (4, 8))]
+    for gm in (100, 200, 400):
+        out.append({"window_gamma_milli": gm})
+    lo = 650 if dim == 3 else 550
+    for d in (-100, -50, 50):
+        out.append({"big_frac_lo_milli": lo + d})
+    out += [{"chunk_rounds": 2}, {"chunk_rounds": 8}]
     return out
 
 
-def time_setting(P: np.ndarray, setting: dict, reps: int) -> float:
-    import voronoids_amd as vo
-    best = float("inf")
-    for _ in range(reps):
-        t = vo.DelaunayTree(P)
-        for k, v in setting.items():
-            t.config(k, v)
-        t.stage(P, seed=0)
-        t.run()
-        best = min(best, t.stats()["ms_run"])
-        t.close()
-    return best
+def tag_of(defines: dict) -> str:
+    return "default" if not defines else "_".join(f"{k[3:].lower()}-{v}" for k, v in sorted(defines.items()))
 
 
-def main() -> int:
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--dim", type=int, default=3)
-    ap.add_argument("--n", type=int, default=1_000_000)
-    ap.add_argument("--reps", type=int, default=2)
-    ap.add_argument("--write", action="store_true", help="store the best setting in tuning.json")
-    args = ap.parse_args()
+def variant_path(dim: int, defines: dict) -> str:
+    return os.path.join(VARIANTS, f"libvoronoids_hip.d{dim}.{tag_of(defines)}.so")
+
+
+def tuned_build(dim: int) -> dict:
+    """build constants of the stored winner for one dimension ({} when none is stored)"""
+    if not os.path.exists(TUNING):
+        return {}
+    with open(TUNING) as f:
+        return dict(json.load(f).get(str(dim), {}).get("build", {}))
+
+
+def tuned_defines() -> tuple:
+    """"NAME=VALUE" for every stored build constant of both dimensions (the macros are disjoint)"""
+    out = []
+    for dim in (3, 2):
+        for k, v in sorted(tuned_build(dim).items()):
+            if k not in BUILD_DEFAULTS[dim]:
+                raise ValueError(f"tuning.json: {k} is not a {dim}D build constant")
+            out.append(f"{k}={int(v)}")
+    return tuple(out)
+
+
+def build_variants(dims=(3, 2), jobs: int = 4) -> list[str]:
+    """compile every build candidate (each with the other dimension's stored constants)"""
+    from concurrent.futures import ThreadPoolExecutor
+    from .build import build
+    os.makedirs(VARIANTS, exist_ok=True)
+    todo = []
+    for dim in dims:
+        other = 2 if dim == 3 else 3
+        for cand in BUILD_CANDIDATES[dim]:
+            defs = tuple(f"{k}={v}" for k, v in sorted({**tuned_build(other), **cand}.items()))
+            todo.append((variant_path(dim, cand), defs))
+
+    def one(item):
+        path, defs = item
+        t0 = time.time()
+        build(out=path, defines=defs, static=False, tuned=False)
+        print(json.dumps({"built": os.path.relpath(path, HERE), "defines": list(defs),
+                          "s": round(time.time() - t0, 1)}), flush=True)
+        return path
+    with ThreadPoolExecutor(max_workers=jobs) as ex:
+        return list(ex.map(one, todo))
+
+
+def trial(dim: int, n: int, reps: int, settings: list[dict]) -> list[dict]:
+    """time vo_tri_run for each runtime setting in this process's library build (VORONOIDS_LIB)"""
     sys.path.insert(0, os.path.dirname(HERE))
+    import numpy as np
+    import voronoids_amd as vo
     from voronoids_amd import datasets
-    P = datasets.uniform(args.n, args.dim, seed=0)
+    P = datasets.uniform(n, dim, seed=0)
+    out = []
+    for s in settings:
+        ms, sims, viol, err = [], None, None, None
+        try:
+            for r in range(reps):
+                t = vo.DelaunayTree(P, tuned=False)
+                for k, v in s.items():
+                    t.config(k, v)
+                t.stage(P, seed=0)
+                t.run()
+                ms.append(t.stats()["ms_run"])
+                if r == 0:
+                    sims = t.counts()[1]
+                    viol = t.violations()
+                t.close()
+        except vo.VoError as e:
+            err = str(e)
+        rec = {"setting": s, "ms": float(np.median(ms)) if ms and err is None else None, "runs": ms,
+               "simplices": sims, "violations": viol, "error": err}
+        print(json.dumps(rec), flush=True)
+        out.append(rec)
+    return out
+
+
+def _trial_in_process(lib: str, dim: int, n: int, reps: int, settings: list[dict], timeout: int) -> list[dict]:
+    env = dict(os.environ, VORONOIDS_LIB=lib)
+    cmd = [sys.executable, "-u", "-m", "voronoids_amd.autotune", "trial", "--dim", str(dim), "--n", str(n),
+           "--reps", str(reps), "--settings", json.dumps(settings)]
+    # the child's lines are echoed as they come, so a long sweep shows its progress
+    out = []
+    t0 = time.time()
+    with subprocess.Popen(cmd, env=env, cwd=os.path.dirname(HERE), stdout=subprocess.PIPE, text=True) as proc:
+        for line in proc.stdout:
+            if line.startswith("{"):
+                out.append(json.loads(line))
+                print(f"# {os.path.basename(lib)}: {line.strip()}", file=sys.stderr, flush=True)
+            if time.time() - t0 > timeout:
+                proc.kill()
+                raise RuntimeError(f"trial with {os.path.basename(lib)} passed {timeout} s")
+        rc = proc.wait()
+    if rc != 0:
+        raise RuntimeError(f"trial with {os.path.basename(lib)} failed with exit status {rc}")
+    return out
+
+
+def pick(candidates: list[dict], margin: float) -> dict:
+    """the fastest valid candidate if it beats the default build at the default setting (the
+    first candidate) by more than `margin` (a fraction: run-to-run noise), else that default"""
+    default = candidates[0]
+    best = min((c for c in candidates if c["valid"]), key=lambda c: c["ms"])
+    return best if best["ms"] < default["ms"] * (1.0 - margin) else default
+
+
+def sweep(dim: int, n: int, reps: int, write: bool, timeout: int = 300, margin: float = 0.01) -> dict:
     results = []
-    for s in candidates(args.dim):
-        ms = time_setting(P, s, args.reps)
-        results.append((ms, s))
-        print(json.dumps({"ms": round(ms, 3), **s}), flush=True)
-    results.sort(key=lambda r: r[0])
-    best_ms, best = results[0]
-    print(json.dumps({"best": best, "ms": best_ms, "dim": args.dim, "n": args.n}), flush=True)
-    if args.write:
+
+    def ok(rec: dict, ref) -> bool:
+        return rec["ms"] is not None and rec["violations"] == 0 and rec["simplices"] == ref
+    # 1. every build at the default runtime setting
+    ref_sims = None
+    builds = []
+    for cand in BUILD_CANDIDATES[dim]:
+        lib = variant_path(dim, cand)
+        if not os.path.exists(lib):
+            raise FileNotFoundError(f"{lib}: run `python -m voronoids_amd.autotune build` first")
+        rec = _trial_in_process(lib, dim, n, reps, [{}], timeout)[0]
+        if ref_sims is None:
+            ref_sims = rec["simplices"]
+        rec.update(build=cand, valid=ok(rec, ref_sims))
+        print(json.dumps({"dim": dim, "build": tag_of(cand), "ms": rec["ms"], "valid": rec["valid"]}), flush=True)
+        results.append(rec)
+        if rec["valid"]:
+            builds.append((rec["ms"], cand))
+    default_ms = results[0]["ms"]
+    best_build = min(builds, key=lambda b: b[0])[1]
+    # 2. every runtime setting on the fastest build
+    for rec in _trial_in_process(variant_path(dim, best_build), dim, n, reps, runtime_candidates(dim), timeout):
+        rec.update(build=best_build, valid=ok(rec, ref_sims))
+        print(json.dumps({"dim": dim, "build": tag_of(best_build), "setting": rec["setting"], "ms": rec["ms"],
+                          "valid": rec["valid"]}), flush=True)
+        results.append(rec)
+    candidates = [{"build": r["build"], "setting": r["setting"], "ms": r["ms"], "valid": r["valid"]} for r in results]
+    best = pick(candidates, margin)
+    summary = {"build": best["build"], "setting": best["setting"], "ms": best["ms"], "default_ms": default_ms,
+               "margin": margin, "n": n, "reps": reps, "workload": f"{dim}D uniform n={n}, seed 0",
+               "candidates": candidates}
+    print(json.dumps({"dim": dim, "best": {k: summary[k] for k in ("build", "setting", "ms", "default_ms")}}), flush=True)
+    if write:
         table = {}
         if os.path.exists(TUNING):
             with open(TUNING) as f:
                 table = json.load(f)
-        table[str(args.dim)] = {"setting": best, "ms": best_ms, "n": args.n}
+        table[str(dim)] = summary
         with open(TUNING, "w") as f:
             json.dump(table, f, indent=1, sort_keys=True)
+            f.write("\n")
+    return summary
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("cmd", choices=("build", "sweep", "trial"))
+    ap.add_argument("--dim", type=int, default=3)
+    ap.add_argument("--n", type=int, default=None)
+    ap.add_argument("--reps", type=int, default=3)
+    ap.add_argument("--settings", default="[{}]", help="trial: JSON list of runtime settings")
+    ap.add_argument("--write", action="store_true", help="sweep: store the winner in tuning.json")
+    ap.add_argument("--jobs", type=int, default=4)
+    args = ap.parse_args()
+    n = args.n or (4_000_000 if args.dim == 3 else 10_000_000)
+    if args.cmd == "build":
+        build_variants(jobs=args.jobs)
+    elif args.cmd == "trial":
+        trial(args.dim, n, args.reps, json.loads(args.settings))
+    else:
+        sweep(args.dim, n, args.reps, args.write)
     return 0
 
 

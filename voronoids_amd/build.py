@@ -38,11 +38,17 @@ def _run(cmd: list, verbose: bool) -> None:
         raise RuntimeError("build step failed: " + os.path.basename(cmd[0]))
 
 
-def build(verbose: bool = False, out: str = LIB, defines: tuple = (), static: bool = True) -> str:
+def build(verbose: bool = False, out: str = LIB, defines: tuple = (), static: bool = True, tuned: bool = True) -> str:
     """compile the sources to objects (in parallel), link the shared library and, with static,
-    archive the same objects next to it; `defines` ("NAME=VALUE", ...) select tuning variants"""
+    archive the same objects next to it.  `defines` ("NAME=VALUE", ...) set round-kernel build
+    constants; with tuned, the autotuner's stored constants (tuning.json) come first and `defines`
+    override them."""
     stem = out[:-3] if out.endswith(".so") else out
     objs = [f"{stem}.{os.path.splitext(os.path.basename(s))[0]}.o" for s in SRC]
+    if tuned:
+        from .autotune import tuned_defines
+        given = {d.split("=", 1)[0] for d in defines}
+        defines = tuple(d for d in tuned_defines() if d.split("=", 1)[0] not in given) + tuple(defines)
     dflags = [f"-D{d}" for d in defines]
     with ThreadPoolExecutor(max_workers=len(SRC)) as ex:
         list(ex.map(lambda so: _run([HIPCC, *FLAGS, *dflags, "-c", "-o", so[1], so[0]], verbose), zip(SRC, objs)))

@@ -44,7 +44,10 @@ template <int G> struct Group {
 #define VO_GROUP_FC3 256
 #endif
 #ifndef VO_COMMIT_WAVES
-#define VO_COMMIT_WAVES 4
+#define VO_COMMIT_WAVES 4    // 3D
+#endif
+#ifndef VO_COMMIT_WAVES2
+#define VO_COMMIT_WAVES2 4   // 2D
 #endif
 #ifndef VO_CAVITY_WAVES2
 #define VO_CAVITY_WAVES2 3   // 2D (the kernel reaches 4 waves at its natural register count)
@@ -52,8 +55,14 @@ template <int G> struct Group {
 #ifndef VO_CAVITY_WAVES
 #define VO_CAVITY_WAVES 3
 #endif
-template <int D> constexpr int kGroupKC = D == 3 ? VO_GROUP_KC3 : 64;    // cavity entries per point
-template <int D> constexpr int kGroupFC = D == 3 ? VO_GROUP_FC3 : 96;    // boundary facets per point
+#ifndef VO_GROUP_KC2
+#define VO_GROUP_KC2 64
+#endif
+#ifndef VO_GROUP_FC2
+#define VO_GROUP_FC2 96
+#endif
+template <int D> constexpr int kGroupKC = D == 3 ? VO_GROUP_KC3 : VO_GROUP_KC2;    // cavity entries per point
+template <int D> constexpr int kGroupFC = D == 3 ? VO_GROUP_FC3 : VO_GROUP_FC2;    // boundary facets per point
 // Cavity members whose adjacency the breadth-first search keeps in LDS: it arrives with the
 // member's coordinates when the member is tested (one run of loads), so expanding the member
 // costs no further request; members past this index read their adjacency from the record.
@@ -439,7 +448,7 @@ __device__ __forceinline__ bool outside_next_star(const Ctx& c, int t, int i, in
 }
 
 template <int D, int G>
-__global__ void __launch_bounds__(64, VO_COMMIT_WAVES) k_commit_g(Ctx c, int nwin, unsigned long long key_hi) {
+__global__ void __launch_bounds__(64, D == 3 ? VO_COMMIT_WAVES : VO_COMMIT_WAVES2) k_commit_g(Ctx c, int nwin, unsigned long long key_hi) {
     constexpr int M = D + 1;
     constexpr int NG = Group<G>::NG;
     constexpr int FC = kGroupFC<D>;
