@@ -118,6 +118,8 @@ struct Ctx {
     int* big;              // points to re-run alone with large cavity buffers
     int* cav;              // killed lists, one run per window slot: cav[slot * kcap + k]
     int* fac;              // boundary facets (t << 2 | face), likewise: fac[slot * fcap + k]
+    uint4* probe = nullptr;  // builds with VO_POINT_PROBE (diagnostics): three entries per window slot
+                             // written by the lane-group cavity kernel (times, walk, hops, sizes)
     int* fnb;              // lane-group kernels: the adjacency entry across each facet as the cavity
                            // search saw it (the simplex outside << 2 | its face, -1 on the hull),
                            // parallel to fac; the commit reads it instead of the record, which
@@ -732,8 +734,26 @@ template <int D> __device__ __forceinline__ int hint_lookup(const HintView& v, t
     return best;
 }
 
-template <int D> __device__ __forceinline__ void hint_store(const HintView& v, typename VT<D>::F pf, int t) {
-    for (int l = 1; l <= v.levels; ++l) v.h[hint_cell<D>(v, pf, l)] = t;
+// A walk that ended in simplex t (vertex coordinates a) leaves t as the hint of pf's cells, at
+// the levels whose cells are at least as wide as t (level 1 always).  A finer cell written while
+// the simplices around it were much larger than the cell would hand a point that arrives there
+// much later a stale simplex: the forward chain from it follows one branch of that simplex's
+// history and can end far from the point, and the point's walk is then long.
+template <int D>
+__device__ __forceinline__ void hint_store(const HintView& v, typename VT<D>::F pf, const typename VT<D>::F* a, int t) {
+    float e2 = 0.f;
+#pragma unroll
+    for (int k = 0; k <= D; ++k) {
+        const float dx = a[k].x - pf.x, dy = a[k].y - pf.y;
+        float d2 = dx * dx + dy * dy;
+        if constexpr (D == 3) d2 += (a[k].z - pf.z) * (a[k].z - pf.z);
+        e2 = fmaxf(e2, d2);
+    }
+    // the cell width of level l is 1 / (ginv 2^l)
+    const float w = v.ginv * sqrtf(e2);
+    const int lw = w > 0.f ? (int)floorf(-log2f(w)) : v.levels;
+    const int lmax = min(max(lw, 1), v.levels);
+    for (int l = 1; l <= lmax; ++l) v.h[hint_cell<D>(v, pf, l)] = t;
 }
 
 __device__ __forceinline__ uint32_t mix32(uint32_t x) {
@@ -804,7 +824,7 @@ __global__ void __launch_bounds__(256) k_cavity(Ctx c, int nwin, unsigned long l
             if (!moved) break;
         }
         c.cp[p] = t;
-        hint_store<D>(hv, pf, t);
+        hint_store<D>(hv, pf, a, t);
         int nk = 0, nf = 0;
         if (st == ST_OK) {
             const int s0 = conflict_at<D>(c, a, t, pv, pf, lc);

@@ -123,19 +123,47 @@ __global__ void __launch_bounds__(64, D == 3 ? VO_CAVITY_WAVES : VO_CAVITY_WAVES
     F pf{};
     int4 tt = make_int4(0, 0, 0, 0);
     unsigned long long key = 0;
+#ifdef VO_POINT_PROBE
+    const unsigned long long pr_t0 = wall_clock64();
+    unsigned pr_walk = 0, pr_hops = 0, pr_tw = 0, pr_src = 0;
+    float pr_d0 = 0.f, pr_dh = 0.f;
+#endif
     if (st == ST_OK) {
         p = win_point(wv, slot);
         pv = c.M + p;
         key = key_hi | (unsigned)p;
         pf = V::ld(c.X32, pv);
         t = c.cp[p];
+#ifdef VO_POINT_PROBE
+        if (t < 0) {
+            const HintView hv = hint_view(c, p);
+            int lv = 0;
+            for (int l = 1; l <= hv.levels; ++l)
+                if (hv.h[hint_cell<D>(hv, pf, l)] >= 0) lv = l;
+            pr_src = 1 + lv;
+        }
+#endif
         if (t < 0) t = hint_lookup<D>(hint_view(c, p), pf);
+#ifdef VO_POINT_PROBE
+        {
+            F a0[4];
+            int4 aa0;
+            ld_ax<D>(c, t, aa0, a0);
+            const float dx = a0[0].x - pf.x, dy = a0[0].y - pf.y;
+            float d2 = dx * dx + dy * dy;
+            if constexpr (D == 3) d2 += (a0[0].z - pf.z) * (a0[0].z - pf.z);
+            pr_dh = sqrtf(d2);
+        }
+#endif
         tt = ld_v<D>(c, t);
         for (int hops = 0; tt.x < 0; ++hops) {
             if (hops > kChainLimit) { if (lead) atomicOr(&c.st->err, ERR_CHAIN); st = ST_BROKEN; break; }
             t = tt.y;
             tt = ld_v<D>(c, t);
             if (lead) lc.chain++;
+#ifdef VO_POINT_PROBE
+            pr_hops++;
+#endif
         }
     }
     // ---- walk: lane k of the group tests face (r + k) mod M
@@ -145,6 +173,14 @@ __global__ void __launch_bounds__(64, D == 3 ? VO_CAVITY_WAVES : VO_CAVITY_WAVES
     F a[4];
     int4 aa = make_int4(-1, -1, -1, -1);
     if (walking) ld_ax<D>(c, t, aa, a);
+#ifdef VO_POINT_PROBE
+    if (walking) {
+        const float dx = a[0].x - pf.x, dy = a[0].y - pf.y;
+        float d2 = dx * dx + dy * dy;
+        if constexpr (D == 3) d2 += (a[0].z - pf.z) * (a[0].z - pf.z);
+        pr_d0 = sqrtf(d2);
+    }
+#endif
     for (int step = 0; __any(walking); ++step) {
         int s = 1, nb = -1;
         bool test = false;
@@ -166,6 +202,9 @@ __global__ void __launch_bounds__(64, D == 3 ? VO_CAVITY_WAVES : VO_CAVITY_WAVES
         const int nb_sel = g.bcast(nb, first);
         if (walking) {
             if (lead) lc.walk++;
+#ifdef VO_POINT_PROBE
+            pr_walk++;
+#endif
             if (und) {
                 st = ST_DEFER;
                 walking = false;
@@ -185,9 +224,12 @@ __global__ void __launch_bounds__(64, D == 3 ? VO_CAVITY_WAVES : VO_CAVITY_WAVES
     if (st == ST_OK || st == ST_DEFER) {
         if (lead) {
             c.cp[p] = t;
-            hint_store<D>(hint_view(c, p), pf, t);
+            hint_store<D>(hint_view(c, p), pf, a, t);
         }
     }
+#ifdef VO_POINT_PROBE
+    pr_tw = (unsigned)(wall_clock64() - pr_t0);
+#endif
     // ---- the containing simplex must conflict (else p repeats one of its vertices)
     int s0 = 0;
     if (st == ST_OK && lead) s0 = conflict_at<D>(c, a, t, pv, pf, lc);
@@ -296,6 +338,14 @@ __global__ void __launch_bounds__(64, D == 3 ? VO_CAVITY_WAVES : VO_CAVITY_WAVES
         c.nk[slot] = nk;
         c.nf[slot] = nf;
         c.status[slot] = st;
+#ifdef VO_POINT_PROBE
+        if (c.probe) {
+            const unsigned long long pr_t1 = wall_clock64();
+            c.probe[3 * (size_t)slot] = make_uint4((unsigned)pr_t0, (unsigned)(pr_t0 >> 32), pr_tw, (unsigned)(pr_t1 - pr_t0));
+            c.probe[3 * (size_t)slot + 1] = make_uint4(pr_walk, pr_hops, (unsigned)nk, (unsigned)st | (unsigned)nf << 8);
+            c.probe[3 * (size_t)slot + 2] = make_uint4(pr_src, (unsigned)p, __float_as_uint(pr_d0), __float_as_uint(pr_dh));
+        }
+#endif
     }
     __builtin_amdgcn_wave_barrier();       // the LDS lists are reused by the next slots
     }

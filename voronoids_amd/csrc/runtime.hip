@@ -129,6 +129,10 @@ struct vo_tri {
     int cur = 0, npend = 0;
     int* cav = nullptr;
     int* fac = nullptr;
+#ifdef VO_POINT_PROBE
+    uint4* probe = nullptr;            // per-point probe of the cavity kernel (diagnostic builds)
+    FILE* probe_f = nullptr;
+#endif
     int* fnb = nullptr;                // the outside link of each facet, as the cavity search saw it
     int* nk = nullptr;
     int* nf = nullptr;
@@ -506,6 +510,10 @@ void free_all(vo_tri* t) {
                     t->big, t->cav1, t->fac1, t->misc1, t->plan, t->fix};
     for (void* p : ptrs)
         if (p) hipFree(p);
+#ifdef VO_POINT_PROBE
+    if (t->probe) hipFree(t->probe);
+    if (t->probe_f) std::fclose(t->probe_f);
+#endif
     if (t->st_host) hipHostFree(t->st_host);
     if (t->plan_host) hipHostFree(t->plan_host);
     for (auto& e : t->ev)
@@ -899,6 +907,15 @@ int run_host_planned(vo_tri* t, FILE* trace) {
         hipLaunchKernelGGL(k_reset_round, dim3(1), dim3(1), 0, t->s, t->st);
         const unsigned long long key_hi = (unsigned long long)(0xFFFFFFFFu - (uint32_t)t->rounds) << 32;
         Ctx c = make_ctx(t);
+#ifdef VO_POINT_PROBE
+        // VO_PROBE=<path>: per round a header (round, nwin, 0, 0) and three uint4 per window slot
+        if (const char* pp = std::getenv("VO_PROBE")) {
+            if (!t->probe && (rc = dev_alloc(&t->probe, 3 * (size_t)t->win_cap))) return rc;
+            if (!t->probe_f) t->probe_f = std::fopen(pp, "ab");
+            VO_HIP(hipMemsetAsync(t->probe, 0, sizeof(uint4) * 3 * (size_t)nwin, t->s));
+            c.probe = t->probe;
+        }
+#endif
         if (t->xcd) {
             if (!t->reg_meta && ((rc = dev_alloc(&t->reg_meta, 24)) || (rc = dev_alloc(&t->win2, t->win_cap)) ||
                                  (rc = dev_alloc(&t->regs, t->win_cap))))
@@ -925,6 +942,14 @@ int run_host_planned(vo_tri* t, FILE* trace) {
         t->ms_com += b;
         t->n_cav++;
         t->n_com++;
+#ifdef VO_POINT_PROBE
+        if (c.probe && t->probe_f) {
+            std::vector<uint4> h(3 * (size_t)nwin + 1);
+            h[0] = make_uint4((unsigned)t->rounds, (unsigned)nwin, 0u, 0u);
+            VO_HIP(hipMemcpy(h.data() + 1, t->probe, sizeof(uint4) * 3 * (size_t)nwin, hipMemcpyDeviceToHost));
+            std::fwrite(h.data(), sizeof(uint4), h.size(), t->probe_f);
+        }
+#endif
         if ((rc = finish_round(t, nwin, nfresh, trace, a, b))) return rc;
         if (t->compact_ratio > 0 && t->n_tets >= t->compact_min && (double)t->n_tets > t->compact_ratio * (double)t->n_live &&
             (rc = compact(t)))
