@@ -136,7 +136,8 @@ int vo_tri_stats(const vo_tri* t, vo_stats* out);
  * and FP64-filter (2) predicate tiers, 3 by default; 0 sends every predicate to the exact
  * expansion tier.  "group_lanes" = lanes cooperating on one window point in the round
  * kernels (1, 4, 8, 16, 32; 0 = chosen per round: the largest size with window x size <=
- * "lanes_target", default 2^21 in 3D and 2^16 in 2D, and at least 4; 1 = one thread per point).  "window_gamma_milli" = initial window
+ * "lanes_target", default 2^21 in 3D and 2^16 in 2D, and at least 4; 1 = one thread per point;
+ * "commit_lanes_target", default 0 = "lanes_target", is the commit kernel's own target).  "window_gamma_milli" = initial window
  * size per inserted point x 1000.  "local_order" = 0 (default: the seeded random priority order) or k > 0
  * (the same order with consecutive chunks of position/k points sorted along the Morton curve; set before
  * vo_tri_stage).  "big_window" (default 16384 in 3D, 65536 in 2D; 0 = off), "big_frac_lo_milli" /

@@ -10,11 +10,12 @@ Two kinds of choice, per dimension:
            matching (VO_MATCH_BYTES3).  Each candidate is its own build of the library
            (voronoids_amd/variants/), timed in its own process.
   runtime  vo_tri_config knobs of that build: lanes per window point (a fixed size, or the
-           per-round rule at another lane target), the window's initial ratio, the winner
-           fractions the large windows aim at, and rounds per submitted chunk.
+           per-round rule at another lane target, for both round kernels or the commit alone),
+           the window's initial ratio, the winner fractions the large and the small windows aim
+           at, the least window, and rounds per submitted chunk.
 
 The search: every build candidate at the default runtime setting, then every runtime candidate on
-the fastest build.  A candidate counts only if its triangulation has no violation and as many
+the fastest build, then the union of the runtime settings that each gained on their own.  A candidate counts only if its triangulation has no violation and as many
 simplices as the default's.  Times are medians of vo_tri_run's device time; the winner replaces
 the defaults only when it is more than 1% faster (run-to-run noise is about that).
 
@@ -65,6 +66,11 @@ def runtime_candidates(dim: int) -> list[dict]:
     for d in (-100, -50, 50):
         out.append({"big_frac_lo_milli": lo + d})
     out += [{"chunk_rounds": 2}, {"chunk_rounds": 8}]
+    for lt in (base_lt >> 1, base_lt << 1, base_lt << 2):
+        out.append({"commit_lanes_target": lt})
+    out += [{"small_frac_lo_milli": 250, "small_frac_hi_milli": 500}, {"small_frac_lo_milli": 200, "small_frac_hi_milli": 450},
+            {"small_frac_lo_milli": 450, "small_frac_hi_milli": 700}]
+    out += [{"min_window": 128}, {"min_window": 256}]
     return out
 
 
@@ -201,11 +207,27 @@ def sweep(dim: int, n: int, reps: int, write: bool, timeout: int = 300, margin: 
     default_ms = results[0]["ms"]
     best_build = min(builds, key=lambda b: b[0])[1]
     # 2. every runtime setting on the fastest build
+    stage2 = []
     for rec in _trial_in_process(variant_path(dim, best_build), dim, n, reps, runtime_candidates(dim), timeout):
         rec.update(build=best_build, valid=ok(rec, ref_sims))
         print(json.dumps({"dim": dim, "build": tag_of(best_build), "setting": rec["setting"], "ms": rec["ms"],
                           "valid": rec["valid"]}), flush=True)
         results.append(rec)
+        stage2.append(rec)
+    # 3. the union of the runtime settings that each beat that build's default by more than the
+    # margin (those that touch different knobs), tried once more together
+    base_ms = stage2[0]["ms"]
+    gains = sorted((r for r in stage2[1:] if r["valid"] and r["ms"] < base_ms * (1.0 - margin)), key=lambda r: r["ms"])
+    merged: dict = {}
+    for r in gains:
+        if not set(r["setting"]) & set(merged):
+            merged.update(r["setting"])
+    if len(gains) > 1 and merged != gains[0]["setting"]:
+        for rec in _trial_in_process(variant_path(dim, best_build), dim, n, reps, [merged], timeout):
+            rec.update(build=best_build, valid=ok(rec, ref_sims))
+            print(json.dumps({"dim": dim, "build": tag_of(best_build), "setting": rec["setting"], "ms": rec["ms"],
+                              "valid": rec["valid"]}), flush=True)
+            results.append(rec)
     candidates = [{"build": r["build"], "setting": r["setting"], "ms": r["ms"], "valid": r["valid"]} for r in results]
     best = pick(candidates, margin)
     summary = {"build": best["build"], "setting": best["setting"], "ms": best["ms"], "default_ms": default_ms,

@@ -119,7 +119,8 @@ struct Ctx {
     int* cav;              // killed lists, one run per window slot: cav[slot * kcap + k]
     int* fac;              // boundary facets (t << 2 | face), likewise: fac[slot * fcap + k]
     uint4* probe = nullptr;  // builds with VO_POINT_PROBE (diagnostics): three entries per window slot
-                             // written by the lane-group cavity kernel (times, walk, hops, sizes)
+                             // from the lane-group cavity kernel (times, walk, hops, sizes), then
+                             // from 3 win_cap on two per slot from the lane-group commit kernel
     int* fnb;              // lane-group kernels: the adjacency entry across each facet as the cavity
                            // search saw it (the simplex outside << 2 | its face, -1 on the hull),
                            // parallel to fac; the commit reads it instead of the record, which

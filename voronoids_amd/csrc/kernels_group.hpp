@@ -522,6 +522,9 @@ __global__ void __launch_bounds__(64, D == 3 ? VO_COMMIT_WAVES : VO_COMMIT_WAVES
     int st = ST_IDLE, p = 0, nk = 0, nf = 0;
     unsigned long long key = 0;
     bool ok = true, adj = false;
+#ifdef VO_POINT_PROBE
+    const unsigned long long pr_t0 = wall_clock64();
+#endif
     if (valid) {
         st = c.status[slot];
         p = win_point(wv, slot);
@@ -761,6 +764,16 @@ __global__ void __launch_bounds__(64, D == 3 ? VO_COMMIT_WAVES : VO_COMMIT_WAVES
             lc.maxf = nf > (int)lc.maxf ? nf : lc.maxf;
         }
     }
+#ifdef VO_POINT_PROBE
+    if (c.probe && valid && lead) {
+        const unsigned long long pr_t1 = wall_clock64();
+        const unsigned flags = (unsigned)st | (unsigned)win << 4 | (unsigned)build << 5 | (unsigned)(nf > MC) << 6 |
+                               (unsigned)need << 7 | (unsigned)nsh << 8;
+        uint4* q = c.probe + 3 * (size_t)c.win_cap + 2 * (size_t)slot;
+        q[0] = make_uint4((unsigned)pr_t0, (unsigned)(pr_t0 >> 32), (unsigned)(pr_t1 - pr_t0), flags);
+        q[1] = make_uint4((unsigned)nk, (unsigned)nf, 0u, 0u);
+    }
+#endif
     // cavities too large for the LDS lists are re-run alone after the round
     const bool big = lead && st == ST_OVERFLOW;
     bool again = lead && ((st == ST_OK && !build) || st == ST_DEFER || st == ST_LOST);

@@ -187,6 +187,7 @@ struct vo_tri {
     int local_order = 0;               // priority order: random, Morton-sorted in chunks of position/this (0: off)
     int group = 0;                     // lanes per window point (0 = chosen per round, 1 = one thread)
     int64_t lanes_target = 0;          // auto group size: largest G with window * G <= this (0 = default)
+    int64_t commit_lanes_target = 0;   // the same for the commit kernel (0 = lanes_target)
     int64_t inserted = 0, dups = 0, rounds = 0, windows_total = 0;
     int64_t rounds0 = 0;               // rounds before the last vo_tri_reset (keys keep decreasing)
     double ms_run = 0, ms_cav = 0, ms_com = 0, ms_ex = 0, ms_other = 0;
@@ -697,7 +698,8 @@ int compact(vo_tri* t) {
 // rotations over 64 lanes): measured faster than 32 lanes for the commit, slower for the cavity.
 int auto_group(const vo_tri* t, int nwin, int which) {
     if (t->group) return t->group;
-    const int64_t target = t->lanes_target ? t->lanes_target : (t->D == 3 ? (1 << 21) : (1 << 16));
+    const int64_t base = t->lanes_target ? t->lanes_target : (t->D == 3 ? (1 << 21) : (1 << 16));
+    const int64_t target = which && t->commit_lanes_target ? t->commit_lanes_target : base;
     static const int sizes[] = {64, 32, 16, 8};
     const int gmax = t->D == 3 ? (which ? 64 : 32) : 16;
     for (int g : sizes)
@@ -908,11 +910,12 @@ int run_host_planned(vo_tri* t, FILE* trace) {
         const unsigned long long key_hi = (unsigned long long)(0xFFFFFFFFu - (uint32_t)t->rounds) << 32;
         Ctx c = make_ctx(t);
 #ifdef VO_POINT_PROBE
-        // VO_PROBE=<path>: per round a header (round, nwin, 0, 0) and three uint4 per window slot
+        // VO_PROBE=<path>: per round a header (round, nwin, 0, 0), three uint4 per window slot from
+        // the cavity kernel and two from the commit kernel
         if (const char* pp = std::getenv("VO_PROBE")) {
-            if (!t->probe && (rc = dev_alloc(&t->probe, 3 * (size_t)t->win_cap))) return rc;
+            if (!t->probe && (rc = dev_alloc(&t->probe, 5 * (size_t)t->win_cap))) return rc;
             if (!t->probe_f) t->probe_f = std::fopen(pp, "ab");
-            VO_HIP(hipMemsetAsync(t->probe, 0, sizeof(uint4) * 3 * (size_t)nwin, t->s));
+            VO_HIP(hipMemsetAsync(t->probe, 0, sizeof(uint4) * 5 * (size_t)t->win_cap, t->s));
             c.probe = t->probe;
         }
 #endif
@@ -944,9 +947,11 @@ int run_host_planned(vo_tri* t, FILE* trace) {
         t->n_com++;
 #ifdef VO_POINT_PROBE
         if (c.probe && t->probe_f) {
-            std::vector<uint4> h(3 * (size_t)nwin + 1);
+            std::vector<uint4> h(5 * (size_t)nwin + 1);
             h[0] = make_uint4((unsigned)t->rounds, (unsigned)nwin, 0u, 0u);
             VO_HIP(hipMemcpy(h.data() + 1, t->probe, sizeof(uint4) * 3 * (size_t)nwin, hipMemcpyDeviceToHost));
+            VO_HIP(hipMemcpy(h.data() + 1 + 3 * (size_t)nwin, t->probe + 3 * (size_t)t->win_cap,
+                             sizeof(uint4) * 2 * (size_t)nwin, hipMemcpyDeviceToHost));
             std::fwrite(h.data(), sizeof(uint4), h.size(), t->probe_f);
         }
 #endif
@@ -2000,6 +2005,9 @@ int vo_tri_config(vo_tri* t, const char* key, int64_t value) {
     } else if (k == "lanes_target") {
         if (value < 0) return fail(VO_ERR_INVALID, "lanes_target must be >= 0");
         t->lanes_target = value;
+    } else if (k == "commit_lanes_target") {
+        if (value < 0) return fail(VO_ERR_INVALID, "commit_lanes_target must be >= 0");
+        t->commit_lanes_target = value;
     } else if (k == "window_gamma_milli") {
         if (value <= 0) return fail(VO_ERR_INVALID, "window_gamma_milli must be positive");
         t->gamma = (double)value / 1000.0;
