@@ -287,11 +287,22 @@ def run_served(seed: int = 0, load: float = 0.8, target_s: float = 8.0) -> dict:
 
     def serve_plain(arrival_times):
         """the plain path as a FIFO server in real time: requests queue as they arrive, one
-        worker serves them one at a time"""
+        worker serves them one at a time; the output check (hashing each result) runs on a
+        thread of its own, as the server's does, so that it does not delay the next request"""
         q = queue.Queue()
         th, err = drive(arrival_times, lambda k: q.put((k, time.perf_counter())))
         hashes, lat, svc = {}, {}, {}
         first, last = None, None
+        hq = queue.Queue()
+
+        def hasher():
+            while True:
+                item = hq.get()
+                if item is None:
+                    return
+                hashes[item[0]] = B.simplex_set_hash(item[1])
+        hth = threading.Thread(target=hasher, daemon=True)
+        hth.start()
         for _ in range(len(arrival_times)):
             k, ta = q.get()
             t0 = time.perf_counter()
@@ -299,9 +310,11 @@ def run_served(seed: int = 0, load: float = 0.8, target_s: float = 8.0) -> dict:
             td = time.perf_counter()
             svc[k] = td - t0
             lat[k] = td - ta
-            hashes[k] = B.simplex_set_hash(s)
+            hq.put((k, s))
             first = ta if first is None else min(first, ta)
             last = td
+        hq.put(None)
+        hth.join()
         th.join()
         if err:
             raise err[0]
