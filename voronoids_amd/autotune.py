@@ -7,7 +7,8 @@ Two kinds of choice, per dimension:
            and the LDS list capacities per window point -- cavity entries and boundary facets
            (VO_GROUP_KC3 / VO_GROUP_FC3, VO_GROUP_KC2 / VO_GROUP_FC2; a point whose cavity
            outgrows them is inserted alone) -- with the bytes of facet tuples kept for ridge
-           matching (VO_MATCH_BYTES3).  Each candidate is its own build of the library
+           matching (VO_MATCH_BYTES3, and VO_MATCH_MIN3: the least tuples per point, which gives
+           the small lane groups more).  Each candidate is its own build of the library
            (voronoids_amd/variants/), timed in its own process.
   runtime  vo_tri_config knobs of that build: lanes per window point (a fixed size, or the
            per-round rule at another lane target, for both round kernels or the commit alone),
@@ -42,13 +43,14 @@ VARIANTS = os.path.join(HERE, "variants")
 
 # build constants and their defaults (kernels_group.hpp); a candidate changes one group of them
 BUILD_DEFAULTS = {
-    3: {"VO_CAVITY_WAVES": 3, "VO_COMMIT_WAVES": 4, "VO_GROUP_KC3": 192, "VO_GROUP_FC3": 256, "VO_MATCH_BYTES3": 4096},
+    3: {"VO_CAVITY_WAVES": 3, "VO_COMMIT_WAVES": 4, "VO_GROUP_KC3": 192, "VO_GROUP_FC3": 256, "VO_MATCH_BYTES3": 4096,
+        "VO_MATCH_MIN3": 0},
     2: {"VO_CAVITY_WAVES2": 3, "VO_COMMIT_WAVES2": 4, "VO_GROUP_KC2": 64, "VO_GROUP_FC2": 96},
 }
 BUILD_CANDIDATES = {
     3: [{}, {"VO_CAVITY_WAVES": 2}, {"VO_CAVITY_WAVES": 4}, {"VO_COMMIT_WAVES": 3}, {"VO_COMMIT_WAVES": 5},
         {"VO_GROUP_KC3": 160, "VO_GROUP_FC3": 224}, {"VO_GROUP_KC3": 256, "VO_GROUP_FC3": 320},
-        {"VO_MATCH_BYTES3": 2048}, {"VO_MATCH_BYTES3": 8192}],
+        {"VO_MATCH_BYTES3": 2048}, {"VO_MATCH_BYTES3": 8192}, {"VO_MATCH_MIN3": 64}],
     2: [{}, {"VO_CAVITY_WAVES2": 2}, {"VO_CAVITY_WAVES2": 4}, {"VO_COMMIT_WAVES2": 3}, {"VO_COMMIT_WAVES2": 5},
         {"VO_GROUP_KC2": 48, "VO_GROUP_FC2": 80}, {"VO_GROUP_KC2": 96, "VO_GROUP_FC2": 128}],
 }

@@ -668,28 +668,34 @@ __device__ __forceinline__ void star_record(const typename VT<D>::F* ox, int i, 
 
 // Coordinates of the new simplex replace_first(v(t), i, p) written to record N straight from
 // t's packed record (3D: scalar selects on the packed floats, no private array)
+// 3D: the packed coordinates (f0, f1, f2) of a killed simplex with vertex i replaced by pf, in
+// replace_first's vertex order, stored as simplex N's packed coordinates
+__device__ __forceinline__ void store_star_coords3(const Ctx& c, float4 f0, float4 f1, float4 f2, int i, float4 pf,
+                                                   int N) {
+    // old vertex k = (X[3k], X[3k+1], X[3k+2]) of the packed floats
+    const float X[12] = {f0.x, f0.y, f0.z, f0.w, f1.x, f1.y, f1.z, f1.w, f2.x, f2.y, f2.z, f2.w};
+    // new vertices 1..3 come from old vertices s1, s2, s3 (replace_first's even permutation)
+    const int s1 = i == 0 ? 1 : 0;
+    const int s2 = i == 0 ? 2 : (i == 1 ? 3 : (i == 2 ? 1 : 2));
+    const int s3 = i == 0 ? 3 : (i == 1 ? 2 : (i == 2 ? 3 : 1));
+    float o[9];
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+        o[j] = s1 == 0 ? X[j] : (s1 == 1 ? X[3 + j] : (s1 == 2 ? X[6 + j] : X[9 + j]));
+        o[3 + j] = s2 == 0 ? X[j] : (s2 == 1 ? X[3 + j] : (s2 == 2 ? X[6 + j] : X[9 + j]));
+        o[6 + j] = s3 == 0 ? X[j] : (s3 == 1 ? X[3 + j] : (s3 == 2 ? X[6 + j] : X[9 + j]));
+    }
+    float4* w = reinterpret_cast<float4*>(rp<3>(c, N) + Rec<3>::X);
+    w[0] = make_float4(pf.x, pf.y, pf.z, o[0]);
+    w[1] = make_float4(o[1], o[2], o[3], o[4]);
+    w[2] = make_float4(o[5], o[6], o[7], o[8]);
+}
+
 template <int D>
 __device__ __forceinline__ void write_star_coords(const Ctx& c, int t, int i, typename VT<D>::F pf, int N) {
     if constexpr (D == 3) {
         const float4* q = reinterpret_cast<const float4*>(rp<D>(c, t) + Rec<D>::X);
-        const float4 f0 = q[0], f1 = q[1], f2 = q[2];
-        // old vertex k = (X[3k], X[3k+1], X[3k+2]) of the packed floats
-        const float X[12] = {f0.x, f0.y, f0.z, f0.w, f1.x, f1.y, f1.z, f1.w, f2.x, f2.y, f2.z, f2.w};
-        // new vertices 1..3 come from old vertices s1, s2, s3 (replace_first's even permutation)
-        const int s1 = i == 0 ? 1 : 0;
-        const int s2 = i == 0 ? 2 : (i == 1 ? 3 : (i == 2 ? 1 : 2));
-        const int s3 = i == 0 ? 3 : (i == 1 ? 2 : (i == 2 ? 3 : 1));
-        float o[9];
-#pragma unroll
-        for (int j = 0; j < 3; ++j) {
-            o[j] = s1 == 0 ? X[j] : (s1 == 1 ? X[3 + j] : (s1 == 2 ? X[6 + j] : X[9 + j]));
-            o[3 + j] = s2 == 0 ? X[j] : (s2 == 1 ? X[3 + j] : (s2 == 2 ? X[6 + j] : X[9 + j]));
-            o[6 + j] = s3 == 0 ? X[j] : (s3 == 1 ? X[3 + j] : (s3 == 2 ? X[6 + j] : X[9 + j]));
-        }
-        float4* w = reinterpret_cast<float4*>(rp<D>(c, N) + Rec<D>::X);
-        w[0] = make_float4(pf.x, pf.y, pf.z, o[0]);
-        w[1] = make_float4(o[1], o[2], o[3], o[4]);
-        w[2] = make_float4(o[5], o[6], o[7], o[8]);
+        store_star_coords3(c, q[0], q[1], q[2], i, pf, N);
     } else {
         typename VT<D>::F ox[4], nx[4];
         load_tx<D>(c, t, ox);

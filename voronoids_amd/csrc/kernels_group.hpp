@@ -413,10 +413,15 @@ __device__ __forceinline__ void facet_tuple(int2& out, int4 nv) { out = make_int
 #ifndef VO_MATCH_BYTES3
 #define VO_MATCH_BYTES3 4096
 #endif
+#ifndef VO_MATCH_MIN3
+#define VO_MATCH_MIN3 0      // least tuples per point in 3D (more LDS for the small lane groups)
+#endif
+template <int D, int G> constexpr int kGroupMC0 =
+    (D == 3 ? VO_MATCH_BYTES3 / (16 * (64 / G)) : 2048 / (8 * (64 / G)));
+template <int D, int G> constexpr int kGroupMC1 =
+    (D == 3 && kGroupMC0<D, G> < VO_MATCH_MIN3) ? VO_MATCH_MIN3 : kGroupMC0<D, G>;
 template <int D, int G> constexpr int kGroupMC =
-    (D == 3 && !VO_RIDGE_MATCH3) ? 0
-    : ((D == 3 ? VO_MATCH_BYTES3 / (16 * (64 / G)) : 2048 / (8 * (64 / G))) < kGroupFC<D>
-           ? (D == 3 ? VO_MATCH_BYTES3 / (16 * (64 / G)) : 2048 / (8 * (64 / G))) : kGroupFC<D>);
+    (D == 3 && !VO_RIDGE_MATCH3) ? 0 : (kGroupMC1<D, G> < kGroupFC<D> ? kGroupMC1<D, G> : kGroupFC<D>);
 
 // the new simplex (p first, then the facet) from its LDS tuple
 __device__ __forceinline__ int4 tuple_simplex(int4 fv, int pv) { return make_int4(pv, fv.y, fv.z, fv.w); }
