@@ -13,7 +13,7 @@ Two kinds of choice, per dimension:
   runtime  vo_tri_config knobs of that build: lanes per window point (a fixed size, or the
            per-round rule at another lane target, for both round kernels or the commit alone),
            the window's initial ratio, the winner fractions the large and the small windows aim
-           at, the least window, and rounds per submitted chunk.
+           at, the least window, rounds per submitted chunk, and the walk's step budget per round.
 
 The search: every build candidate at the default runtime setting, then every runtime candidate on
 the fastest build, then the union of the runtime settings that each gained on their own.  A candidate counts only if its triangulation has no violation and as many
@@ -73,6 +73,7 @@ def runtime_candidates(dim: int) -> list[dict]:
     out += [{"small_frac_lo_milli": 250, "small_frac_hi_milli": 500}, {"small_frac_lo_milli": 200, "small_frac_hi_milli": 450},
             {"small_frac_lo_milli": 450, "small_frac_hi_milli": 700}]
     out += [{"min_window": 128}, {"min_window": 256}]
+    out += [{"walk_budget": 16}, {"walk_budget": 32}, {"walk_budget": 64}]
     return out
 
 

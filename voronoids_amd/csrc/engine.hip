@@ -512,6 +512,7 @@ struct vo_engine {
     int tpp_milli = 0;
     int filters = 3;
     int abort_early = 1;               // as vo_tri_config "abort_early"
+    int walk_budget = 0;               // as vo_tri_config "walk_budget"
     int reuse = 0;                     // as vo_tri_config "reuse" (0 or 1 here)
     int2* fix = nullptr;               // shared facets of a round's winners (k_fix_shared)
     int fix_cap = 0;
@@ -666,6 +667,7 @@ Ctx engine_ctx(vo_engine* e) {
     c.filters = e->filters;
     c.solo = 0;
     c.abort_early = e->abort_early;
+    c.walk_budget = e->walk_budget;
     c.reuse = e->reuse;
     c.fix = e->fix;
     c.fix_cap = e->fix_cap;
@@ -1329,6 +1331,11 @@ int vo_engine_config(vo_engine* e, const char* key, int64_t value) {
     } else if (k == "abort_early") {
         if (value != 0 && value != 1) return fail(VO_ERR_INVALID, "abort_early must be 0 or 1");
         e->abort_early = (int)value;
+        drop_graphs(e);
+    } else if (k == "walk_budget") {
+        if (value < 0 || value > (1 << 20)) return fail(VO_ERR_INVALID, "walk_budget must be in 0..2^20");
+        e->walk_budget = (int)value;
+        drop_graphs(e);
     } else if (k == "reuse") {
         if (value != 0 && value != 1) return fail(VO_ERR_INVALID, "reuse must be 0 or 1");
         e->reuse = (int)value;

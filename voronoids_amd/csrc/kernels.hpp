@@ -29,7 +29,8 @@
 namespace vo {
 
 // ST_LOST: the point stopped its cavity search at a simplex that a higher-priority point of the
-// round had already reserved (it cannot win the round; it is retried in the next one)
+// round had already reserved, or its walk ran out of the round's step budget (it cannot win the
+// round; it is retried in the next one)
 enum : int { ST_OK = 0, ST_DEFER = 1, ST_DUP = 2, ST_OVERFLOW = 3, ST_BROKEN = 4, ST_IDLE = 5, ST_LOST = 6 };
 enum : uint32_t { ERR_OVERFLOW = 1, ERR_WALK = 2, ERR_CHAIN = 4, ERR_TETCAP = 8, ERR_HULL = 16, ERR_CACHE = 32,
                   ERR_PRECISION = 64, ERR_PENDING = 128 };
@@ -146,6 +147,8 @@ struct Ctx {
                            // each shared facet; k_fix_shared links the two sides after the commit
     int abort_early;       // lane-group cavity search: reserve each member as it is found and stop
                            // once a reservation shows a higher-priority point of the round
+    int walk_budget;       // lane-group cavity walk in windows of at least 64 points: a point that
+                           // has moved this many times stops and resumes there next round (0 = off)
     int M;                 // vertex id of point p is M + p
     int win_cap, kcap, fcap;
     int tet_cap;

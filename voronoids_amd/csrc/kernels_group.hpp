@@ -166,8 +166,12 @@ __global__ void __launch_bounds__(64, D == 3 ? VO_CAVITY_WAVES : VO_CAVITY_WAVES
 #endif
         }
     }
-    // ---- walk: lane k of the group tests face (r + k) mod M
+    // ---- walk: lane k of the group tests face (r + k) mod M.  A walk far longer than the round's
+    // others would hold the whole round: past walk_budget moves the point stops where it is and
+    // resumes from there next round (only in windows of 64 or more, whose other points progress)
     bool walking = st == ST_OK;
+    bool paused = false;
+    const int budget = (c.walk_budget > 0 && nwin >= 64) ? c.walk_budget : 0x7fffffff;
     int came = -1;
     uint32_t rng = mix32((uint32_t)p * 0x9E3779B9u + 0x632BE5ABu);
     F a[4];
@@ -218,6 +222,11 @@ __global__ void __launch_bounds__(64, D == 3 ? VO_CAVITY_WAVES : VO_CAVITY_WAVES
                 came = t;
                 t = nb_sel >> 2;
                 ld_ax<D>(c, t, aa, a);
+                if (step + 1 >= budget) {
+                    st = ST_LOST;
+                    paused = true;
+                    walking = false;
+                }
             }
         }
     }
@@ -226,6 +235,8 @@ __global__ void __launch_bounds__(64, D == 3 ? VO_CAVITY_WAVES : VO_CAVITY_WAVES
             c.cp[p] = t;
             hint_store<D>(hint_view(c, p), pf, a, t);
         }
+    } else if (paused && lead) {
+        c.cp[p] = t;          // no hint: t need not be near p
     }
 #ifdef VO_POINT_PROBE
     pr_tw = (unsigned)(wall_clock64() - pr_t0);

@@ -169,6 +169,7 @@ struct vo_tri {
     int chunk_rounds = 4;              // rounds submitted per chunk
     int round_timing = 0;              // events around each round's kernels (cost: ~3% at n = 4M)
     int abort_early = 1;               // losing cavity searches stop at the first reservation they lose
+    int walk_budget = 0;               // lane-group walk moves per round before a point pauses (0 = off)
     int reuse = -1;                    // new simplices take their winner's killed records first: 1 on,
                                        // 0 off, -1 only when append-only records would pass 2^29
     int reuse_on = 0;                  // this run's choice
@@ -226,6 +227,7 @@ Ctx make_ctx(vo_tri* t) {
     c.filters = t->filters;
     c.solo = 0;
     c.abort_early = t->abort_early;
+    c.walk_budget = t->walk_budget;
     c.reuse = t->reuse_on;
     c.fix = t->shared_facets ? t->fix : nullptr;
     c.fix_cap = t->fix_cap;
@@ -2044,6 +2046,9 @@ int vo_tri_config(vo_tri* t, const char* key, int64_t value) {
     } else if (k == "abort_early") {
         if (value != 0 && value != 1) return fail(VO_ERR_INVALID, "abort_early must be 0 or 1");
         t->abort_early = (int)value;
+    } else if (k == "walk_budget") {
+        if (value < 0 || value > (1 << 20)) return fail(VO_ERR_INVALID, "walk_budget must be in 0..2^20");
+        t->walk_budget = (int)value;
     } else if (k == "shared_facets") {
         if (value != 0 && value != 1) return fail(VO_ERR_INVALID, "shared_facets must be 0 or 1");
         t->shared_facets = (int)value;
