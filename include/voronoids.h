@@ -163,7 +163,9 @@ int vo_tri_stats(const vo_tri* t, vo_stats* out);
  * in 2D instead of 0.22 ms through the device planner, which submits at least two chunks of rounds).
  * "walk_budget" (default per the autotuner's table, else 0 = off): in windows of 64 or more points, a
  * point whose walk has moved this many times stops there and resumes next round, so that one long
- * walk does not hold a whole round.
+ * walk does not hold a whole round.  Like abort_early, a budget makes the winners of a round depend
+ * on timing (a walk's length depends on which concurrent write a hint cell kept); 0 with
+ * abort_early 0 gives the same rounds on every run.
  * "shared_facets" (1 default / 0): a window point whose cavity touches (shares a facet with) a
  * higher-priority point's may still win when it lies outside the sphere of the simplex that point
  * builds on the shared facet (inserting that point first would leave its cavity unchanged); the links

@@ -85,6 +85,8 @@ def test_pick_keeps_the_default_within_the_margin():
     assert A.pick(c, 0.01) is c[0]
     c[1]["ms"] = 98.0
     assert A.pick(c, 0.01) is c[1]
+    c[1]["rejected"] = "slower on clustered points"
+    assert A.pick(c, 0.01) is c[0]
 
 
 @pytest.mark.gpu

@@ -252,7 +252,8 @@ def test_device_planned_rounds_match_host_planned(dim):
     for pipelined, chunk in ((0, 4), (1, 1), (1, 4), (1, 7)):
         t = vo.DelaunayTree(np.vstack([P, Q]))
         t.config("abort_early", 0)              # the schedule that does not depend on timing
-        t.config("pipelined", pipelined)
+        t.config("walk_budget", 0)              # (a walk budget pauses walks whose length depends on
+        t.config("pipelined", pipelined)        # which concurrent write a hint cell kept)
         t.config("chunk_rounds", chunk)
         t.add_points_to_tree(P, seed=8)
         t.add_points_to_tree(Q, seed=9)
@@ -279,6 +280,7 @@ def test_device_planned_rounds_hand_over_exact_and_solo_rounds():
         for pipelined in (0, 1):
             t = vo.DelaunayTree(P)
             t.config("abort_early", 0)
+            t.config("walk_budget", 0)
             t.config("pipelined", pipelined)
             t.add_points_to_tree(P, seed=11)
             T, E = t.canonical()
@@ -294,14 +296,17 @@ def test_device_planned_rounds_hand_over_exact_and_solo_rounds():
 def test_full_size_3d_4m_invariants():
     # the headline workload at full size (3D uniform n = 4,000,000): the exact device check
     # (local Delaunay property of every face, adjacency and orientation consistency), vertex
-    # count, the super simplex's four hull faces, and (without early abort, whose winners depend
-    # on timing) the same rounds and windows from the device planner as from the host loop
+    # count, the super simplex's four hull faces, and (without early abort and the walk budget,
+    # whose winners depend on timing) the same rounds and windows from the device planner as from
+    # the host loop
     n = 4_000_000
     P = datasets.uniform(n, 3, seed=0)
     seen = []
     for pipelined, abort in ((1, 1), (1, 0), (0, 0)):
         t = vo.DelaunayTree(P)
         t.config("abort_early", abort)
+        if not abort:
+            t.config("walk_budget", 0)
         t.config("pipelined", pipelined)
         t.add_points_to_tree(P, seed=0)
         assert t.violations() == 0

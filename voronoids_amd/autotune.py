@@ -16,7 +16,9 @@ Two kinds of choice, per dimension:
            at, the least window, rounds per submitted chunk, and the walk's step budget per round.
 
 The search: every build candidate at the default runtime setting, then every runtime candidate on
-the fastest build, then the union of the runtime settings that each gained on their own.  A candidate counts only if its triangulation has no violation and as many
+the fastest build, then the union of the runtime settings that each gained on their own.  The
+winner must also not lose on clustered points (64 Gaussian clusters): it is timed there against
+the defaults and rejected, the next one tried, when slower by more than the margin.  A candidate counts only if its triangulation has no violation and as many
 simplices as the default's.  Times are medians of vo_tri_run's device time; the winner replaces
 the defaults only when it is more than 1% faster (run-to-run noise is about that).
 
@@ -127,13 +129,14 @@ def build_variants(dims=(3, 2), jobs: int = 4) -> list[str]:
         return list(ex.map(one, todo))
 
 
-def trial(dim: int, n: int, reps: int, settings: list[dict]) -> list[dict]:
-    """time vo_tri_run for each runtime setting in this process's library build (VORONOIDS_LIB)"""
+def trial(dim: int, n: int, reps: int, settings: list[dict], workload: str = "uniform") -> list[dict]:
+    """time vo_tri_run for each runtime setting in this process's library build (VORONOIDS_LIB),
+    on uniform points or on clustered ones (64 Gaussian clusters, sigma 0.02, 80% of the points)"""
     sys.path.insert(0, os.path.dirname(HERE))
     import numpy as np
     import voronoids_amd as vo
     from voronoids_amd import datasets
-    P = datasets.uniform(n, dim, seed=0)
+    P = datasets.uniform(n, dim, seed=0) if workload == "uniform" else datasets.clustered(n, seed=0, dim=dim)
     out = []
     for s in settings:
         ms, sims, viol, err = [], None, None, None
@@ -158,10 +161,11 @@ def trial(dim: int, n: int, reps: int, settings: list[dict]) -> list[dict]:
     return out
 
 
-def _trial_in_process(lib: str, dim: int, n: int, reps: int, settings: list[dict], timeout: int) -> list[dict]:
+def _trial_in_process(lib: str, dim: int, n: int, reps: int, settings: list[dict], timeout: int,
+                      workload: str = "uniform") -> list[dict]:
     env = dict(os.environ, VORONOIDS_LIB=lib)
     cmd = [sys.executable, "-u", "-m", "voronoids_amd.autotune", "trial", "--dim", str(dim), "--n", str(n),
-           "--reps", str(reps), "--settings", json.dumps(settings)]
+           "--reps", str(reps), "--settings", json.dumps(settings), "--workload", workload]
     # the child's lines are echoed as they come, so a long sweep shows its progress
     out = []
     t0 = time.time()
@@ -180,10 +184,11 @@ def _trial_in_process(lib: str, dim: int, n: int, reps: int, settings: list[dict
 
 
 def pick(candidates: list[dict], margin: float) -> dict:
-    """the fastest valid candidate if it beats the default build at the default setting (the
-    first candidate) by more than `margin` (a fraction: run-to-run noise), else that default"""
+    """the fastest valid candidate that no check rejected, if it beats the default build at the
+    default setting (the first candidate) by more than `margin` (a fraction: run-to-run noise),
+    else that default"""
     default = candidates[0]
-    best = min((c for c in candidates if c["valid"]), key=lambda c: c["ms"])
+    best = min((c for c in candidates if c["valid"] and not c.get("rejected")), key=lambda c: c["ms"])
     return best if best["ms"] < default["ms"] * (1.0 - margin) else default
 
 
@@ -232,7 +237,23 @@ def sweep(dim: int, n: int, reps: int, write: bool, timeout: int = 300, margin: 
                               "valid": rec["valid"]}), flush=True)
             results.append(rec)
     candidates = [{"build": r["build"], "setting": r["setting"], "ms": r["ms"], "valid": r["valid"]} for r in results]
-    best = pick(candidates, margin)
+    # 4. the winner must not lose on clustered points: tried there against the defaults, and
+    # rejected (the next one tried) when slower by more than the margin
+    base = None
+    while True:
+        best = pick(candidates, margin)
+        if best is candidates[0]:
+            break
+        if base is None:
+            base = _trial_in_process(variant_path(dim, {}), dim, n, reps, [{}], timeout, "clustered")[0]
+        rec = _trial_in_process(variant_path(dim, best["build"]), dim, n, reps, [best["setting"]], timeout,
+                                "clustered")[0]
+        best["clustered_ms"], best["clustered_default_ms"] = rec["ms"], base["ms"]
+        if rec["ms"] is not None and rec["violations"] == 0 and rec["ms"] <= base["ms"] * (1.0 + margin):
+            break
+        best["rejected"] = "slower on clustered points"
+        print(json.dumps({"dim": dim, "rejected": {k: best[k] for k in ("build", "setting", "ms", "clustered_ms",
+                                                                       "clustered_default_ms")}}), flush=True)
     summary = {"build": best["build"], "setting": best["setting"], "ms": best["ms"], "default_ms": default_ms,
                "margin": margin, "n": n, "reps": reps, "workload": f"{dim}D uniform n={n}, seed 0",
                "candidates": candidates}
@@ -258,12 +279,13 @@ def main() -> int:
     ap.add_argument("--settings", default="[{}]", help="trial: JSON list of runtime settings")
     ap.add_argument("--write", action="store_true", help="sweep: store the winner in tuning.json")
     ap.add_argument("--jobs", type=int, default=4)
+    ap.add_argument("--workload", choices=("uniform", "clustered"), default="uniform", help="trial: the points")
     args = ap.parse_args()
     n = args.n or (4_000_000 if args.dim == 3 else 10_000_000)
     if args.cmd == "build":
         build_variants(jobs=args.jobs)
     elif args.cmd == "trial":
-        trial(args.dim, n, args.reps, json.loads(args.settings))
+        trial(args.dim, n, args.reps, json.loads(args.settings), args.workload)
     else:
         sweep(args.dim, n, args.reps, args.write)
     return 0

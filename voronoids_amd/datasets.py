@@ -45,17 +45,17 @@ def uniform(n: int, dim: int, seed: int = 0, stream: int = 0) -> np.ndarray:
 
 
 def clustered(n: int = 4_000_000, seed: int = 0, n_centers: int = 64, sigma: float = 0.02,
-              frac_clustered: float = 0.8) -> np.ndarray:
-    """3D: frac_clustered of the points from isotropic Gaussians around n_centers uniform
-    centers (component chosen uniformly, no clipping), the rest uniform in [0,1)^3."""
+              frac_clustered: float = 0.8, dim: int = 3) -> np.ndarray:
+    """frac_clustered of the points from isotropic Gaussians around n_centers uniform centers
+    (component chosen uniformly, no clipping), the rest uniform in [0,1)^dim (3D by default)."""
     nc = int(round(n * frac_clustered))
-    centers = uniform(n_centers, 3, seed, stream=11)
+    centers = uniform(n_centers, dim, seed, stream=11)
     comp = (u64(seed, 12, 0, nc) % np.uint64(n_centers)).astype(np.int64)
-    u1 = unit(seed, 13, 0, nc * 3)
-    u2 = unit(seed, 14, 0, nc * 3)
+    u1 = unit(seed, 13, 0, nc * dim)
+    u2 = unit(seed, 14, 0, nc * dim)
     g = np.sqrt(-2.0 * np.log1p(-u1)) * np.cos(2.0 * np.pi * u2)    # Box-Muller, u1 in [0,1)
-    pts_c = centers[comp] + sigma * g.reshape(nc, 3)
-    pts_u = uniform(n - nc, 3, seed, stream=15)
+    pts_c = centers[comp] + sigma * g.reshape(nc, dim)
+    pts_u = uniform(n - nc, dim, seed, stream=15)
     return np.ascontiguousarray(np.vstack([pts_c, pts_u]))
 
 
