@@ -76,6 +76,8 @@ def runtime_candidates(dim: int) -> list[dict]:
             {"small_frac_lo_milli": 450, "small_frac_hi_milli": 700}]
     out += [{"min_window": 128}, {"min_window": 256}]
     out += [{"walk_budget": 16}, {"walk_budget": 32}, {"walk_budget": 64}]
+    boot = 0 if dim == 3 else 256
+    out += [{"boot_window": b} for b in (0, 128, 256, 512) if b != boot]
     return out
 
 

@@ -1202,15 +1202,19 @@ __device__ __forceinline__ bool shared_facets(const Ctx& c) { return !c.reuse &&
 // that point did not win, the old simplex across the facet, which nobody patched.  One thread per
 // entry; the two ends are set here, after every commit of the round has finished.
 template <int D>
-__global__ void __launch_bounds__(256) k_fix_shared(Ctx c) {
+__device__ __forceinline__ void fix_shared_pass(const Ctx& c, int first, int stride) {
     const int n = min(c.st->n_fix, c.fix_cap);
-    for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < n; e += gridDim.x * blockDim.x) {
+    for (int e = first; e < n; e += stride) {
         const int2 f = c.fix[e];
         if (f.x < 0) continue;                  // reserved by a point that then did not win
         const int X = *a_ptr<D>(c, f.y >> 2, f.y & 3);
         *a_ptr<D>(c, f.x, 0) = X;
         if (X >= 0) *a_ptr<D>(c, X >> 2, X & 3) = f.x << 2;
     }
+}
+template <int D>
+__global__ void __launch_bounds__(256) k_fix_shared(Ctx c) {
+    fix_shared_pass<D>(c, blockIdx.x * blockDim.x + threadIdx.x, gridDim.x * blockDim.x);
 }
 
 // ---------------------------------------------------------------- locate query

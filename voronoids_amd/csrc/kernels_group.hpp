@@ -95,26 +95,35 @@ __device__ __forceinline__ int lds_find(const int* list, int n, int x) {
     return -1;
 }
 
+// LDS of one wavefront of the lane-group cavity search
+template <int D, int G> struct alignas(16) CavityLds {
+    int4 adj[Group<G>::NG][kGroupAC<D, G>];   // adjacency of the first members of each point's cavity
+    int4 na[64];                               // adjacency of each lane's candidate
+    int cav[Group<G>::NG][kGroupKC<D>];        // killed lists
+    int tmp[64];
+};
+
+// One wavefront's share of a round's cavity searches: window slots it.off + kb + lane group for
+// kb = it.base, it.base + it.stride, ... below it.lim.  Run by k_cavity_g (one wavefront per
+// block) and by the one-workgroup early-round kernel (one share per wavefront).
 template <int D, int G>
-__global__ void __launch_bounds__(64, D == 3 ? VO_CAVITY_WAVES : VO_CAVITY_WAVES2) k_cavity_g(Ctx c, int nwin, unsigned long long key_hi) {
+__device__ __forceinline__ void cavity_pass(const Ctx c, int nwin, unsigned long long key_hi, const SlotIter it,
+                                            CavityLds<D, G>& L) {
     using V = VT<D>;
     using F = typename V::F;
     constexpr int M = D + 1;
-    constexpr int NG = Group<G>::NG;
     constexpr int KC = kGroupKC<D>;
     constexpr int FC = kGroupFC<D>;
     static_assert(KC % 4 == 0, "int4 scans");
-    __shared__ __align__(16) int s_cav[NG][KC];
-    __shared__ __align__(16) int s_tmp[64];
     constexpr int AC = kGroupAC<D, G>;
-    __shared__ __align__(16) int4 s_adj[NG][AC];
-    __shared__ __align__(16) int4 s_na[64];           // adjacency of each lane's candidate
+    auto& s_cav = L.cav;
+    auto& s_tmp = L.tmp;
+    auto& s_adj = L.adj;
+    auto& s_na = L.na;
     const Group<G> g;
     const bool lead = g.gl == 0;
     LocalCnt lc = {};
-    round_params(c, nwin, key_hi);
     const WinView wv = win_view(c);
-    const SlotIter it = slot_iter(c, nwin, NG);
     for (int kb = it.base; kb < it.lim; kb += it.stride) {
     const int slot = it.off + kb + g.gid;
     const bool valid = kb + g.gid < it.lim;
@@ -363,6 +372,13 @@ __global__ void __launch_bounds__(64, D == 3 ? VO_CAVITY_WAVES : VO_CAVITY_WAVES
     flush_counters(c.cnt, lc);
 }
 
+template <int D, int G>
+__global__ void __launch_bounds__(64, D == 3 ? VO_CAVITY_WAVES : VO_CAVITY_WAVES2) k_cavity_g(Ctx c, int nwin, unsigned long long key_hi) {
+    __shared__ CavityLds<D, G> L;
+    round_params(c, nwin, key_hi);
+    cavity_pass<D, G>(c, nwin, key_hi, slot_iter(c, nwin, Group<G>::NG), L);
+}
+
 // Partner of ridge k of facet j of a winner's cavity boundary (fv: the new simplices' vertex
 // tuples, p first, then the facet's D vertices): the ridge is the facet's vertices other than
 // fv[j][k]; the boundary of a cavity is a closed manifold, so exactly one other facet holds all
@@ -513,25 +529,32 @@ __device__ __forceinline__ bool outside_next_star(const Ctx& c, int t, int i, in
     return s <= 0;
 }
 
+// LDS of one wavefront of the lane-group commit
+template <int D, int G> struct alignas(16) CommitLds {
+    static constexpr int DW = (kGroupFC<D> * D + 31) / 32;   // one bit per (facet, ridge): internal face linked
+    int fac[Group<G>::NG][kGroupFC<D>];                     // facet list, then the new simplices
+    typename FacetTuple<D>::T fv[Group<G>::NG][kGroupMC<D, G> > 0 ? kGroupMC<D, G> : 1];   // facet tuples
+    unsigned done[Group<G>::NG][DW];
+};
+
+// One wavefront's share of a round's commits (slots as in cavity_pass)
 template <int D, int G>
-__global__ void __launch_bounds__(64, D == 3 ? VO_COMMIT_WAVES : VO_COMMIT_WAVES2) k_commit_g(Ctx c, int nwin, unsigned long long key_hi) {
+__device__ __forceinline__ void commit_pass(const Ctx c, int nwin, unsigned long long key_hi, const SlotIter it,
+                                            CommitLds<D, G>& L) {
     constexpr int M = D + 1;
-    constexpr int NG = Group<G>::NG;
     constexpr int FC = kGroupFC<D>;
     static_assert(FC % 4 == 0, "int4 scans");
-    constexpr int DW = (FC * D + 31) / 32;     // one bit per (facet, ridge): internal face linked
+    constexpr int DW = CommitLds<D, G>::DW;
     constexpr int MC = kGroupMC<D, G>;
-    __shared__ __align__(16) int s_fac[NG][FC];
-    __shared__ unsigned s_done[NG][DW];
-    __shared__ __align__(16) typename FacetTuple<D>::T s_fv[NG][MC > 0 ? MC : 1];
+    auto& s_fac = L.fac;
+    auto& s_done = L.done;
+    auto& s_fv = L.fv;
     const Group<G> g;
     const bool lead = g.gl == 0;
     // winners may share facets (see outside_next_star; linked by k_fix_shared)
     const bool refined = shared_facets(c);
     LocalCnt lc = {};
-    round_params(c, nwin, key_hi);
     const WinView wv = win_view(c);
-    const SlotIter it = slot_iter(c, nwin, NG);
     for (int kb = it.base; kb < it.lim; kb += it.stride) {
     const int slot = it.off + kb + g.gid;
     const bool valid = kb + g.gid < it.lim;
@@ -817,6 +840,13 @@ __global__ void __launch_bounds__(64, D == 3 ? VO_COMMIT_WAVES : VO_COMMIT_WAVES
     __builtin_amdgcn_wave_barrier();       // s_fac is reused by the next slots
     }
     flush_counters(c.cnt, lc);
+}
+
+template <int D, int G>
+__global__ void __launch_bounds__(64, D == 3 ? VO_COMMIT_WAVES : VO_COMMIT_WAVES2) k_commit_g(Ctx c, int nwin, unsigned long long key_hi) {
+    __shared__ CommitLds<D, G> L;
+    round_params(c, nwin, key_hi);
+    commit_pass<D, G>(c, nwin, key_hi, slot_iter(c, nwin, Group<G>::NG), L);
 }
 
 }  // namespace vo

@@ -77,7 +77,7 @@ int canonical_from_simplices(int M, const int32_t* s, int64_t nl, int32_t** T, i
 // window, so that rounds run back to back while the host only watches snapshots.  A round that
 // needs the host (points re-run alone, queued exact predicates, an error) or more simplex
 // space halts the planner; the rounds after a halt are empty until the host takes over.
-enum : int { HALT_HOST = 1, HALT_SPACE = 2, HALT_DONE = 4, HALT_STALL = 8 };
+enum : int { HALT_HOST = 1, HALT_SPACE = 2, HALT_DONE = 4, HALT_STALL = 8, HALT_BOOT = 16 };
 struct RunPlan {
     BatchCtl ctl;              // nwin / key_hi of the planned round (read by the round kernels)
     long long inserted, dups, windows, n_live, tet_cap;
@@ -90,6 +90,8 @@ struct RunPlan {
                                // each commit) instead of halting for the host
     int nq_done;               // the halted round's queue count, evaluated in the stream
     int min_window;            // least window size
+    int boot_limit;            // k_boot: the planner halts (HALT_BOOT) before a window larger than
+    int boot_rounds_end;       // this, or once `rounds` reaches boot_rounds_end (0: never)
 };
 
 struct vo_tri {
@@ -170,6 +172,8 @@ struct vo_tri {
     int round_timing = 0;              // events around each round's kernels (cost: ~3% at n = 4M)
     int abort_early = 1;               // losing cavity searches stop at the first reservation they lose
     int walk_budget = 0;               // lane-group walk moves per round before a point pauses (0 = off)
+    int boot_window = 0;               // first rounds in one workgroup (k_boot) while windows stay at
+                                       // or below this many points (0 = off; set per dimension)
     int reuse = -1;                    // new simplices take their winner's killed records first: 1 on,
                                        // 0 off, -1 only when append-only records would pass 2^29
     int reuse_on = 0;                  // this run's choice
@@ -411,6 +415,10 @@ __device__ void plan_step(RunPlan& p, RoundState* st, bool s_absorb) {
     if (p.npend > nwin) nwin = p.npend;
     const int nfresh = min(p.n_pts - p.head, nwin - p.npend);
     nwin = p.npend + nfresh;
+    if (p.boot_limit > 0 && (nwin > p.boot_limit || p.rounds >= p.boot_rounds_end)) {
+        p.halt |= HALT_BOOT;
+        return;
+    }
     if ((long long)st->n_tets + (long long)nwin * p.per_point + 1024 > p.tet_cap) {
         p.halt |= HALT_SPACE;
         return;
@@ -429,6 +437,65 @@ __device__ void plan_step(RunPlan& p, RoundState* st, bool s_absorb) {
     st->n_live_delta = 0;
     st->n_big = 0;
     p.launched = 1;
+}
+
+// ---------------------------------------------------------------- early rounds, one workgroup
+// A run's first rounds have windows of tens to a few hundred points.  Such a round is a short
+// chain of dependent memory accesses per point, and as launched rounds it costs four launches
+// (plan, cavity, commit, shared-facet links), each starting with cold caches.  k_boot runs these
+// rounds in one workgroup: its wavefronts take the window's slots as the blocks of k_cavity_g /
+// k_commit_g do (the same pass functions), barriers take the place of launches, and the records
+// stay in the CU's caches from round to round.  The planner is the launched rounds' planner
+// (plan_step), with boot_limit set: it halts before a window above the limit or at
+// boot_rounds_end, and the kernel clears that halt on exit, leaving no round planned, so the
+// host carries on from the plan as after any halt.  A round the kernel cannot absorb (a cavity
+// beyond the LDS lists, queued exact predicates, an error) halts the planner for the host, as in
+// k_plan_one without in-stream exact evaluation.  The loop is bounded by max_rounds.
+constexpr int kBootMaxRounds = 4096;
+template <int D> constexpr int kBootWaves = D == 3 ? 8 : 12;
+template <int D> constexpr int kBootLanes = 8;      // lanes per point in both passes
+
+template <int D>
+__global__ void __launch_bounds__(64 * kBootWaves<D>) k_boot(Ctx c, RunPlan* P, int* pend0, int* pend1, int max_rounds) {
+    constexpr int G = kBootLanes<D>;
+    constexpr int NW = kBootWaves<D>;
+    __shared__ union {
+        CavityLds<D, G> cav[NW];
+        CommitLds<D, G> com[NW];
+    } U;
+    __shared__ int s_absorb, s_go;     // two flags: a wave may still read one while thread 0 sets the other
+    const int w = threadIdx.x >> 6;
+    RunPlan& p = *P;
+    for (int r = 0; r <= max_rounds; ++r) {
+        if (threadIdx.x == 0)
+            s_absorb = p.launched && !(c.st->n_big > 0 || c.st->n_q > 0 || c.st->err);
+        __syncthreads();
+        const bool absorb = s_absorb != 0;
+        if (absorb) round_totals(c.cnt, c.st);
+        if (threadIdx.x == 0) {
+            if (r == max_rounds && !p.halt) p.halt |= HALT_BOOT;    // bound: plan nothing more
+            plan_step(p, c.st, absorb);
+            s_go = !p.halt && p.ctl.nwin > 0;
+        }
+        __syncthreads();
+        if (!s_go) break;
+        const int nwin = p.ctl.nwin;
+        const unsigned long long key_hi = p.ctl.key_hi;
+        c.win = p.cur ? pend1 : pend0;
+        c.pend_next = p.cur ? pend0 : pend1;
+        cavity_pass<D, G>(c, nwin, key_hi, SlotIter{w * Group<G>::NG, NW * Group<G>::NG, nwin, 0}, U.cav[w]);
+        __syncthreads();
+        commit_pass<D, G>(c, nwin, key_hi, SlotIter{w * Group<G>::NG, NW * Group<G>::NG, nwin, 0}, U.com[w]);
+        __syncthreads();
+        if (!c.reuse && c.fix) {
+            fix_shared_pass<D>(c, threadIdx.x, blockDim.x);
+            __syncthreads();
+        }
+    }
+    if (threadIdx.x == 0) {
+        p.halt &= ~HALT_BOOT;
+        p.boot_limit = 0;
+    }
 }
 
 
@@ -1024,9 +1091,30 @@ void add_chunk_times(vo_tri* t, int slot, int R) {
 // simplex space, or done).  Each round's grid is sized from the snapshot two chunks back,
 // scaled by the growth the window rule allows over that many rounds; the kernels stop at the
 // planned window.
+// The host's view after the device planner halted with snapshot P (the stream drained): its
+// counters and lists, and the planned round it could not absorb, if any, finished on the host
+int absorb_halted_plan(vo_tri* t, const RunPlan& P) {
+    int rc;
+    if ((rc = sync_state(t))) return rc;
+    // launches counted for the rounds that ran (the empty rounds after a halt are not)
+    t->n_cav += P.rounds - t->rounds + P.launched;
+    t->n_com += P.rounds - t->rounds + P.launched;
+    plan_to_host(t, P);
+    t->n_tets = t->st_host->n_tets;
+    // a round that queued exact predicates: the rest of this run evaluates them in the stream
+    const bool exact_done = P.exact_dev != 0;
+    if (exact_done && P.nq_done > 0) t->st_host->n_q = P.nq_done;
+    if (P.launched && t->st_host->n_q > 0) t->exact_dev = 1;
+    if (P.launched && (rc = finish_round(t, P.nwin, P.nfresh, nullptr, 0.f, 0.f, exact_done))) return rc;
+    if ((rc = check_err(t))) return rc;
+    if (P.halt & HALT_STALL) return fail(VO_ERR_INTERNAL, "no progress for 64 rounds");
+    return 0;
+}
+
 int run_device_planned(vo_tri* t) {
     const int D = t->D, R = t->chunk_rounds;
     int rc;
+    bool booted = false;
     if (!t->plan) {
         if ((rc = dev_alloc(&t->plan, 1))) return rc;
         if (hipHostMalloc((void**)&t->plan_host, 2 * sizeof(RunPlan)) != hipSuccess) return fail(VO_ERR_OOM, "pinned");
@@ -1052,7 +1140,38 @@ int run_device_planned(vo_tri* t) {
             if ((rc = ensure_records(t, (int64_t)t->n_tets + (int64_t)nwin * window_records(D) + 1024))) return rc;
         }
         t->plan_host[0] = plan_from_host(t);
+        // small first windows: rounds in one workgroup (k_boot) until the window outgrows
+        // boot_window, once per run; the host then plans again from the kernel's final plan
+        const bool boot = !booted && t->boot_window > 0 && t->group != 1 && !t->exact_dev &&
+                          host_window(t).first <= t->boot_window;
+        if (boot) {
+            t->plan_host[0].boot_limit = t->boot_window;
+            t->plan_host[0].boot_rounds_end = (int)std::min<int64_t>(t->rounds + kBootMaxRounds, INT32_MAX);
+        }
         VO_HIP(hipMemcpyAsync(t->plan, &t->plan_host[0], sizeof(RunPlan), hipMemcpyHostToDevice, t->s));
+        if (boot) {
+            booted = true;
+            Ctx c = make_ctx(t);
+            c.ctl = &t->plan->ctl;
+            if (t->round_timing) VO_HIP(hipEventRecord(t->ev[4], t->s));
+            if (D == 3)
+                hipLaunchKernelGGL(k_boot<3>, dim3(1), dim3(64 * kBootWaves<3>), 0, t->s, c, t->plan, t->pend[0],
+                                   t->pend[1], kBootMaxRounds);
+            else
+                hipLaunchKernelGGL(k_boot<2>, dim3(1), dim3(64 * kBootWaves<2>), 0, t->s, c, t->plan, t->pend[0],
+                                   t->pend[1], kBootMaxRounds);
+            VO_HIP(hipGetLastError());
+            if (t->round_timing) VO_HIP(hipEventRecord(t->ev[5], t->s));
+            VO_HIP(hipMemcpyAsync(&t->plan_host[0], t->plan, sizeof(RunPlan), hipMemcpyDeviceToHost, t->s));
+            VO_HIP(hipStreamSynchronize(t->s));
+            if (t->round_timing) {
+                float ms = 0;
+                hipEventElapsedTime(&ms, t->ev[4], t->ev[5]);
+                t->ms_cav += ms;
+            }
+            if ((rc = absorb_halted_plan(t, t->plan_host[0]))) return rc;
+            continue;
+        }
         int cur = t->cur;
         int k = 0, inflight = 0;
         double base = host_window(t).first, gam = t->gamma;
@@ -1110,20 +1229,7 @@ int run_device_planned(vo_tri* t) {
         // the newer chunk ends with no-op rounds; its snapshot is the planner's final state
         VO_HIP(hipStreamSynchronize(t->s));
         add_chunk_times(t, (k - 1) & 1, R);
-        const RunPlan P = t->plan_host[(k - 1) & 1];
-        if ((rc = sync_state(t))) return rc;
-        // launches counted for the rounds that ran (the empty rounds after a halt are not)
-        t->n_cav += P.rounds - t->rounds + P.launched;
-        t->n_com += P.rounds - t->rounds + P.launched;
-        plan_to_host(t, P);
-        t->n_tets = t->st_host->n_tets;
-        // a round that queued exact predicates: the rest of this run evaluates them in the stream
-        const bool exact_done = P.exact_dev != 0;
-        if (exact_done && P.nq_done > 0) t->st_host->n_q = P.nq_done;
-        if (P.launched && t->st_host->n_q > 0) t->exact_dev = 1;
-        if (P.launched && (rc = finish_round(t, P.nwin, P.nfresh, nullptr, 0.f, 0.f, exact_done))) return rc;
-        if ((rc = check_err(t))) return rc;
-        if (P.halt & HALT_STALL) return fail(VO_ERR_INTERNAL, "no progress for 64 rounds");
+        if ((rc = absorb_halted_plan(t, t->plan_host[(k - 1) & 1]))) return rc;
     }
     return 0;
 }
@@ -1408,6 +1514,9 @@ int vo_tri_create(int hip_device, int dim, const double* pts, int64_t n, vo_tri*
     t->ws_cap = 1 << 14;
     t->gamma = dim == 3 ? 0.2 : 0.6;
     t->big_window = dim == 3 ? 16384 : 65536;
+    // measured at 3D n = 4M: 95 -> 109 ms with 256 (one CU holds too few lanes for 3D cavities and
+    // commits); 2D n = 10M: 35.4 -> 34.9 ms
+    t->boot_window = dim == 3 ? 0 : 256;
     // large 3D windows aim at 65% winners (with shared facets: 3D n = 4M 115 -> 112 ms; 2D unchanged)
     t->big_frac_lo = dim == 3 ? 0.65 : 0.55;
     if ((rc = dev_alloc(&t->win, t->win_cap)) || (rc = dev_alloc(&t->pend[0], t->win_cap)) ||
@@ -2046,6 +2155,9 @@ int vo_tri_config(vo_tri* t, const char* key, int64_t value) {
     } else if (k == "abort_early") {
         if (value != 0 && value != 1) return fail(VO_ERR_INVALID, "abort_early must be 0 or 1");
         t->abort_early = (int)value;
+    } else if (k == "boot_window") {
+        if (value < 0 || value > 65536) return fail(VO_ERR_INVALID, "boot_window must be in 0..65536");
+        t->boot_window = (int)value;
     } else if (k == "walk_budget") {
         if (value < 0 || value > (1 << 20)) return fail(VO_ERR_INVALID, "walk_budget must be in 0..2^20");
         t->walk_budget = (int)value;
