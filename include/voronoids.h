@@ -277,8 +277,9 @@ int vo_engine_finish(vo_engine* e, int64_t* n_left);
 /* move up to max completed results into out (ownership of simplices passes to the caller) */
 int vo_engine_results(vo_engine* e, vo_set_result* out, int64_t max, int64_t* n_out);
 int vo_engine_stats_get(const vo_engine* e, vo_engine_stats* out);
-/* knobs: "rounds_per_replay" (1..64, default 8), "lanes_target" (as vo_tri_config),
- * "group_lanes" (0 = per replay, or 1/4/8/16/32), "min_window" (least fresh points per set
+/* knobs: "rounds_per_replay" (1..64, default 8), "lanes_target" (as vo_tri_config; unset in 3D,
+ * 32 lanes per point up to 65,536 window points and 16 beyond), "group_lanes" (0 = per replay, or
+ * 1/4/8/16/32), "min_window" (least fresh points per set
  * and round, default 8), "window_cap" (points per round, up to the allocated 2^20/2^21),
  * "tets_per_point_milli" (first simplex range per point x 1000, default 30000 (3D) /
  * 7000 (2D)), "window_gamma_milli" (initial window per inserted point x 1000), "filters",
