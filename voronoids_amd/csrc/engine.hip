@@ -531,6 +531,13 @@ struct vo_engine {
     // that the copies overlap the next replay
     std::vector<vo_set_result> copy_res;
     std::vector<int64_t> copy_off;              // first row in out_h of each
+    // the export's device-to-host copies run on a stream of their own, so that the next replay
+    // starts while they are in flight; copy_done follows them, and the host waits for it before
+    // it reads out_h / ocur_h or the export reuses out_d
+    hipStream_t cs = nullptr;
+    hipEvent_t export_done = nullptr, copy_done = nullptr;
+    bool copy_pending = false;
+    int* ocur_h = nullptr;                      // pinned: rows the export wrote per set (checked)
     // stats
     vo_engine_stats stats{};
 };
@@ -621,6 +628,7 @@ void parallel_for(vo_engine* e, int n, const std::function<void(int)>& fn) {
 
 void engine_free(vo_engine* e) {
     stop_workers(e);
+    if (e->cs) hipStreamSynchronize(e->cs);
     void* ptrs[] = {e->a.X64, e->a.X32, e->a.cp, e->a.vmap, e->a.pset, e->a.rec, e->a.hint, e->a.sets,
                     e->win, e->pend[0], e->pend[1], e->cav, e->fac, e->fnb, e->nk, e->nf, e->status, e->big, e->cav1,
                     e->fac1, e->misc1, e->st, e->ctl, e->cnt, e->q, e->cache, e->ws, e->done, e->stage_d, e->out_d,
@@ -636,6 +644,10 @@ void engine_free(vo_engine* e) {
         if (g) hipGraphExecDestroy(g);
     if (e->ev0) hipEventDestroy(e->ev0);
     if (e->ev1) hipEventDestroy(e->ev1);
+    if (e->export_done) hipEventDestroy(e->export_done);
+    if (e->copy_done) hipEventDestroy(e->copy_done);
+    if (e->ocur_h) hipHostFree(e->ocur_h);
+    if (e->cs) hipStreamDestroy(e->cs);
     if (e->s) hipStreamDestroy(e->s);
     for (auto& r : e->ready) std::free(r.simplices);
 }
@@ -1045,11 +1057,24 @@ int extend_ranges(vo_engine* e) {
     return 0;
 }
 
+// the export's copies of the last collect have landed in out_h / ocur_h
+int wait_copy(vo_engine* e) {
+    if (!e->copy_pending) return 0;
+    VO_HIP(hipEventSynchronize(e->copy_done));
+    e->copy_pending = false;
+    return 0;
+}
+
 // rows of exported sets from the pinned bounce buffer into each result's own buffer (worker
 // threads); the results join the ready queue
 int flush_copies(vo_engine* e) {
+    int rc = wait_copy(e);
+    if (rc) return rc;
     const int ns = (int)e->copy_res.size();
     if (ns == 0) return 0;
+    for (int i = 0; i < ns; ++i)
+        if (e->ocur_h[i] != (int)e->copy_res[i].n_simplices)
+            return fail(VO_ERR_INTERNAL, "export count differs from the set's live count");
     const int M = e->M;
     const int32_t* src = e->out_h;
     std::atomic<int> oom{0};
@@ -1095,6 +1120,8 @@ int collect(vo_engine* e, int n_done) {
             if (k1 > k0 && (size_t)(rows + lv) * M > e->out_cap) break;
             if ((size_t)lv * M > e->out_cap) {
                 // a set larger than the buffer: grow it (device buffer and its pinned bounce)
+                int rcw = wait_copy(e);
+                if (rcw) return rcw;
                 hipFree(e->out_d);
                 hipHostFree(e->out_h);
                 e->out_d = nullptr;
@@ -1150,17 +1177,22 @@ int collect(vo_engine* e, int n_done) {
                                reinterpret_cast<const int*>(e->stage_d + ol_off),
                                reinterpret_cast<int*>(e->stage_d + oc_off), e->out_d);
         VO_HIP(hipGetLastError());
-        std::vector<int> ocur(ns);
-        VO_HIP(hipMemcpyAsync(ocur.data(), e->stage_d + oc_off, sizeof(int) * ns, hipMemcpyDeviceToHost, e->s));
-        // one device-to-host copy into the pinned bounce buffer, then every set's rows into
-        // its own result buffer on the worker threads
+        // the row counts (from the staging buffer, which the next admission overwrites: on this
+        // stream), then on the copy stream one device-to-host copy of the rows into the pinned
+        // bounce buffer (every set's rows go to its own result buffer later, on the worker
+        // threads).  The next replay need not wait for it: the export has read the finished
+        // sets' records by then, their ranges are handed on only to later work of this stream,
+        // and out_d is rewritten only after the host has waited for copy_done.
+        VO_HIP(hipMemcpyAsync(e->ocur_h, e->stage_d + oc_off, sizeof(int) * ns, hipMemcpyDeviceToHost, e->s));
+        VO_HIP(hipEventRecord(e->export_done, e->s));
+        VO_HIP(hipStreamWaitEvent(e->cs, e->export_done, 0));
         if (rows > 0)
-            VO_HIP(hipMemcpyAsync(e->out_h, e->out_d, sizeof(int32_t) * (size_t)rows * M, hipMemcpyDeviceToHost, e->s));
-        VO_HIP(hipStreamSynchronize(e->s));
+            VO_HIP(hipMemcpyAsync(e->out_h, e->out_d, sizeof(int32_t) * (size_t)rows * M, hipMemcpyDeviceToHost, e->cs));
+        VO_HIP(hipEventRecord(e->copy_done, e->cs));
+        e->copy_pending = true;
         for (int i = 0; i < ns; ++i) {
             const DoneRec& d = dr[k0 + i];
             std::unique_ptr<HSet> h = std::move(e->resident[d.slot]);
-            if (ocur[i] != d.live) return fail(VO_ERR_INTERNAL, "export count differs from the set's live count");
             vo_set_result r{};
             r.tag = h->tag;
             r.dim = e->D;
@@ -1230,6 +1262,12 @@ int vo_engine_create(int hip_device, int dim, int64_t max_points, int64_t max_si
         return rc;
     };
     if (hipStreamCreateWithFlags(&E->s, hipStreamNonBlocking) != hipSuccess) return bail(fail(VO_ERR_HIP, "stream"));
+    if (hipStreamCreateWithFlags(&E->cs, hipStreamNonBlocking) != hipSuccess) return bail(fail(VO_ERR_HIP, "stream"));
+    if (hipEventCreateWithFlags(&E->export_done, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&E->copy_done, hipEventDisableTiming) != hipSuccess)
+        return bail(fail(VO_ERR_HIP, "event"));
+    if (hipHostMalloc((void**)&E->ocur_h, sizeof(int) * (size_t)E->nslots) != hipSuccess)
+        return bail(fail(VO_ERR_OOM, "pinned export counts"));
     if (hipEventCreate(&E->ev0) != hipSuccess || hipEventCreate(&E->ev1) != hipSuccess)
         return bail(fail(VO_ERR_HIP, "event"));
     const int S = E->S;
