@@ -169,8 +169,8 @@ int vo_tri_stats(const vo_tri* t, vo_stats* out);
  * "boot_window" (default 0 in 3D, 256 in 2D; 0..65536, 0 = off): a vo_tri_run whose first windows hold at
  * most this many points runs those rounds in one workgroup (one launch, barriers between the
  * phases, the records in one CU's caches) until the window outgrows it; same rounds, windows and
- * result (measured 2D n = 10M 35.4 -> 34.9 ms; 3D n = 4M 95 -> 109 ms, so off there).  Its time
- * counts in ms_locate_cavity when round_timing is on.
+ * result (measured 2D n = 10M 35.4 -> 34.9 ms; 3D n = 4M 95 -> 109 ms, so off there).  Those rounds
+ * count in vo_stats rounds, not in the launch counts, and their time only in ms_run.
  * "shared_facets" (1 default / 0): a window point whose cavity touches (shares a facet with) a
  * higher-priority point's may still win when it lies outside the sphere of the simplex that point
  * builds on the shared facet (inserting that point first would leave its cavity unchanged); the links

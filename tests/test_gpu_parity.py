@@ -262,10 +262,12 @@ def test_device_planned_rounds_match_host_planned(dim):
         T, E = t.canonical()
         assert np.array_equal(T, ref.T) and np.array_equal(E, ref.E), (pipelined, chunk, boot)
         st = t.stats()
+        # (rounds run inside the one-workgroup kernel are not launches of the round kernels)
         seen.append((st["rounds"], st["points_inserted"], st["windows_total"], st["created"], st["killed"],
-                     st["launches_locate_cavity"]))
+                     st["launches_locate_cavity"] if boot == 0 else None))
         assert t.violations() == 0
-    assert all(x == seen[0] for x in seen), seen
+    assert all(x[:5] == seen[0][:5] for x in seen), seen
+    assert all(x[5] == seen[0][5] for x in seen if x[5] is not None), seen
 
 
 def test_device_planned_rounds_hand_over_exact_and_solo_rounds():

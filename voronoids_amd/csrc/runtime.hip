@@ -1153,7 +1153,6 @@ int run_device_planned(vo_tri* t) {
             booted = true;
             Ctx c = make_ctx(t);
             c.ctl = &t->plan->ctl;
-            if (t->round_timing) VO_HIP(hipEventRecord(t->ev[4], t->s));
             if (D == 3)
                 hipLaunchKernelGGL(k_boot<3>, dim3(1), dim3(64 * kBootWaves<3>), 0, t->s, c, t->plan, t->pend[0],
                                    t->pend[1], kBootMaxRounds);
@@ -1161,15 +1160,14 @@ int run_device_planned(vo_tri* t) {
                 hipLaunchKernelGGL(k_boot<2>, dim3(1), dim3(64 * kBootWaves<2>), 0, t->s, c, t->plan, t->pend[0],
                                    t->pend[1], kBootMaxRounds);
             VO_HIP(hipGetLastError());
-            if (t->round_timing) VO_HIP(hipEventRecord(t->ev[5], t->s));
             VO_HIP(hipMemcpyAsync(&t->plan_host[0], t->plan, sizeof(RunPlan), hipMemcpyDeviceToHost, t->s));
             VO_HIP(hipStreamSynchronize(t->s));
-            if (t->round_timing) {
-                float ms = 0;
-                hipEventElapsedTime(&ms, t->ev[4], t->ev[5]);
-                t->ms_cav += ms;
-            }
+            // the kernel's rounds are no launches of the round kernels, and their time is not
+            // in the round kernels' timings (ms_run holds it)
+            const int64_t n_cav = t->n_cav, n_com = t->n_com;
             if ((rc = absorb_halted_plan(t, t->plan_host[0]))) return rc;
+            t->n_cav = n_cav;
+            t->n_com = n_com;
             continue;
         }
         int cur = t->cur;
