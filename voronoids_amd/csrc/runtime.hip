@@ -1239,8 +1239,21 @@ namespace {
 // RAII device buffer for the canonical export's temporaries
 template <class T> struct DevBuf {
     T* p = nullptr;
-    ~DevBuf() { if (p) hipFree(p); }
+    bool own = true;           // false: a view of one of the handle's idle buffers
+    ~DevBuf() { if (p && own) hipFree(p); }
 };
+
+// n elements in an idle buffer of the handle when it holds them (the window's lists are free
+// between runs, and every call of the handle waits for its stream before it returns), else a
+// new allocation: large exports then allocate nothing
+template <class T> int idle_or_alloc(DevBuf<T>& b, void* idle, size_t idle_bytes, size_t n) {
+    if (idle && sizeof(T) * std::max<size_t>(n, 1) <= idle_bytes) {
+        b.p = static_cast<T*>(idle);
+        b.own = false;
+        return 0;
+    }
+    return dev_alloc(&b.p, n);
+}
 
 // exclusive scan of n ints (device), total of the inputs to *total
 int device_excl_scan(hipStream_t s, const int* in, int* out, int64_t n, int64_t* total) {
@@ -1722,7 +1735,10 @@ static int live_positions(vo_tri* t, DevBuf<int>& pos, int64_t* n_live) {
     const int nb = (n + 255) / 256;
     DevBuf<int> flag, bsum;
     int rc;
-    if ((rc = dev_alloc(&flag.p, n)) || (rc = dev_alloc(&pos.p, n)) || (rc = dev_alloc(&bsum.p, nb))) return rc;
+    if ((rc = idle_or_alloc(flag, t->cav, sizeof(int) * (size_t)t->win_cap * t->kcap, (size_t)n)) ||
+        (rc = idle_or_alloc(pos, t->fac, sizeof(int) * (size_t)t->win_cap * t->fcap, (size_t)n)) ||
+        (rc = idle_or_alloc(bsum, t->big, sizeof(int) * (size_t)t->win_cap, (size_t)nb)))
+        return rc;
     const Ctx c = make_ctx(t);
     if (t->D == 3) hipLaunchKernelGGL(k_live_flags<3>, dim3(nb), dim3(256), 0, t->s, c, n, flag.p);
     else hipLaunchKernelGGL(k_live_flags<2>, dim3(nb), dim3(256), 0, t->s, c, n, flag.p);
@@ -1749,7 +1765,14 @@ static int export_device(const vo_tri* tc, DevBuf<int32_t>& simp, DevBuf<int32_t
     int rc;
     if ((rc = live_positions(t, pos, &acc))) return rc;
     const int M = t->M;
-    if ((rc = dev_alloc(&simp.p, acc * M)) || (rc = dev_alloc(&nbr.p, acc * M))) return rc;
+    const size_t fnb_bytes = sizeof(int) * (size_t)t->win_cap * t->fcap;
+    if (sizeof(int32_t) * 2 * (size_t)std::max<int64_t>(acc, 1) * M <= fnb_bytes) {
+        simp.p = t->fnb;
+        nbr.p = t->fnb + (size_t)std::max<int64_t>(acc, 1) * M;
+        simp.own = nbr.own = false;
+    } else if ((rc = dev_alloc(&simp.p, acc * M)) || (rc = dev_alloc(&nbr.p, acc * M))) {
+        return rc;
+    }
     const Ctx c = make_ctx(t);
     if (t->D == 3)
         hipLaunchKernelGGL(k_export<3>, dim3(nb), dim3(256), 0, t->s, c, pos.p, n, t->vmap, simp.p, nbr.p);
@@ -1784,7 +1807,7 @@ static int canonical_device(const vo_tri* t, int32_t** T, int64_t* n_T, int32_t*
     int64_t nl = 0;
     int rc = export_device(t, ds, dn, &nl);
     if (rc) return rc;
-    hipFree(dn.p);
+    if (dn.own) hipFree(dn.p);
     dn.p = nullptr;
     const int first = 2 * M;
     const int nb = std::max(1, bit_length((int64_t)t->input_of.size() - 1));
