@@ -11,7 +11,7 @@
 //
 // Tier 1 evaluates in FP32 on coordinates stored as fl32(x - c) (c = bounding-sphere
 // center): Shewchuk's stage-A expression plus a closed-form bound on the storage error
-// |s - (x - c)| <= 2^-23 |s| (derivation below).  Tier 2 is the FP64 stage-A
+// |s - (x - c)| <= b |s|, b = 2^-24 (1 + 2^-22) (derivation below).  Tier 2 is the FP64 stage-A
 // filter of Shewchuk (1997, "Adaptive Precision Floating-Point Arithmetic and Fast Robust
 // Geometric Predicates"): evaluation on the raw f64 coordinates, bound = c * eps * permanent.
 // Tier 3 is exact: FP64 expansion arithmetic (same paper, sections 2.2-2.6) on the exact
@@ -27,9 +27,12 @@ namespace vo {
 enum : int { SIGN_UNDECIDED = 2 };
 
 // ----------------------------------------------------------------------------- tier 1
-// FP32 on stored coordinates s = fl32(x - c).  Each computed difference d~ = fl(s_a - s_b)
-// differs from the exact difference of the inputs by at most
-//     eps = 2^-23 (|s_a| + |s_b|) + 2^-24 |d~| + 2e-37  <=  2^-22 m + 2^-24 delta + 2e-37,
+// FP32 on stored coordinates s = fl32(fl64(x - c)).  The f64 difference is off by at most
+// 2^-53 |x - c| and its rounding to fp32 by at most 2^-24 |s| (round to nearest; a subnormal s
+// is off by at most 2^-150, inside the 2e-37 below), so |s - (x - c)| <= b |s| with
+// b = 2^-24 (1 + 2^-22) > 2^-24 + 2^-52 (1 + 2^-23).  Each computed difference d~ = fl(s_a - s_b)
+// then differs from the exact difference of the inputs by at most
+//     eps = b (|s_a| + |s_b|) + 2^-24 |d~| + 2e-37  <=  2b m + 2^-24 delta + 2e-37,
 // m = max |s|, delta = max |d~| over the call.  The determinant is then evaluated from the
 // d~ with Shewchuk's stage-A expression (its bound c * u * permanent covers the evaluation,
 // u = 2^-24), and the input error is bounded by multilinearity and Hadamard's inequality:
@@ -46,8 +49,9 @@ constexpr float kIccA32 = (10.0f + 96.0f * kU32) * kU32;
 constexpr float kIspA32 = (16.0f + 224.0f * kU32) * kU32;
 constexpr float kTiny32 = 1e-30f;
 
-// 2^-22 m + 2^-24 delta, plus 2e-37 for coordinates that underflowed in the fp32 store
-__device__ __forceinline__ float in_err(float m, float dl) { return m * 2.384185791015625e-7f + dl * kU32 + 2e-37f; }
+// 2b m + 2^-24 delta (2b = 2^-23 + 2^-45, exact in fp32), plus 2e-37 for coordinates that
+// underflowed in the fp32 store
+__device__ __forceinline__ float in_err(float m, float dl) { return m * 1.1920931797249068e-7f + dl * kU32 + 2e-37f; }
 
 __device__ __forceinline__ int t1_decide(float det, float perm, float pin) {
     if (!(perm > kTiny32) || !(perm < 1e30f)) return SIGN_UNDECIDED;   // also NaN
