@@ -700,9 +700,16 @@ __device__ __forceinline__ void write_star_coords(const Ctx& c, int t, int i, ty
         const float4* q = reinterpret_cast<const float4*>(rp<D>(c, t) + Rec<D>::X);
         store_star_coords3(c, q[0], q[1], q[2], i, pf, N);
     } else {
-        typename VT<D>::F ox[4], nx[4];
-        load_tx<D>(c, t, ox);
-        star_record<D>(ox, i, pf, nx);
+        // scalar selects on the three loaded points (selecting elements of a private array by i
+        // became an indexed array in scratch memory, a store and two reloads per facet)
+        const float2* q = reinterpret_cast<const float2*>(rp<D>(c, t) + Rec<D>::X);
+        const float2 o0 = q[0], o1 = q[1], o2 = q[2];
+        const bool i0 = i == 0, i1 = i == 1;
+        typename VT<D>::F nx[4];
+        nx[0] = pf;
+        nx[1] = make_float2(i0 ? o1.x : (i1 ? o2.x : o0.x), i0 ? o1.y : (i1 ? o2.y : o0.y));
+        nx[2] = make_float2(i0 ? o2.x : (i1 ? o0.x : o1.x), i0 ? o2.y : (i1 ? o0.y : o1.y));
+        nx[3] = pf;
         store_tx<D>(c, N, nx);
     }
 }
