@@ -285,7 +285,11 @@ __device__ __forceinline__ void cavity_pass(const Ctx c, int nwin, unsigned long
             const int mi = lo + i / M;
             tet = s_cav[g.gid][mi];
             face = i % M;
-            nb = mi < AC ? reinterpret_cast<const int*>(s_adj[g.gid])[mi * 4 + face] : *a_ptr<D>(c, tet, face);
+            // the LDS entry, or past AC (rare: large cavities) the record's, read by a volatile
+            // load: a plain one let the compiler merge the two addresses into one flat load,
+            // slower than an LDS read on the common path
+            if (mi < AC) nb = reinterpret_cast<const int*>(s_adj[g.gid])[mi * 4 + face];
+            else nb = *reinterpret_cast<const volatile int*>(a_ptr<D>(c, tet, face));
             hull = nb < 0;
             if (!hull) {
                 n = nb >> 2;
