@@ -276,6 +276,15 @@ template <int D> __device__ __forceinline__ int4 ld_v(const Ctx& c, int t) {
         return make_int4(q[0], q[1], q[2], -1);
     }
 }
+// v[0] and v[1] of record t (a dead record's mark and forward pointer) in one 8-byte load.  A
+// relaxed atomic load, which the compiler never splits: from an int4 it loaded v[0], waited,
+// and only then loaded v[1] when v[0] < 0, two dependent round trips per forward hop.  Nothing
+// writes these words while a cavity search runs.
+template <int D> __device__ __forceinline__ int2 ld_v01(const Ctx& c, int t) {
+    const unsigned long long w = __hip_atomic_load(reinterpret_cast<unsigned long long*>(v_ptr<D>(c, t, 0)),
+                                                   __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    return make_int2((int)(unsigned)w, (int)(unsigned)(w >> 32));
+}
 template <int D> __device__ __forceinline__ void st_v(const Ctx& c, int t, int4 v) {
     if constexpr (D == 3) *reinterpret_cast<int4*>(rp<D>(c, t) + Rec<D>::V) = v;
     else {
@@ -803,11 +812,11 @@ __global__ void __launch_bounds__(256) k_cavity(Ctx c, int nwin, unsigned long l
         int st = ST_OK;
         int t = c.cp[p];
         if (t < 0) t = hint_lookup<D>(hv, pf);
-        int4 tt = ld_v<D>(c, t);
+        int2 tt = ld_v01<D>(c, t);
         for (int hops = 0; tt.x < 0; ++hops) {
             if (hops > kChainLimit) { atomicOr(&c.st->err, ERR_CHAIN); st = ST_BROKEN; break; }
             t = tt.y;
-            tt = ld_v<D>(c, t);
+            tt = ld_v01<D>(c, t);
             lc.chain++;
         }
         // remembering stochastic visibility walk

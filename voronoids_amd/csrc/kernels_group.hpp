@@ -130,7 +130,7 @@ __device__ __forceinline__ void cavity_pass(const Ctx c, int nwin, unsigned long
     int st = valid ? ST_OK : ST_IDLE;
     int p = 0, pv = 0, t = 0;
     F pf{};
-    int4 tt = make_int4(0, 0, 0, 0);
+    int2 tt = make_int2(0, 0);
     unsigned long long key = 0;
 #ifdef VO_POINT_PROBE
     const unsigned long long pr_t0 = wall_clock64();
@@ -164,11 +164,11 @@ __device__ __forceinline__ void cavity_pass(const Ctx c, int nwin, unsigned long
             pr_dh = sqrtf(d2);
         }
 #endif
-        tt = ld_v<D>(c, t);
+        tt = ld_v01<D>(c, t);
         for (int hops = 0; tt.x < 0; ++hops) {
             if (hops > kChainLimit) { if (lead) atomicOr(&c.st->err, ERR_CHAIN); st = ST_BROKEN; break; }
             t = tt.y;
-            tt = ld_v<D>(c, t);
+            tt = ld_v01<D>(c, t);
             if (lead) lc.chain++;
 #ifdef VO_POINT_PROBE
             pr_hops++;
