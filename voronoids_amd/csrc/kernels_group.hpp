@@ -570,11 +570,14 @@ __device__ __forceinline__ void commit_pass(const Ctx c, int nwin, unsigned long
 #endif
     if (valid) {
         st = c.status[slot];
+        // the list sizes load with the status (selected after, not loaded under the status
+        // test, which costs a second round trip)
+        const int nk0 = c.nk[slot], nf0 = c.nf[slot];
         p = win_point(wv, slot);
         key = key_hi | (unsigned)p;
+        nk = st == ST_OK ? nk0 : 0;
+        nf = st == ST_OK ? nf0 : 0;
         if (st == ST_OK) {
-            nk = c.nk[slot];
-            nf = c.nf[slot];
             // members hold their kill reservation; outside neighbors are killed by no lower key
             // (with reuse, also: members are nobody's outside neighbor with a lower key; without,
             // an outside neighbor killed by a lower key marks its facet (bit 31) for the sphere
