@@ -703,16 +703,32 @@ __device__ __forceinline__ void store_star_coords3(const Ctx& c, float4 f0, floa
     w[2] = make_float4(o[5], o[6], o[7], o[8]);
 }
 
-template <int D>
-__device__ __forceinline__ void write_star_coords(const Ctx& c, int t, int i, typename VT<D>::F pf, int N) {
+// the packed coordinates of killed simplex t (3D: three float4, 2D: three float2), loaded
+// apart from their use so that a caller can issue the load before its stores: a load placed
+// after a store to another record waits for it (the compiler cannot tell the records apart)
+template <int D> struct StarSrc;
+template <> struct StarSrc<3> { float4 f0, f1, f2; };
+template <> struct StarSrc<2> { float2 o0, o1, o2; };
+template <int D> __device__ __forceinline__ StarSrc<D> ld_star_src(const Ctx& c, int t) {
     if constexpr (D == 3) {
         const float4* q = reinterpret_cast<const float4*>(rp<D>(c, t) + Rec<D>::X);
-        store_star_coords3(c, q[0], q[1], q[2], i, pf, N);
+        return StarSrc<3>{q[0], q[1], q[2]};
     } else {
-        // scalar selects on the three loaded points (selecting elements of a private array by i
-        // became an indexed array in scratch memory, a store and two reloads per facet)
         const float2* q = reinterpret_cast<const float2*>(rp<D>(c, t) + Rec<D>::X);
-        const float2 o0 = q[0], o1 = q[1], o2 = q[2];
+        return StarSrc<2>{q[0], q[1], q[2]};
+    }
+}
+
+// coordinates of replace_first(v(t), i, p) as simplex N's record, from t's (src)
+template <int D>
+__device__ __forceinline__ void store_star_coords(const Ctx& c, const StarSrc<D>& src, int i, typename VT<D>::F pf,
+                                                  int N) {
+    if constexpr (D == 3) {
+        store_star_coords3(c, src.f0, src.f1, src.f2, i, pf, N);
+    } else {
+        // scalar selects on the three points (selecting elements of a private array by i
+        // became an indexed array in scratch memory, a store and two reloads per facet)
+        const float2 o0 = src.o0, o1 = src.o1, o2 = src.o2;
         const bool i0 = i == 0, i1 = i == 1;
         typename VT<D>::F nx[4];
         nx[0] = pf;

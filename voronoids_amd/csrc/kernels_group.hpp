@@ -705,11 +705,14 @@ __device__ __forceinline__ void commit_pass(const Ctx c, int nwin, unsigned long
             if (j < nf) {
                 const int fe = fm & 0x7fffffff;
                 const int t = fe >> 2, i = fe & 3;
-                const int4 nv = replace_first<D>(ld_v<D>(c, t), i, pv);
+                // every load of killed record t ahead of the new record's stores, in one round trip
+                const int4 tv = ld_v<D>(c, t);
                 const int outside = *a_ptr<D>(c, t, i);
+                const StarSrc<D> xs = ld_star_src<D>(c, t);
+                const int4 nv = replace_first<D>(tv, i, pv);
                 const int N = base + j;
                 st_v<D>(c, N, nv);
-                write_star_coords<D>(c, t, i, pf, N);
+                store_star_coords<D>(c, xs, i, pf, N);
                 *a_ptr<D>(c, N, 0) = outside;
                 if (sh) c.fix[fix_base + run + g.below(shm)] = make_int2(N, fe);
                 else if (outside >= 0) *a_ptr<D>(c, outside >> 2, outside & 3) = N << 2;
