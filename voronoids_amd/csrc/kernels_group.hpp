@@ -379,6 +379,9 @@ __device__ __forceinline__ void cavity_pass(const Ctx c, int nwin, unsigned long
 template <int D, int G>
 __global__ void __launch_bounds__(64, D == 3 ? VO_CAVITY_WAVES : VO_CAVITY_WAVES2) k_cavity_g(Ctx c, int nwin, unsigned long long key_hi) {
     __shared__ CavityLds<D, G> L;
+    // the shared-facet list starts empty for this round's commit (the launch before this one
+    // applied the previous round's)
+    if (blockIdx.x == 0 && threadIdx.x == 0) c.st->n_fix = 0;
     round_params(c, nwin, key_hi);
     cavity_pass<D, G>(c, nwin, key_hi, slot_iter(c, nwin, Group<G>::NG), L);
 }

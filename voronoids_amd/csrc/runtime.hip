@@ -343,6 +343,8 @@ __global__ void k_add_offsets(int* out, const int* boff, int n) {
 }
 
 
+__global__ void k_clear_fix(RoundState* st) { st->n_fix = 0; }
+
 __global__ void k_reset_round(RoundState* st) {
     st->n_next_pending = 0;
     st->n_winners = 0;
@@ -363,7 +365,16 @@ __global__ void __launch_bounds__(kCntStripes) k_round_totals(const Counters* cn
 
 __device__ void plan_step(RunPlan& p, RoundState* st, bool absorb);
 
-__global__ void __launch_bounds__(kCntStripes) k_plan_one(RunPlan* P, RoundState* st, const Counters* cnt) {
+// Block 0 plans the next round; blocks 1.. link the facets the last round's winners share (the
+// shared-facet list, as k_fix_shared), which the planner does not read: one launch for both.
+// The list count is cleared by the next cavity kernel, which every planning launch precedes.
+template <int D>
+__global__ void __launch_bounds__(256) k_plan_one(RunPlan* P, Ctx c, const Counters* cnt) {
+    if (blockIdx.x > 0) {
+        if (!c.reuse && c.fix) fix_shared_pass<D>(c, (blockIdx.x - 1) * blockDim.x + threadIdx.x, (gridDim.x - 1) * blockDim.x);
+        return;
+    }
+    RoundState* st = c.st;
     __shared__ int s_absorb;
     RunPlan& p = *P;
     if (threadIdx.x == 0) s_absorb = p.launched && !(st->n_big > 0 || (st->n_q > 0 && !p.exact_dev) || st->err);
@@ -433,7 +444,6 @@ __device__ void plan_step(RunPlan& p, RoundState* st, bool s_absorb) {
     st->n_next_pending = 0;
     st->n_winners = 0;
     st->n_q = 0;
-    st->n_fix = 0;
     st->n_live_delta = 0;
     st->n_big = 0;
     p.launched = 1;
@@ -491,6 +501,7 @@ __global__ void __launch_bounds__(64 * kBootWaves<D>) k_boot(Ctx c, RunPlan* P, 
             fix_shared_pass<D>(c, threadIdx.x, blockDim.x);
             __syncthreads();
         }
+        if (threadIdx.x == 0) c.st->n_fix = 0;     // applied: the next round's commit starts a new list
     }
     if (threadIdx.x == 0) {
         p.halt &= ~HALT_BOOT;
@@ -778,7 +789,7 @@ int auto_group(const vo_tri* t, int nwin, int which) {
 
 // nwin sizes the grid (the kernels read the exact window from c.ctl when it is set); G = 0:
 // lanes per point chosen from nwin
-int launch_round(vo_tri* t, const Ctx& c, int nwin, unsigned long long key_hi, int which, int G = 0) {
+int launch_round(vo_tri* t, const Ctx& c, int nwin, unsigned long long key_hi, int which, int G = 0, bool fix = true) {
     if (!G) G = auto_group(t, nwin, which);
     // with regions, a multiple of 8 blocks and slack for unequal region sizes
     auto blocks = [&](int per_block) {
@@ -814,7 +825,7 @@ int launch_round(vo_tri* t, const Ctx& c, int nwin, unsigned long long key_hi, i
     VO_HIP(hipGetLastError());
     // links across the facets two winners share (only without reuse; the thread commit keeps to
     // disjoint, non-adjacent cavities)
-    if (which && G != 1 && !c.reuse && c.fix) {
+    if (fix && which && G != 1 && !c.reuse && c.fix) {
         const int fb = std::max(1, std::min(1024, nwin / 64 + 1));
         if (t->D == 3) hipLaunchKernelGGL(k_fix_shared<3>, dim3(fb), dim3(256), 0, t->s, c);
         else hipLaunchKernelGGL(k_fix_shared<2>, dim3(fb), dim3(256), 0, t->s, c);
@@ -1115,6 +1126,10 @@ int run_device_planned(vo_tri* t) {
     const int D = t->D, R = t->chunk_rounds;
     int rc;
     bool booted = false;
+    // the shared-facet list of a round planned on the host is applied already; the first
+    // planning launch must not apply it again (the cavity kernels clear the count after that)
+    hipLaunchKernelGGL(k_clear_fix, dim3(1), dim3(1), 0, t->s, t->st);
+    VO_HIP(hipGetLastError());
     if (!t->plan) {
         if ((rc = dev_alloc(&t->plan, 1))) return rc;
         if (hipHostMalloc((void**)&t->plan_host, 2 * sizeof(RunPlan)) != hipSuccess) return fail(VO_ERR_OOM, "pinned");
@@ -1183,17 +1198,21 @@ int run_device_planned(vo_tri* t) {
             const int Gc = auto_group(t, est_mid, 0), Gm = auto_group(t, est_mid, 1);
             const int slot = k & 1;
             for (int r = 0; r < R; ++r) {
-                hipLaunchKernelGGL(k_plan_one, dim3(1), dim3(kCntStripes), 0, t->s, t->plan, t->st, t->cnt);
                 // the window is virtual: the kernels read the pending list, then count fresh points
                 Ctx c = make_ctx(t);
                 c.ctl = &t->plan->ctl;
                 c.win = t->pend[cur];
                 c.pend_next = t->pend[cur ^ 1];
+                // the plan, with the previous round's shared-facet links (the thread kernels of
+                // group_lanes 1 leave none)
+                const int fixb = (!c.reuse && c.fix && t->group != 1) ? std::max(1, std::min(1024, est_hi / 64 + 1)) : 0;
+                if (D == 3) hipLaunchKernelGGL(k_plan_one<3>, dim3(1 + fixb), dim3(256), 0, t->s, t->plan, c, t->cnt);
+                else hipLaunchKernelGGL(k_plan_one<2>, dim3(1 + fixb), dim3(256), 0, t->s, t->plan, c, t->cnt);
                 hipEvent_t* e = &t->rev[(size_t)(slot * R + r) * 3];
                 if (t->round_timing) VO_HIP(hipEventRecord(e[0], t->s));
                 if ((rc = launch_round(t, c, est_hi, 0, 0, Gc))) return rc;
                 if (t->round_timing) VO_HIP(hipEventRecord(e[1], t->s));
-                if ((rc = launch_round(t, c, est_hi, 0, 1, Gm))) return rc;
+                if ((rc = launch_round(t, c, est_hi, 0, 1, Gm, false))) return rc;
                 if (t->round_timing) VO_HIP(hipEventRecord(e[2], t->s));
                 if (t->exact_dev) {
                     // the round's queued predicates (count read from the round state; none: the
