@@ -109,6 +109,17 @@ struct BatchCtl {
                            // pending list (Ctx::win), slot i >= npend is point vhead + i - npend
 };
 
+// Simplex slots of one set's device-planned rounds come from kAllocStripes chunks that the
+// planner reserves for each round at the top of the record array (AllocStripe: next free, end);
+// a commit wavefront takes its range from the chunk of stripe blockIdx % kAllocStripes, and from
+// the shared tail (RoundState::n_tets) once that chunk is spent.  Returning atomics on one
+// address serialize (about 11 ns each on MI355X), so a single counter for every wavefront of a
+// large round was a chain as long as the round's wavefronts.
+constexpr int kAllocStripes = 64;
+struct alignas(256) AllocStripe {
+    int cur, end;
+};
+
 struct Ctx {
     char* rec;             // simplex records (layout in Rec<D>)
     const double* X64;
@@ -134,6 +145,7 @@ struct Ctx {
     float ginv;            // 1 / grid extent
     int hlevels;           // finest level
     RoundState* st;
+    AllocStripe* ast = nullptr;   // one set, device-planned rounds: the round's allocation chunks
     Counters* cnt;
     QEntry* q;
     int qcap;
@@ -454,7 +466,15 @@ __device__ inline int alloc_slots(const Ctx& c, bool req, int n, int p) {
         int total = 0;
         const int off = wave_excl_scan(req ? n : 0, total);
         int base0 = 0;
-        if (__lane_id() == 0 && total > 0) base0 = atomicAdd(&c.st->n_tets, total);
+        if (__lane_id() == 0 && total > 0) {
+            base0 = -1;
+            if (c.ast) {
+                AllocStripe& a = c.ast[blockIdx.x & (kAllocStripes - 1)];
+                const int b = atomicAdd(&a.cur, total);
+                if (b + total <= a.end) base0 = b;
+            }
+            if (base0 < 0) base0 = atomicAdd(&c.st->n_tets, total);
+        }
         base0 = __shfl(base0, 0, 64);
         if (!req) return -1;
         if (base0 + off + n > c.tet_cap) {

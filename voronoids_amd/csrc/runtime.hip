@@ -92,6 +92,9 @@ struct RunPlan {
     int min_window;            // least window size
     int boot_limit;            // k_boot: the planner halts (HALT_BOOT) before a window larger than
     int boot_rounds_end;       // this, or once `rounds` reaches boot_rounds_end (0: never)
+    int chunk_rate_milli;      // records reserved per window point x 1000 across the allocation
+                               // stripes of a round (0: no stripes)
+    int chunk;                 // the planned round's chunk per stripe
 };
 
 struct vo_tri {
@@ -172,6 +175,8 @@ struct vo_tri {
     int round_timing = 0;              // events around each round's kernels (cost: ~3% at n = 4M)
     int abort_early = 1;               // losing cavity searches stop at the first reservation they lose
     int walk_budget = 0;               // lane-group walk moves per round before a point pauses (0 = off)
+    int alloc_stripes = 1;             // device-planned rounds allocate from per-round striped chunks
+    AllocStripe* ast = nullptr;        // their cursors (kAllocStripes)
     int boot_window = 0;               // first rounds in one workgroup (k_boot) while windows stay at
                                        // or below this many points (0 = off; set per dimension)
     int reuse = -1;                    // new simplices take their winner's killed records first: 1 on,
@@ -375,18 +380,29 @@ __global__ void __launch_bounds__(256) k_plan_one(RunPlan* P, Ctx c, const Count
         return;
     }
     RoundState* st = c.st;
-    __shared__ int s_absorb;
+    __shared__ int s_absorb, s_base, s_chunk;
     RunPlan& p = *P;
     if (threadIdx.x == 0) s_absorb = p.launched && !(st->n_big > 0 || (st->n_q > 0 && !p.exact_dev) || st->err);
     __syncthreads();
     if (s_absorb) round_totals(cnt, st);
-    if (threadIdx.x != 0) return;
-    plan_step(p, st, s_absorb);
-    // halted with the last queue already evaluated in the stream: the empty rounds that follow
-    // must not evaluate it again (the host reads the count from nq_done)
-    if (p.halt && p.exact_dev && st->n_q > 0) {
-        p.nq_done = st->n_q;
-        st->n_q = 0;
+    if (threadIdx.x == 0) {
+        plan_step(p, st, s_absorb);
+        // halted with the last queue already evaluated in the stream: the empty rounds that
+        // follow must not evaluate it again (the host reads the count from nq_done)
+        if (p.halt && p.exact_dev && st->n_q > 0) {
+            p.nq_done = st->n_q;
+            st->n_q = 0;
+        }
+        // a planned round: its allocation chunks at the top of the records (the space check
+        // of plan_step counted them)
+        s_chunk = (c.ast && p.launched && p.ctl.nwin > 0) ? p.chunk : 0;
+        s_base = st->n_tets;
+        if (s_chunk) st->n_tets = s_base + kAllocStripes * s_chunk;
+    }
+    __syncthreads();
+    if (s_chunk && threadIdx.x < kAllocStripes) {
+        c.ast[threadIdx.x].cur = s_base + threadIdx.x * s_chunk;
+        c.ast[threadIdx.x].end = s_base + (threadIdx.x + 1) * s_chunk;
     }
 }
 
@@ -430,12 +446,15 @@ __device__ void plan_step(RunPlan& p, RoundState* st, bool s_absorb) {
         p.halt |= HALT_BOOT;
         return;
     }
-    if ((long long)st->n_tets + (long long)nwin * p.per_point + 1024 > p.tet_cap) {
+    const long long chunk =
+        p.chunk_rate_milli > 0 ? ((long long)nwin * p.chunk_rate_milli / 1000 + kAllocStripes - 1) / kAllocStripes + 16 : 0;
+    if ((long long)st->n_tets + (long long)nwin * p.per_point + chunk * kAllocStripes + 1024 > p.tet_cap) {
         p.halt |= HALT_SPACE;
         return;
     }
     p.nfresh = nfresh;
     p.nwin = nwin;
+    p.chunk = (int)chunk;
     p.ctl.nwin = nwin;
     p.ctl.npend = p.npend;
     p.ctl.virt = 1;            // pending points, then fresh points head .. (not written out)
@@ -588,7 +607,7 @@ void free_all(vo_tri* t) {
     void* ptrs[] = {t->X64, t->X32, t->cp, t->vmap, t->rec, t->rec_spare, t->cmap, t->chist, t->reg_meta, t->win2,
                     t->regs, t->win, t->pend[0], t->pend[1],
                     t->cav, t->fac, t->fnb, t->nk, t->nf, t->status, t->hint, t->st, t->cnt, t->q, t->cache, t->ws,
-                    t->big, t->cav1, t->fac1, t->misc1, t->plan, t->fix};
+                    t->big, t->cav1, t->fac1, t->misc1, t->plan, t->fix, t->ast};
     for (void* p : ptrs)
         if (p) hipFree(p);
 #ifdef VO_POINT_PROBE
@@ -745,8 +764,6 @@ int compact(vo_tri* t) {
     if (t->npend) hipLaunchKernelGGL(k_remap_points, dim3(256), dim3(256), 0, t->s, t->cp, pend, t->npend, t->cmap);
     hipLaunchKernelGGL(k_remap_hints, dim3(1024), dim3(256), 0, t->s, t->hint, cells, t->cmap);
     VO_HIP(hipGetLastError());
-    // records past the live ones keep stale contents: the commit writes every field of a new
-    // record except its reservations, and stale keys are larger than any later round's
     VO_HIP(hipMemcpyAsync(&t->st->n_tets, &live, sizeof(int), hipMemcpyHostToDevice, t->s));
     VO_HIP(hipEventRecord(t->ev[5], t->s));
     if ((rc = sync_state(t)) || (rc = check_err(t))) return rc;
@@ -754,6 +771,12 @@ int compact(vo_tri* t) {
     hipEventElapsedTime(&ms, t->ev[4], t->ev[5]);
     t->ms_compact += ms;
     std::swap(t->rec, t->rec_spare);
+    // the records past the live ones are made fresh again (all 0xFF: dead): the allocation
+    // stripes leave unused slots below the count, which must not look live
+    {
+        const size_t B = (size_t)record_bytes(t->D);
+        VO_HIP(hipMemsetAsync(t->rec + (size_t)live * B, 0xFF, (size_t)(t->tcap - live) * B, t->s));
+    }
     t->retired += n - live;
     t->n_tets = live;
     t->n_live = live;
@@ -915,7 +938,19 @@ void choose_reuse(vo_tri* t, int64_t to_insert) {
 // reuse all but about 7 (3D) / 2 (2D) of them take the winner's killed records
 int64_t records_per_point(const vo_tri* t) {
     if (t->reuse_on) return t->D == 3 ? 12 : 4;
+    // allocation stripes leave the unused part of each round's chunks behind (measured: 35-39
+    // records per point in 3D, uniform and clustered; 8.5 in 2D)
+    if (t->alloc_stripes) return t->D == 3 ? 44 : 12;
     return t->D == 3 ? 30 : 8;
+}
+
+// records a planned round may claim: per_point for each window point, plus the stripes' chunks
+// (the planner's space check, plan_step)
+int chunk_rate_milli(const vo_tri* t) { return (t->alloc_stripes && !t->reuse_on) ? (t->D == 3 ? 25000 : 5400) : 0; }
+int64_t round_records(const vo_tri* t, int64_t nwin) {
+    const int64_t rate = chunk_rate_milli(t);
+    const int64_t chunk = rate ? (nwin * rate / 1000 + kAllocStripes - 1) / kAllocStripes + 16 : 0;
+    return nwin * window_records(t->D) + chunk * kAllocStripes + 1024;
 }
 
 // The host's side of a finished round whose state is in st_host: errors, points re-run alone,
@@ -1067,6 +1102,7 @@ RunPlan plan_from_host(const vo_tri* t) {
     p.per_point = (int)window_records(t->D);
     p.exact_dev = t->exact_dev;
     p.min_window = t->min_window;
+    p.chunk_rate_milli = chunk_rate_milli(t);
     return p;
 }
 
@@ -1152,7 +1188,7 @@ int run_device_planned(vo_tri* t) {
         if (t->npend > t->win_cap) return fail(VO_ERR_INTERNAL, "pending list beyond the window capacity");
         {
             const int nwin = host_window(t).first;
-            if ((rc = ensure_records(t, (int64_t)t->n_tets + (int64_t)nwin * window_records(D) + 1024))) return rc;
+            if ((rc = ensure_records(t, (int64_t)t->n_tets + round_records(t, nwin)))) return rc;
         }
         t->plan_host[0] = plan_from_host(t);
         // small first windows: rounds in one workgroup (k_boot) until the window outgrows
@@ -1203,6 +1239,7 @@ int run_device_planned(vo_tri* t) {
                 c.ctl = &t->plan->ctl;
                 c.win = t->pend[cur];
                 c.pend_next = t->pend[cur ^ 1];
+                if (chunk_rate_milli(t)) c.ast = t->ast;
                 // the plan, with the previous round's shared-facet links (the thread kernels of
                 // group_lanes 1 leave none)
                 const int fixb = (!c.reuse && c.fix && t->group != 1) ? std::max(1, std::min(1024, est_hi / 64 + 1)) : 0;
@@ -1558,7 +1595,7 @@ int vo_tri_create(int hip_device, int dim, const double* pts, int64_t n, vo_tri*
         (rc = dev_alloc(&t->cache, (size_t)t->cmask + 1)) || (rc = dev_alloc(&t->ws, (size_t)t->ws_threads * t->ws_cap)) ||
         (rc = dev_alloc(&t->big, t->win_cap)) || (rc = dev_alloc(&t->cav1, t->big_cap)) ||
         (rc = dev_alloc(&t->fac1, t->big_cap)) || (rc = dev_alloc(&t->misc1, 4)) ||
-        (rc = dev_alloc(&t->fix, (size_t)2 * t->win_cap)))
+        (rc = dev_alloc(&t->fix, (size_t)2 * t->win_cap)) || (rc = dev_alloc(&t->ast, kAllocStripes)))
         return bail(rc);
     t->fix_cap = 2 * t->win_cap;
     if (hipHostMalloc((void**)&t->st_host, sizeof(RoundState)) != hipSuccess) return bail(fail(VO_ERR_OOM, "pinned"));
@@ -1607,6 +1644,10 @@ int vo_tri_reset(vo_tri* t, const double* pts, int64_t n) {
     // of the new set is below the keys left in the records of the old one
     t->n_pts = t->head = 0;
     t->input_of.clear();
+    // records above the count stay fresh (all 0xFF: dead), which the allocation stripes rely on:
+    // a chunk's unused slots lie below the count and must not look live
+    if (t->n_tets > 1)
+        VO_HIP(hipMemsetAsync(t->rec + record_bytes(dim), 0xFF, (size_t)(t->n_tets - 1) * record_bytes(dim), t->s));
     t->n_tets = 1;
     t->n_live = 1;
     t->retired = 0;
@@ -1810,6 +1851,8 @@ int vo_tri_export(const vo_tri* t, int32_t* simplices, int32_t* neighbors) {
     int64_t nl;
     int rc = export_device(t, ds, dn, &nl);
     if (rc) return rc;
+    // the caller's arrays hold n_live rows (vo_tri_counts): never write past them
+    if (nl != t->n_live) return fail(VO_ERR_INTERNAL, "live records differ from the live simplex count");
     const size_t bytes = sizeof(int32_t) * (size_t)nl * t->M;
     if (simplices && nl) VO_HIP(hipMemcpy(simplices, ds.p, bytes, hipMemcpyDeviceToHost));
     if (neighbors && nl) VO_HIP(hipMemcpy(neighbors, dn.p, bytes, hipMemcpyDeviceToHost));
@@ -2195,6 +2238,9 @@ int vo_tri_config(vo_tri* t, const char* key, int64_t value) {
     } else if (k == "abort_early") {
         if (value != 0 && value != 1) return fail(VO_ERR_INVALID, "abort_early must be 0 or 1");
         t->abort_early = (int)value;
+    } else if (k == "alloc_stripes") {
+        if (value != 0 && value != 1) return fail(VO_ERR_INVALID, "alloc_stripes must be 0 or 1");
+        t->alloc_stripes = (int)value;
     } else if (k == "boot_window") {
         if (value < 0 || value > 65536) return fail(VO_ERR_INVALID, "boot_window must be in 0..65536");
         t->boot_window = (int)value;
