@@ -249,9 +249,12 @@ def test_device_planned_rounds_match_host_planned(dim):
     Q = datasets.uniform(20000, dim, seed=150 + dim) * 0.5 + 0.25
     ref = O.Tree.delaunay(np.vstack([P, Q]), O.MODE_EXACT).canonical()
     seen = []
-    # boot: the first rounds in one workgroup (k_boot) off, or on up to 256-point windows
-    for pipelined, chunk, boot in ((0, 4, 0), (1, 1, 0), (1, 4, 0), (1, 7, 0), (1, 4, 256), (1, 1, 128)):
+    # boot: the first rounds in one workgroup (k_boot) off, or on up to 256-point windows;
+    # flags: the commit's per-slot outcomes compacted by the planning launch, or appended
+    for pipelined, chunk, boot, flags in ((0, 4, 0, 1), (1, 1, 0, 1), (1, 4, 0, 1), (1, 7, 0, 1), (1, 4, 256, 1),
+                                          (1, 1, 128, 1), (1, 4, 0, 0)):
         t = vo.DelaunayTree(np.vstack([P, Q]))
+        t.config("outcome_flags", flags)
         t.config("abort_early", 0)              # the schedule that does not depend on timing
         t.config("walk_budget", 0)              # (a walk budget pauses walks whose length depends on
         t.config("pipelined", pipelined)        # which concurrent write a hint cell kept)
@@ -260,7 +263,7 @@ def test_device_planned_rounds_match_host_planned(dim):
         t.add_points_to_tree(P, seed=8)
         t.add_points_to_tree(Q, seed=9)
         T, E = t.canonical()
-        assert np.array_equal(T, ref.T) and np.array_equal(E, ref.E), (pipelined, chunk, boot)
+        assert np.array_equal(T, ref.T) and np.array_equal(E, ref.E), (pipelined, chunk, boot, flags)
         st = t.stats()
         # (rounds run inside the one-workgroup kernel are not launches of the round kernels)
         seen.append((st["rounds"], st["points_inserted"], st["windows_total"], st["created"], st["killed"],
@@ -281,20 +284,22 @@ def test_device_planned_rounds_hand_over_exact_and_solo_rounds():
         P = np.vstack([0.5 + 0.3 * u / np.linalg.norm(u, axis=1)[:, None], 0.5 + 0.05 * rng.random((60, dim))])
         ref = O.Tree.delaunay(P, O.MODE_EXACT).canonical()
         seen = []
-        # the one-workgroup first rounds (boot 256) halt for the host on such rounds as well
-        for pipelined, boot in ((0, 0), (1, 0), (1, 256)):
+        # the one-workgroup first rounds (boot 256) halt for the host on such rounds as well, and
+        # so do rounds whose pending list is appended rather than compacted (flags 0)
+        for pipelined, boot, flags in ((0, 0, 1), (1, 0, 1), (1, 256, 1), (1, 0, 0)):
             t = vo.DelaunayTree(P)
+            t.config("outcome_flags", flags)
             t.config("abort_early", 0)
             t.config("walk_budget", 0)
             t.config("pipelined", pipelined)
             t.config("boot_window", boot)
             t.add_points_to_tree(P, seed=11)
             T, E = t.canonical()
-            assert np.array_equal(T, ref.T) and np.array_equal(E, ref.E), (dim, pipelined, boot)
+            assert np.array_equal(T, ref.T) and np.array_equal(E, ref.E), (dim, pipelined, boot, flags)
             st = t.stats()
             assert st["exact_evaluated"] > 0
             seen.append((st["rounds"], st["points_inserted"], st["windows_total"], st["solo_runs"]))
-        assert seen[0] == seen[1] == seen[2], (dim, seen)
+        assert all(x == seen[0] for x in seen), (dim, seen)
         solo += seen[1][3]
     assert solo > 0
 

@@ -48,6 +48,7 @@ struct alignas(256) Counters {
     unsigned long long t1[2], t2[2], t3[2], def[2];
     unsigned long long walk_steps, chain_hops, killed, created, winners, dups, exact_evaluated;
     unsigned long long max_killed, max_facets;
+    unsigned long long pend;       // points sent on to the next window
 };
 
 // Round totals.  The two that every wavefront of a round updates atomically sit on lines of
@@ -66,6 +67,9 @@ struct RoundState {
     int n_live_delta;                 // sum over its winners of (created - killed) (round_totals)
     unsigned long long acc_winners;   // statistics totals at the last round_totals call
     long long acc_live;
+    unsigned long long acc_pend;
+    int flag_n;                       // window slots of the last round whose outcomes are in Ctx::pflag
+    alignas(256) int n_compact;       // pending points placed by the compaction of Ctx::pflag
 };
 
 // Several independent point sets sharing one device arena (batched and served modes).
@@ -146,6 +150,10 @@ struct Ctx {
     int hlevels;           // finest level
     RoundState* st;
     AllocStripe* ast = nullptr;   // one set, device-planned rounds: the round's allocation chunks
+    int* pflag = nullptr;  // one set, device-planned rounds: the lane-group commit writes each window
+                           // slot's outcome here (its point if it goes on to the next window, else
+                           // -1) instead of appending to pend_next; the next planning launch
+                           // compacts it (compact_pending) and the statistics give the count
     Counters* cnt;
     QEntry* q;
     int qcap;
@@ -347,7 +355,7 @@ template <int D> __device__ __forceinline__ int4 replace_first(int4 v, int i, in
 
 struct LocalCnt {
     unsigned t1[2], t2[2], t3[2], def[2];
-    unsigned walk, chain, killed, created, wins, dups;
+    unsigned walk, chain, killed, created, wins, dups, pend;
     unsigned maxk, maxf;
 };
 
@@ -382,6 +390,7 @@ inline Counters sum_counters(const Counters* stripes) {
         r.created += x.created;
         r.winners += x.winners;
         r.dups += x.dups;
+        r.pend += x.pend;
         r.exact_evaluated += x.exact_evaluated;
         r.max_killed = x.max_killed > r.max_killed ? x.max_killed : r.max_killed;
         r.max_facets = x.max_facets > r.max_facets ? x.max_facets : r.max_facets;
@@ -400,7 +409,7 @@ __device__ inline void flush_counters(Counters* c, const LocalCnt& l) {
     VO_FLUSH(t1[0], l.t1[0]) VO_FLUSH(t1[1], l.t1[1]) VO_FLUSH(t2[0], l.t2[0]) VO_FLUSH(t2[1], l.t2[1])
     VO_FLUSH(t3[0], l.t3[0]) VO_FLUSH(t3[1], l.t3[1]) VO_FLUSH(def[0], l.def[0]) VO_FLUSH(def[1], l.def[1])
     VO_FLUSH(walk_steps, l.walk) VO_FLUSH(chain_hops, l.chain) VO_FLUSH(killed, l.killed)
-    VO_FLUSH(created, l.created) VO_FLUSH(winners, l.wins) VO_FLUSH(dups, l.dups)
+    VO_FLUSH(created, l.created) VO_FLUSH(winners, l.wins) VO_FLUSH(dups, l.dups) VO_FLUSH(pend, l.pend)
 #undef VO_FLUSH
     unsigned mk = wave_max(l.maxk), mf = wave_max(l.maxf);
     if (lead) {
@@ -411,35 +420,43 @@ __device__ inline void flush_counters(Counters* c, const LocalCnt& l) {
 
 // Winners and the change in live simplices since the last call, from the striped statistics
 // (each commit wavefront adds its winners, created and killed simplices there), into
-// st->n_winners / st->n_live_delta.  Called by every thread of one block (block-uniform).
-__device__ inline void round_totals(const Counters* cnt, RoundState* st) {
-    __shared__ unsigned long long s_w[16];
+// st->n_winners / st->n_live_delta, and with pend_count the points sent on to the next window
+// into st->n_next_pending (rounds whose commit wrote Ctx::pflag instead of appending).  Called
+// by every thread of one block (block-uniform).
+__device__ inline void round_totals(const Counters* cnt, RoundState* st, bool pend_count = false) {
+    __shared__ unsigned long long s_w[16], s_p[16];
     __shared__ long long s_l[16];
-    unsigned long long w = 0;
+    unsigned long long w = 0, pn = 0;
     long long l = 0;
     for (int k = threadIdx.x; k < kCntStripes; k += blockDim.x) {
         w += cnt[k].winners;
+        pn += cnt[k].pend;
         l += (long long)cnt[k].created - (long long)cnt[k].killed;
     }
     w = wave_sum(w);
+    pn = wave_sum(pn);
     l = wave_sum_i64(l);
     const int wv = threadIdx.x / 64, nw = (blockDim.x + 63) / 64;
     if (__lane_id() == 0 && wv < 16) {
         s_w[wv] = w;
+        s_p[wv] = pn;
         s_l[wv] = l;
     }
     __syncthreads();
     if (threadIdx.x == 0) {
-        unsigned long long W = 0;
+        unsigned long long W = 0, P = 0;
         long long L = 0;
         for (int i = 0; i < nw && i < 16; ++i) {
             W += s_w[i];
+            P += s_p[i];
             L += s_l[i];
         }
         st->n_winners = (int)(W - st->acc_winners);
         st->n_live_delta = (int)(L - st->acc_live);
+        if (pend_count) st->n_next_pending = (int)(P - st->acc_pend);
         st->acc_winners = W;
         st->acc_live = L;
+        st->acc_pend = P;
     }
     __syncthreads();
 }
@@ -1085,6 +1102,7 @@ __global__ void __launch_bounds__(256) k_commit(Ctx c, int nwin, unsigned long l
     // next window: losers (valid cavity, lost a reservation) and deferred points
     const unsigned long long mask = __ballot(again);
     const int cnt = __popcll(mask);
+    if (again) lc.pend++;
     int pos0 = 0;
     if (__lane_id() == 0 && cnt) pos0 = atomicAdd(&c.st->n_next_pending, cnt);
     pos0 = __shfl(pos0, 0, 64);

@@ -381,8 +381,14 @@ __global__ void __launch_bounds__(64, D == 3 ? VO_CAVITY_WAVES : VO_CAVITY_WAVES
     __shared__ CavityLds<D, G> L;
     // the shared-facet list starts empty for this round's commit (the launch before this one
     // applied the previous round's)
-    if (blockIdx.x == 0 && threadIdx.x == 0) c.st->n_fix = 0;
     round_params(c, nwin, key_hi);
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        c.st->n_fix = 0;
+        // the slots whose outcomes this round's commit writes to pflag, for the next planning
+        // launch; an empty round (after a planner halt) keeps the last compaction's count
+        c.st->flag_n = c.pflag ? nwin : 0;
+        if (nwin > 0) c.st->n_compact = 0;
+    }
     cavity_pass<D, G>(c, nwin, key_hi, slot_iter(c, nwin, Group<G>::NG), L);
 }
 
@@ -840,15 +846,21 @@ __device__ __forceinline__ void commit_pass(const Ctx c, int nwin, unsigned long
         }
     }
     set_outcome(c, lead && st != ST_IDLE, p, build, nf - nk, !build && again, !build && !again && st == ST_DUP);
-    const unsigned long long mask = __ballot(again);
-    const int cnt = __popcll(mask);
-    int pos0 = 0;
-    if (g.lane == 0 && cnt) pos0 = atomicAdd(&c.st->n_next_pending, cnt);
-    pos0 = __shfl(pos0, 0, 64);
-    if (again) {
-        const int pos = pos0 + __popcll(mask & ((1ull << g.lane) - 1ull));
-        if (pos < c.pend_cap) c.pend_next[pos] = p;
-        else atomicOr(&c.st->err, ERR_PENDING);
+    if (again) lc.pend++;
+    if (c.pflag) {
+        // the outcome at the slot's own position: no returning atomic on one counter per wavefront
+        if (lead && valid) c.pflag[slot] = again ? p : -1;
+    } else {
+        const unsigned long long mask = __ballot(again);
+        const int cnt = __popcll(mask);
+        int pos0 = 0;
+        if (g.lane == 0 && cnt) pos0 = atomicAdd(&c.st->n_next_pending, cnt);
+        pos0 = __shfl(pos0, 0, 64);
+        if (again) {
+            const int pos = pos0 + __popcll(mask & ((1ull << g.lane) - 1ull));
+            if (pos < c.pend_cap) c.pend_next[pos] = p;
+            else atomicOr(&c.st->err, ERR_PENDING);
+        }
     }
     __builtin_amdgcn_wave_barrier();       // s_fac is reused by the next slots
     }

@@ -177,6 +177,9 @@ struct vo_tri {
     int walk_budget = 0;               // lane-group walk moves per round before a point pauses (0 = off)
     int alloc_stripes = 1;             // device-planned rounds allocate from per-round striped chunks
     AllocStripe* ast = nullptr;        // their cursors (kAllocStripes)
+    int outcome_flags = 1;             // device-planned rounds: the commit writes each slot's outcome
+                                       // and the planning launch compacts the pending list
+    int* pflag = nullptr;              // those outcomes (win_cap)
     int boot_window = 0;               // first rounds in one workgroup (k_boot) while windows stay at
                                        // or below this many points (0 = off; set per dimension)
     int reuse = -1;                    // new simplices take their winner's killed records first: 1 on,
@@ -348,7 +351,13 @@ __global__ void k_add_offsets(int* out, const int* boff, int n) {
 }
 
 
-__global__ void k_clear_fix(RoundState* st) { st->n_fix = 0; }
+// the lists a device-planned chunk starts from are complete: no planning launch may apply the
+// shared-facet list or compact the outcome flags of a round finished before it
+__global__ void k_clear_fix(RoundState* st) {
+    st->n_fix = 0;
+    st->flag_n = 0;
+    st->n_compact = 0;
+}
 
 __global__ void k_reset_round(RoundState* st) {
     st->n_next_pending = 0;
@@ -370,13 +379,50 @@ __global__ void __launch_bounds__(kCntStripes) k_round_totals(const Counters* cn
 
 __device__ void plan_step(RunPlan& p, RoundState* st, bool absorb);
 
+// The last round's outcome flags (Ctx::pflag, st->flag_n slots) into the pending list dst, by the
+// blocks 1.. of a planning launch: 1024 slots per step and block, one atomic per step for the
+// block's range of dst (order within the list is free: the next window's points are the same).
+__device__ void compact_pending(const Ctx& c, int* dst) {
+    __shared__ int s_w[4], s_base;
+    const int n = c.st->flag_n;
+    const int wv = threadIdx.x >> 6, lane = (int)__lane_id();
+    for (int b0 = (blockIdx.x - 1) * 1024; b0 < n; b0 += (gridDim.x - 1) * 1024) {
+        int v[4], k = 0;
+        for (int j = 0; j < 4; ++j) {
+            const int i = b0 + j * 256 + threadIdx.x;
+            v[j] = i < n ? c.pflag[i] : -1;
+            k += v[j] >= 0;
+        }
+        int tot = 0;
+        const int off = wave_excl_scan(k, tot);
+        if (lane == 0) s_w[wv] = tot;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            const int all = s_w[0] + s_w[1] + s_w[2] + s_w[3];
+            s_base = all ? atomicAdd(&c.st->n_compact, all) : 0;
+        }
+        __syncthreads();
+        int pos = s_base + off;
+        for (int w = 0; w < wv; ++w) pos += s_w[w];
+        for (int j = 0; j < 4; ++j) {
+            if (v[j] < 0) continue;
+            if (pos < c.pend_cap) dst[pos] = v[j];
+            else atomicOr(&c.st->err, ERR_PENDING);
+            ++pos;
+        }
+        __syncthreads();                   // s_w and s_base are rewritten by the next step
+    }
+}
+
 // Block 0 plans the next round; blocks 1.. link the facets the last round's winners share (the
-// shared-facet list, as k_fix_shared), which the planner does not read: one launch for both.
+// shared-facet list, as k_fix_shared) and compact its outcome flags into pend_dst, the pending
+// list of the next window (with Ctx::pflag).  The planner reads neither: one launch for all.
 // The list count is cleared by the next cavity kernel, which every planning launch precedes.
 template <int D>
-__global__ void __launch_bounds__(256) k_plan_one(RunPlan* P, Ctx c, const Counters* cnt) {
+__global__ void __launch_bounds__(256) k_plan_one(RunPlan* P, Ctx c, const Counters* cnt, int* pend_dst) {
     if (blockIdx.x > 0) {
         if (!c.reuse && c.fix) fix_shared_pass<D>(c, (blockIdx.x - 1) * blockDim.x + threadIdx.x, (gridDim.x - 1) * blockDim.x);
+        if (c.pflag) compact_pending(c, pend_dst);
         return;
     }
     RoundState* st = c.st;
@@ -384,7 +430,9 @@ __global__ void __launch_bounds__(256) k_plan_one(RunPlan* P, Ctx c, const Count
     RunPlan& p = *P;
     if (threadIdx.x == 0) s_absorb = p.launched && !(st->n_big > 0 || (st->n_q > 0 && !p.exact_dev) || st->err);
     __syncthreads();
-    if (s_absorb) round_totals(cnt, st);
+    // with outcome flags the count of points sent on comes from the statistics (the compaction
+    // runs beside this block)
+    if (s_absorb) round_totals(cnt, st, c.pflag != nullptr);
     if (threadIdx.x == 0) {
         plan_step(p, st, s_absorb);
         // halted with the last queue already evaluated in the stream: the empty rounds that
@@ -607,7 +655,7 @@ void free_all(vo_tri* t) {
     void* ptrs[] = {t->X64, t->X32, t->cp, t->vmap, t->rec, t->rec_spare, t->cmap, t->chist, t->reg_meta, t->win2,
                     t->regs, t->win, t->pend[0], t->pend[1],
                     t->cav, t->fac, t->fnb, t->nk, t->nf, t->status, t->hint, t->st, t->cnt, t->q, t->cache, t->ws,
-                    t->big, t->cav1, t->fac1, t->misc1, t->plan, t->fix, t->ast};
+                    t->big, t->cav1, t->fac1, t->misc1, t->plan, t->fix, t->ast, t->pflag};
     for (void* p : ptrs)
         if (p) hipFree(p);
 #ifdef VO_POINT_PROBE
@@ -930,7 +978,10 @@ int64_t window_records(int D) { return D == 3 ? 64 : 16; }
 // simplices scatter over its killed records' lines instead of filling fresh consecutive ones:
 // measured 3D n = 4M 147 -> 154 ms, 2D n = 10M 99 -> 114 ms) and needs the stricter round rule.
 void choose_reuse(vo_tri* t, int64_t to_insert) {
-    const int64_t append_only = (int64_t)t->n_tets + (t->D == 3 ? 30 : 8) * to_insert + 4096;
+    // append-only growth per point: about 27 (3D) / 6 (2D) live simplices, and with allocation
+    // stripes the unused chunk tails (measured up to 39 / 8.5 records per point in all)
+    const int64_t per = t->alloc_stripes ? (t->D == 3 ? 40 : 9) : (t->D == 3 ? 30 : 8);
+    const int64_t append_only = (int64_t)t->n_tets + per * to_insert + 4096;
     t->reuse_on = t->reuse == 1 || (t->reuse < 0 && append_only > ((int64_t)1 << 29));
 }
 
@@ -1140,9 +1191,16 @@ void add_chunk_times(vo_tri* t, int slot, int R) {
 // planned window.
 // The host's view after the device planner halted with snapshot P (the stream drained): its
 // counters and lists, and the planned round it could not absorb, if any, finished on the host
-int absorb_halted_plan(vo_tri* t, const RunPlan& P) {
+int absorb_halted_plan(vo_tri* t, const RunPlan& P, bool flags = false) {
     int rc;
     if ((rc = sync_state(t))) return rc;
+    // a round the planner did not absorb, run with outcome flags: its pending list was compacted
+    // by the halting launch (count in n_compact); the host's finish appends after it
+    if (flags && P.launched) {
+        t->st_host->n_next_pending = t->st_host->n_compact;
+        VO_HIP(hipMemcpyAsync(&t->st->n_next_pending, &t->st_host->n_next_pending, sizeof(int), hipMemcpyHostToDevice,
+                              t->s));
+    }
     // launches counted for the rounds that ran (the empty rounds after a halt are not)
     t->n_cav += P.rounds - t->rounds + P.launched;
     t->n_com += P.rounds - t->rounds + P.launched;
@@ -1162,10 +1220,6 @@ int run_device_planned(vo_tri* t) {
     const int D = t->D, R = t->chunk_rounds;
     int rc;
     bool booted = false;
-    // the shared-facet list of a round planned on the host is applied already; the first
-    // planning launch must not apply it again (the cavity kernels clear the count after that)
-    hipLaunchKernelGGL(k_clear_fix, dim3(1), dim3(1), 0, t->s, t->st);
-    VO_HIP(hipGetLastError());
     if (!t->plan) {
         if ((rc = dev_alloc(&t->plan, 1))) return rc;
         if (hipHostMalloc((void**)&t->plan_host, 2 * sizeof(RunPlan)) != hipSuccess) return fail(VO_ERR_OOM, "pinned");
@@ -1184,8 +1238,16 @@ int run_device_planned(vo_tri* t) {
             std::min<int64_t>((int64_t)t->n_tets + left * records_per_point(t) + 4096, (int64_t)1 << 29);
         if ((rc = ensure_records(t, want))) return rc;
     }
+    // the lane-group commit of device-planned rounds writes outcome flags (the planning launch's
+    // extra blocks, which link shared facets, compact them)
+    const bool flags = t->outcome_flags && !t->reuse_on && t->shared_facets && t->group != 1;
     while (t->head < t->n_pts || t->npend > 0) {
         if (t->npend > t->win_cap) return fail(VO_ERR_INTERNAL, "pending list beyond the window capacity");
+        // the shared-facet list and outcome flags of a round finished before this point are
+        // applied already; the first planning launch must not apply them again (the cavity
+        // kernels clear the counts after that)
+        hipLaunchKernelGGL(k_clear_fix, dim3(1), dim3(1), 0, t->s, t->st);
+        VO_HIP(hipGetLastError());
         {
             const int nwin = host_window(t).first;
             if ((rc = ensure_records(t, (int64_t)t->n_tets + round_records(t, nwin)))) return rc;
@@ -1240,11 +1302,14 @@ int run_device_planned(vo_tri* t) {
                 c.win = t->pend[cur];
                 c.pend_next = t->pend[cur ^ 1];
                 if (chunk_rate_milli(t)) c.ast = t->ast;
+                if (flags) c.pflag = t->pflag;
                 // the plan, with the previous round's shared-facet links (the thread kernels of
-                // group_lanes 1 leave none)
+                // group_lanes 1 leave none) and pending list
                 const int fixb = (!c.reuse && c.fix && t->group != 1) ? std::max(1, std::min(1024, est_hi / 64 + 1)) : 0;
-                if (D == 3) hipLaunchKernelGGL(k_plan_one<3>, dim3(1 + fixb), dim3(256), 0, t->s, t->plan, c, t->cnt);
-                else hipLaunchKernelGGL(k_plan_one<2>, dim3(1 + fixb), dim3(256), 0, t->s, t->plan, c, t->cnt);
+                if (D == 3)
+                    hipLaunchKernelGGL(k_plan_one<3>, dim3(1 + fixb), dim3(256), 0, t->s, t->plan, c, t->cnt, t->pend[cur]);
+                else
+                    hipLaunchKernelGGL(k_plan_one<2>, dim3(1 + fixb), dim3(256), 0, t->s, t->plan, c, t->cnt, t->pend[cur]);
                 hipEvent_t* e = &t->rev[(size_t)(slot * R + r) * 3];
                 if (t->round_timing) VO_HIP(hipEventRecord(e[0], t->s));
                 if ((rc = launch_round(t, c, est_hi, 0, 0, Gc))) return rc;
@@ -1283,7 +1348,7 @@ int run_device_planned(vo_tri* t) {
         // the newer chunk ends with no-op rounds; its snapshot is the planner's final state
         VO_HIP(hipStreamSynchronize(t->s));
         add_chunk_times(t, (k - 1) & 1, R);
-        if ((rc = absorb_halted_plan(t, t->plan_host[(k - 1) & 1]))) return rc;
+        if ((rc = absorb_halted_plan(t, t->plan_host[(k - 1) & 1], flags))) return rc;
     }
     return 0;
 }
@@ -1595,7 +1660,8 @@ int vo_tri_create(int hip_device, int dim, const double* pts, int64_t n, vo_tri*
         (rc = dev_alloc(&t->cache, (size_t)t->cmask + 1)) || (rc = dev_alloc(&t->ws, (size_t)t->ws_threads * t->ws_cap)) ||
         (rc = dev_alloc(&t->big, t->win_cap)) || (rc = dev_alloc(&t->cav1, t->big_cap)) ||
         (rc = dev_alloc(&t->fac1, t->big_cap)) || (rc = dev_alloc(&t->misc1, 4)) ||
-        (rc = dev_alloc(&t->fix, (size_t)2 * t->win_cap)) || (rc = dev_alloc(&t->ast, kAllocStripes)))
+        (rc = dev_alloc(&t->fix, (size_t)2 * t->win_cap)) || (rc = dev_alloc(&t->ast, kAllocStripes)) ||
+        (rc = dev_alloc(&t->pflag, t->win_cap)))
         return bail(rc);
     t->fix_cap = 2 * t->win_cap;
     if (hipHostMalloc((void**)&t->st_host, sizeof(RoundState)) != hipSuccess) return bail(fail(VO_ERR_OOM, "pinned"));
@@ -2211,6 +2277,9 @@ int vo_tri_config(vo_tri* t, const char* key, int64_t value) {
     } else if (k == "big_frac_lo_milli" || k == "big_frac_hi_milli") {
         if (value <= 0 || value >= 1000) return fail(VO_ERR_INVALID, k + " must be in 1..999");
         (k == "big_frac_lo_milli" ? t->big_frac_lo : t->big_frac_hi) = (double)value / 1000.0;
+    } else if (k == "outcome_flags") {
+        if (value != 0 && value != 1) return fail(VO_ERR_INVALID, "outcome_flags must be 0 or 1");
+        t->outcome_flags = (int)value;
     } else if (k == "xcd_regions") {
         if (value != 0 && value != 1) return fail(VO_ERR_INVALID, "xcd_regions must be 0 or 1");
         t->xcd = (int)value;
