@@ -293,14 +293,11 @@ __device__ __forceinline__ void cavity_pass(const Ctx c, int nwin, unsigned long
             hull = nb < 0;
             if (!hull) {
                 n = nb >> 2;
-                // the candidate's record is requested before the membership scan of the killed
-                // list, so that the scan (LDS) runs under the load; a member's record, loaded
-                // when it was tested, is in the caches
-                F an[4];
-                int4 na;
-                ld_ax<D>(c, n, na, an);
                 test = !lds_contains(s_cav[g.gid], nk, n);
                 if (test) {
+                    F an[4];
+                    int4 na;
+                    ld_ax<D>(c, n, na, an);
                     s_na[g.lane] = na;
                     s = conflict_at<D>(c, an, n, pv, pf, lc);
                 }
