@@ -273,6 +273,35 @@ def test_device_planned_rounds_match_host_planned(dim):
     assert all(x[5] == seen[0][5] for x in seen if x[5] is not None), seen
 
 
+@pytest.mark.parametrize("dim", [3, 2])
+def test_allocation_chunks_and_outcome_flags_equal_the_oracle(dim):
+    # simplex records from one counter or from the planner's per-round chunks, and the next
+    # window's points appended through one counter or written per slot and compacted by the
+    # planning launch: the exact triangulation either way; the chunks' unused tails stay
+    # fresh records (never live, never exported), so records exceed the created simplices only
+    # with the chunks on
+    P = datasets.uniform(60000, dim, seed=170 + dim)
+    ref = O.Tree.delaunay(P, O.MODE_EXACT).canonical()
+    for stripes, flags in ((0, 0), (0, 1), (1, 0), (1, 1)):
+        print(f"case dim={dim} alloc_stripes={stripes} outcome_flags={flags}", flush=True)
+        t = vo.DelaunayTree(P)
+        t.config("alloc_stripes", stripes)
+        t.config("outcome_flags", flags)
+        t.add_points_to_tree(P, seed=12)
+        T, E = t.canonical()
+        assert np.array_equal(T, ref.T) and np.array_equal(E, ref.E), (dim, stripes, flags)
+        st = t.stats()
+        assert t.violations() == 0
+        # records: the root, then one per created simplex, plus the chunks' unused tails (each
+        # planned round reserves 64 chunks of at least 16 records)
+        extra = st["records"] - st["created"] - 1
+        assert (extra > 64 * 16) if stripes else (0 <= extra < 1024), (dim, stripes, flags, st["records"], st["created"])
+    t = vo.DelaunayTree(P[:100])
+    for key, bad in (("alloc_stripes", 2), ("outcome_flags", -1), ("boot_window", 70000), ("no_such_knob", 1)):
+        with pytest.raises(vo.VoError):
+            t.config(key, bad)
+
+
 def test_device_planned_rounds_hand_over_exact_and_solo_rounds():
     # points rounded onto a sphere / circle: rounds queue exact predicates and some cavities
     # outgrow the per-point LDS lists (solo runs); the planner halts on each such round and the
