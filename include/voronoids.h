@@ -293,7 +293,9 @@ int vo_engine_stats_get(const vo_engine* e, vo_engine_stats* out);
  * and round, default 8), "window_cap" (points per round, up to the allocated 2^20/2^21),
  * "tets_per_point_milli" (first simplex range per point x 1000, default 30000 (3D) /
  * 7000 (2D)), "window_gamma_milli" (initial window per inserted point x 1000), "filters",
- * "abort_early" and "big_window" (as vo_tri_config) */
+ * "abort_early" and "big_window" (as vo_tri_config), "outcome_flags" (1 default / 0: as vo_tri_config;
+ * the lane-group commit writes per-slot outcomes and a kernel after it appends them to the pending
+ * list; measured 4,096 3D sets: replays 588 -> 576 ms, 2D: 122 -> 115 ms) */
 int vo_engine_config(vo_engine* e, const char* key, int64_t value);
 /* batched mode: n_sets sets, set k = points set_offsets[k] .. set_offsets[k+1]-1 of pts
  * (row-major, dim columns), seed of set k = seed + k; runs to completion and writes results[k]

@@ -203,6 +203,14 @@ __global__ void __launch_bounds__(256) k_fill(const BatchCtl* ctl, SetRec* sets,
     }
 }
 
+// the points a captured round's lane-group commit sent on (its outcome flags), appended to the
+// round's pending list after the part k_fill carried over: one atomic per 1,024 window slots
+// instead of one per commit wavefront on the list's counter
+__global__ void __launch_bounds__(256) k_append_pending(Ctx c) {
+    append_flagged(c.pflag, c.ctl->nwin, c.pend_next, c.pend_cap, &c.st->n_next_pending, &c.st->err, blockIdx.x,
+                   gridDim.x);
+}
+
 // completed sets (every point inserted or dropped as a duplicate) leave the rounds
 __global__ void k_collect(BatchCtl* ctl, SetRec* sets, int nslots, DoneRec* done) {
     nslots = min(nslots, ctl->slots);
@@ -515,6 +523,8 @@ struct vo_engine {
     int walk_budget = 0;               // as vo_tri_config "walk_budget"
     int reuse = 0;                     // as vo_tri_config "reuse" (0 or 1 here)
     int2* fix = nullptr;               // shared facets of a round's winners (k_fix_shared)
+    int outcome_flags = 1;             // as vo_tri_config "outcome_flags": lane-group commits write
+    int* pflag = nullptr;              // per-slot outcomes (win_cap), appended by k_append_pending
     int fix_cap = 0;
     float gamma0 = 0.2f;
     bool launched = false;
@@ -632,7 +642,7 @@ void engine_free(vo_engine* e) {
     void* ptrs[] = {e->a.X64, e->a.X32, e->a.cp, e->a.vmap, e->a.pset, e->a.rec, e->a.hint, e->a.sets,
                     e->win, e->pend[0], e->pend[1], e->cav, e->fac, e->fnb, e->nk, e->nf, e->status, e->big, e->cav1,
                     e->fac1, e->misc1, e->st, e->ctl, e->cnt, e->q, e->cache, e->ws, e->done, e->stage_d, e->out_d,
-                    e->fix};
+                    e->fix, e->pflag};
     for (void* p : ptrs)
         if (p) hipFree(p);
     if (e->st_host) hipHostFree(e->st_host);
@@ -797,12 +807,16 @@ int get_graph(vo_engine* e, int G, hipGraphExec_t* out) {
         Ctx c = engine_ctx(e);
         int* src = e->pend[r & 1];
         c.pend_next = e->pend[(r + 1) & 1];
+        if (G != 1 && e->outcome_flags) c.pflag = e->pflag;
         hipLaunchKernelGGL(k_plan, dim3(1), dim3(kPlanThreads), 0, e->s, e->ctl, e->a.sets, e->nslots, e->st,
                            e->win_cap, e->win_limit, e->win_cap + e->big_cap, e->cnt);
         hipLaunchKernelGGL(k_fill, dim3(2048), dim3(256), 0, e->s, e->ctl, e->a.sets, e->nslots, src, c.pend_next,
                            e->win);
         launch_round_kernels(e, c, G, 0);
         launch_round_kernels(e, c, G, 1);
+        if (c.pflag)
+            hipLaunchKernelGGL(k_append_pending, dim3(std::max(1, std::min(1024, (e->win_cap + 1023) / 1024))), dim3(256), 0,
+                               e->s, c);
         if (e->D == 3)
             hipLaunchKernelGGL(k_exact<4>, dim3(exact_blocks), dim3(64), 0, e->s, e->q, -1, e->a.X64, e->ws, e->ws_cap,
                                e->cache, e->cmask, e->st, e->cnt, e->qcap);
@@ -1286,7 +1300,8 @@ int vo_engine_create(int hip_device, int dim, int64_t max_points, int64_t max_si
         (rc = dev_alloc(&E->misc1, 4)) || (rc = dev_alloc(&E->st, 1)) || (rc = dev_alloc(&E->ctl, 1)) ||
         (rc = dev_alloc(&E->cnt, kCntStripes)) || (rc = dev_alloc(&E->q, E->qcap)) ||
         (rc = dev_alloc(&E->cache, (size_t)E->cmask + 1)) || (rc = dev_alloc(&E->ws, (size_t)E->ws_threads * E->ws_cap)) ||
-        (rc = dev_alloc(&E->done, E->nslots)) || (rc = dev_alloc(&E->fix, (size_t)2 * E->win_cap)))
+        (rc = dev_alloc(&E->done, E->nslots)) || (rc = dev_alloc(&E->fix, (size_t)2 * E->win_cap)) ||
+        (rc = dev_alloc(&E->pflag, E->win_cap)))
         return bail(rc);
     E->fix_cap = 2 * E->win_cap;
     E->out_cap = (size_t)1 << 26;
@@ -1375,6 +1390,10 @@ int vo_engine_config(vo_engine* e, const char* key, int64_t value) {
     } else if (k == "abort_early") {
         if (value != 0 && value != 1) return fail(VO_ERR_INVALID, "abort_early must be 0 or 1");
         e->abort_early = (int)value;
+        drop_graphs(e);
+    } else if (k == "outcome_flags") {
+        if (value != 0 && value != 1) return fail(VO_ERR_INVALID, "outcome_flags must be 0 or 1");
+        e->outcome_flags = (int)value;
         drop_graphs(e);
     } else if (k == "walk_budget") {
         if (value < 0 || value > (1 << 20)) return fail(VO_ERR_INVALID, "walk_budget must be in 0..2^20");

@@ -150,10 +150,10 @@ struct Ctx {
     int hlevels;           // finest level
     RoundState* st;
     AllocStripe* ast = nullptr;   // one set, device-planned rounds: the round's allocation chunks
-    int* pflag = nullptr;  // one set, device-planned rounds: the lane-group commit writes each window
-                           // slot's outcome here (its point if it goes on to the next window, else
-                           // -1) instead of appending to pend_next; the next planning launch
-                           // compacts it (compact_pending) and the statistics give the count
+    int* pflag = nullptr;  // device-planned rounds (one set) and the engine's captured rounds: the
+                           // lane-group commit writes each window slot's outcome here (its point if
+                           // it goes on to the next window, else -1) instead of appending to
+                           // pend_next through one counter; append_flagged compacts it later
     Counters* cnt;
     QEntry* q;
     int qcap;
@@ -471,6 +471,42 @@ __device__ __forceinline__ int wave_excl_scan(int v, int& total) {
     }
     total = __shfl(x, 63, 64);
     return x - v;
+}
+
+// The points of a round's outcome flags (Ctx::pflag, slots 0 .. n-1; -1: nothing) appended to
+// dst at positions taken from *counter: block b of nb handles 1,024 slots per step with one
+// atomic per step (256 threads, every thread of the block calls this).  The list's order is
+// free: the next window holds the same points either way.
+__device__ inline void append_flagged(const int* flags, int n, int* dst, int cap, int* counter, unsigned* err, int b,
+                                      int nb) {
+    __shared__ int s_w[4], s_base;
+    const int wv = threadIdx.x >> 6, lane = (int)__lane_id();
+    for (int b0 = b * 1024; b0 < n; b0 += nb * 1024) {
+        int v[4], k = 0;
+        for (int j = 0; j < 4; ++j) {
+            const int i = b0 + j * 256 + threadIdx.x;
+            v[j] = i < n ? flags[i] : -1;
+            k += v[j] >= 0;
+        }
+        int tot = 0;
+        const int off = wave_excl_scan(k, tot);
+        if (lane == 0) s_w[wv] = tot;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            const int all = s_w[0] + s_w[1] + s_w[2] + s_w[3];
+            s_base = all ? atomicAdd(counter, all) : 0;
+        }
+        __syncthreads();
+        int pos = s_base + off;
+        for (int w = 0; w < wv; ++w) pos += s_w[w];
+        for (int j = 0; j < 4; ++j) {
+            if (v[j] < 0) continue;
+            if (pos < cap) dst[pos] = v[j];
+            else atomicOr(err, ERR_PENDING);
+            ++pos;
+        }
+        __syncthreads();                   // s_w and s_base are rewritten by the next step
+    }
 }
 
 // Simplex slots for this lane's request of n (req false: none): the base slot, or -1 when

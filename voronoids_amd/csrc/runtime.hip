@@ -379,41 +379,6 @@ __global__ void __launch_bounds__(kCntStripes) k_round_totals(const Counters* cn
 
 __device__ void plan_step(RunPlan& p, RoundState* st, bool absorb);
 
-// The last round's outcome flags (Ctx::pflag, st->flag_n slots) into the pending list dst, by the
-// blocks 1.. of a planning launch: 1024 slots per step and block, one atomic per step for the
-// block's range of dst (order within the list is free: the next window's points are the same).
-__device__ void compact_pending(const Ctx& c, int* dst) {
-    __shared__ int s_w[4], s_base;
-    const int n = c.st->flag_n;
-    const int wv = threadIdx.x >> 6, lane = (int)__lane_id();
-    for (int b0 = (blockIdx.x - 1) * 1024; b0 < n; b0 += (gridDim.x - 1) * 1024) {
-        int v[4], k = 0;
-        for (int j = 0; j < 4; ++j) {
-            const int i = b0 + j * 256 + threadIdx.x;
-            v[j] = i < n ? c.pflag[i] : -1;
-            k += v[j] >= 0;
-        }
-        int tot = 0;
-        const int off = wave_excl_scan(k, tot);
-        if (lane == 0) s_w[wv] = tot;
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            const int all = s_w[0] + s_w[1] + s_w[2] + s_w[3];
-            s_base = all ? atomicAdd(&c.st->n_compact, all) : 0;
-        }
-        __syncthreads();
-        int pos = s_base + off;
-        for (int w = 0; w < wv; ++w) pos += s_w[w];
-        for (int j = 0; j < 4; ++j) {
-            if (v[j] < 0) continue;
-            if (pos < c.pend_cap) dst[pos] = v[j];
-            else atomicOr(&c.st->err, ERR_PENDING);
-            ++pos;
-        }
-        __syncthreads();                   // s_w and s_base are rewritten by the next step
-    }
-}
-
 // Block 0 plans the next round; blocks 1.. link the facets the last round's winners share (the
 // shared-facet list, as k_fix_shared) and compact its outcome flags into pend_dst, the pending
 // list of the next window (with Ctx::pflag).  The planner reads neither: one launch for all.
@@ -422,7 +387,10 @@ template <int D>
 __global__ void __launch_bounds__(256) k_plan_one(RunPlan* P, Ctx c, const Counters* cnt, int* pend_dst) {
     if (blockIdx.x > 0) {
         if (!c.reuse && c.fix) fix_shared_pass<D>(c, (blockIdx.x - 1) * blockDim.x + threadIdx.x, (gridDim.x - 1) * blockDim.x);
-        if (c.pflag) compact_pending(c, pend_dst);
+        // the last round's outcome flags (st->flag_n slots) into the next window's pending list
+        if (c.pflag)
+            append_flagged(c.pflag, c.st->flag_n, pend_dst, c.pend_cap, &c.st->n_compact, &c.st->err, blockIdx.x - 1,
+                           gridDim.x - 1);
         return;
     }
     RoundState* st = c.st;
