@@ -77,8 +77,10 @@ def test_batch_small_arena_waits_and_extends():
 def test_batch_window_cap_and_group_sizes():
     sizes = log_uniform_sizes(24, 200, 5000, seed=30)
     sets = make_sets(sizes, 3, seed=31)
+    # (window_cap 256 carries pending points over a round while the commit's outcome flags are
+    # appended after them; outcome_flags 0 appends through the list's counter instead)
     for cfg in ({"window_cap": 256}, {"group_lanes": 1}, {"group_lanes": 8, "rounds_per_replay": 2},
-                {"group_lanes": 32, "min_window": 64}):
+                {"group_lanes": 32, "min_window": 64}, {"outcome_flags": 0}, {"outcome_flags": 0, "window_cap": 256}):
         eng = vo.Engine(3, max_points=1 << 18, **cfg)
         res = eng.batch(sets, seed=13)
         eng.close()
