@@ -178,7 +178,8 @@ int vo_tri_stats(const vo_tri* t, vo_stats* out);
  * (same triangulation; measured 3D n = 4M 149 -> 117 ms and 2D n = 10M 99 -> 74 ms with 1).
  * "reuse" (-1 default = auto / 0 / 1): new simplices take their winner's killed records first, so that
  * records stay about as many as live simplices (append-only: about 4x in 3D); auto turns it on only when
- * the append-only records of a run would pass 2^29, since it is slower (3D n = 4M +5%, 2D n = 10M +15%:
+ * the append-only records of a run would pass 2^29 (with the default allocation chunks: from about
+ * 11.6 M points in 3D, 38 M in 2D), since it is slower (3D n = 4M +5%, 2D n = 10M +15%:
  * a winner's new simplices then scatter over its killed records' lines); with reuse the rounds keep
  * winners' cavities non-adjacent.
  * "alloc_stripes" (1 default / 0): device-planned rounds without reuse take new simplex records from 64

@@ -946,9 +946,11 @@ int64_t window_records(int D) { return D == 3 ? 64 : 16; }
 // simplices scatter over its killed records' lines instead of filling fresh consecutive ones:
 // measured 3D n = 4M 147 -> 154 ms, 2D n = 10M 99 -> 114 ms) and needs the stricter round rule.
 void choose_reuse(vo_tri* t, int64_t to_insert) {
-    // append-only growth per point: about 27 (3D) / 6 (2D) live simplices, and with allocation
-    // stripes the unused chunk tails (measured up to 39 / 8.5 records per point in all)
-    const int64_t per = t->alloc_stripes ? (t->D == 3 ? 40 : 9) : (t->D == 3 ? 30 : 8);
+    // append-only growth per point: about 27 (3D) / 6 (2D) live simplices, with allocation
+    // stripes the unused chunk tails (measured up to 39 / 8.5 records per point in all), and
+    // the reservation of the run's last rounds (64 / 16 records per window point, plus their
+    // chunks: about 5 per input point), which must fit below the cap as well
+    const int64_t per = t->alloc_stripes ? (t->D == 3 ? 46 : 14) : (t->D == 3 ? 34 : 10);
     const int64_t append_only = (int64_t)t->n_tets + per * to_insert + 4096;
     t->reuse_on = t->reuse == 1 || (t->reuse < 0 && append_only > ((int64_t)1 << 29));
 }
