@@ -57,7 +57,8 @@ struct alignas(256) Counters {
 // striped statistics after the round.
 struct RoundState {
     alignas(256) int n_tets;          // simplices allocated (append-only high-water mark; one set only)
-    alignas(256) int n_next_pending;  // losers and deferred points appended for the next window
+    alignas(256) int n_next_pending;  // losers and deferred points appended for the next window (a
+                                      // round with outcome flags: from the statistics, round_totals)
     alignas(256) int n_q;             // exact-queue fill (may exceed capacity: excess dropped)
     alignas(256) int n_fix;           // shared-facet links to repair after the commit (see Ctx::fix)
     unsigned int err;                 // sticky ERR_* bits
@@ -69,7 +70,7 @@ struct RoundState {
     long long acc_live;
     unsigned long long acc_pend;
     int flag_n;                       // window slots of the last round whose outcomes are in Ctx::pflag
-    alignas(256) int n_compact;       // pending points placed by the compaction of Ctx::pflag
+    alignas(256) int n_compact;       // pending points the planning launch placed from Ctx::pflag
 };
 
 // Several independent point sets sharing one device arena (batched and served modes).
